@@ -1,0 +1,257 @@
+"""ctypes binding of libcmmdft_hip.so (C ABI declared in include/cmmdft_hip.h).
+
+The shared library is built in-tree (``__graft_entry__.build()`` or
+``cmmdft_amd/csrc/build.sh``).  There is no fallback: if the library or a GPU
+is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libcmmdft_hip.so')
+
+CMM_OK, CMM_E_ARG, CMM_E_HIP, CMM_E_NONFINITE, CMM_E_UNSUPPORTED = 0, 1, 2, 3, 4
+NSCAL = 32
+MMAX = 8
+
+# scalar slots per instance (see cmm_kernels.h)
+S_PHI, S_FEX, S_N3MAX = 0, 1, 2
+S_NCUR, S_FID, S_EXT = 3, 4, 5
+S_PIC_NNEW, S_PIC_IUE, S_PIC_DMAX, S_PIC_NONFIN = 6, 7, 8, 9
+S_AND_NONFIN, S_GRAM = 6, 7
+S_PN, S_PIUE, S_PDMAX, S_PNONFIN, S_PFID, S_PEXT = 16, 17, 18, 19, 20, 21
+
+
+class CmmFunctional(C.Structure):
+    _fields_ = [
+        ('hs_base', C.c_int), ('hs_antisym', C.c_int), ('use_wda', C.c_int), ('n_eos', C.c_int),
+        ('eos_kind', C.c_int * 4), ('eos_factor', C.c_double * 4), ('eos_par', (C.c_double * 16) * 4),
+        ('R', C.c_double), ('D', C.c_double), ('kT', C.c_double),
+    ]
+
+
+EXPORTS = {
+    'cmm_version': ([], C.c_int),
+    'cmm_last_error': ([], C.c_char_p),
+    'cmm_plan_create': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    'cmm_plan_destroy': ([C.c_void_p], C.c_int),
+    'cmm_plan_set_stream': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_plan_set_tiles': ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    'cmm_plan_get_info': ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    'cmm_plan_set_active': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_set_functional': ([C.c_void_p, C.POINTER(CmmFunctional)], C.c_int),
+    'cmm_set_instances': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_fft_r2c': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_fft_c2r': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_seed': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double], C.c_int),
+    'cmm_derive': ([C.c_void_p], C.c_int),
+    'cmm_value': ([C.c_void_p], C.c_int),
+    'cmm_weighted_densities': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_finish_c1': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_double], C.c_int),
+    'cmm_finish_picard': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
+                           C.c_double, C.c_double], C.c_int),
+    'cmm_finish_anderson': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                             C.c_void_p, C.c_double, C.c_double], C.c_int),
+    'cmm_anderson_init': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_anderson_coef': ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                           C.c_double, C.c_double, C.c_double], C.c_int),
+    'cmm_anderson_mix': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                          C.c_void_p, C.c_void_p, C.c_int, C.c_double], C.c_int),
+    'cmm_anderson_state': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_trial': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                   C.c_int], C.c_int),
+    'cmm_get_scalars': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_synchronize': ([C.c_void_p], C.c_int),
+    'cmm_lj_grid': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                     C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_lj_points': ([C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                       C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (once) and declare every exported symbol."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError('libcmmdft_hip.so is not built (%s); run __graft_entry__.build()' % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    for name, (args, res) in EXPORTS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status == CMM_OK:
+        return
+    msg = load().cmm_last_error().decode()
+    if status == CMM_E_ARG:
+        raise ValueError(msg)
+    if status == CMM_E_NONFINITE:
+        raise FloatingPointError(msg)
+    if status == CMM_E_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _ints(vals):
+    a = (C.c_int * max(1, len(vals)))(*vals)
+    return a
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError('the HIP path needs a GPU (torch.cuda.is_available() is False)')
+    return torch
+
+
+class Plan:
+    """A grid plan for ``nbatch`` instances (owner of the device workspace)."""
+
+    def __init__(self, npoints, rvecs, nbatch=1):
+        torch = _torch()
+        self.lib = load()
+        self.npoints = tuple(int(n) for n in npoints)
+        self.nbatch = int(nbatch)
+        self.rvecs = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(3, 3))
+        h = C.c_void_p()
+        check(self.lib.cmm_plan_create(self.npoints[0], self.npoints[1], self.npoints[2],
+                                       self.rvecs.ctypes.data_as(C.c_void_p), self.nbatch, C.byref(h)))
+        self.h = h
+        self.device = torch.device('cuda', torch.cuda.current_device())
+        self._scal = np.zeros(self.nbatch * NSCAL)
+        self.bind_stream()
+
+    def bind_stream(self, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.stream = s
+        check(self.lib.cmm_plan_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def __del__(self):
+        try:
+            if getattr(self, 'h', None):
+                self.lib.cmm_plan_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def info(self):
+        out = np.zeros(10, dtype=np.int64)
+        check(self.lib.cmm_plan_get_info(self.h, out.ctypes.data_as(C.c_void_p), 10))
+        return dict(zip(['nx', 'ny', 'nz', 'nbatch', 'tb_x', 'tb_y', 'p_z', 'p_phi', 'workspace_bytes', 'nzh'],
+                        [int(v) for v in out]))
+
+    def set_tiles(self, tb_x=0, tb_y=0, p_z=0, p_phi=0):
+        check(self.lib.cmm_plan_set_tiles(self.h, tb_x, tb_y, p_z, p_phi))
+
+    def set_active(self, mask):
+        self._active = mask
+        check(self.lib.cmm_plan_set_active(self.h, _ptr(mask)))
+
+    def set_functional(self, f: CmmFunctional):
+        self._func = f
+        check(self.lib.cmm_set_functional(self.h, C.byref(f)))
+
+    def set_instances(self, fug, alpha):
+        fug = np.ascontiguousarray(np.broadcast_to(np.asarray(fug, dtype=np.float64), (self.nbatch,)))
+        alpha = np.ascontiguousarray(np.broadcast_to(np.asarray(alpha, dtype=np.float64), (self.nbatch,)))
+        check(self.lib.cmm_set_instances(self.h, fug.ctypes.data_as(C.c_void_p), alpha.ctypes.data_as(C.c_void_p)))
+
+    # ---- transforms -------------------------------------------------------
+    def fft_r2c(self, x, X):
+        check(self.lib.cmm_fft_r2c(self.h, _ptr(x), _ptr(X)))
+
+    def fft_c2r(self, X, x):
+        check(self.lib.cmm_fft_c2r(self.h, _ptr(X), _ptr(x)))
+
+    # ---- functional chain -------------------------------------------------
+    def seed(self, rho, vext=None, vext_batched=False, lam3=1.0):
+        check(self.lib.cmm_seed(self.h, _ptr(rho), _ptr(vext), int(vext_batched), float(lam3)))
+
+    def derive(self):
+        check(self.lib.cmm_derive(self.h))
+
+    def value(self):
+        check(self.lib.cmm_value(self.h))
+
+    def weighted_densities(self, out):
+        check(self.lib.cmm_weighted_densities(self.h, _ptr(out)))
+
+    def finish_c1(self, rho, vext, vext_batched, c1, lam3=1.0):
+        check(self.lib.cmm_finish_c1(self.h, _ptr(rho), _ptr(vext), int(vext_batched), _ptr(c1), float(lam3)))
+
+    def finish_picard(self, rho, vext, vext_batched, rho_out, c1_out, beta, lam3, zero_thr):
+        check(self.lib.cmm_finish_picard(self.h, _ptr(rho), _ptr(vext), int(vext_batched), _ptr(rho_out),
+                                         _ptr(c1_out), float(beta), float(lam3), float(zero_thr)))
+
+    def finish_anderson(self, rho, vext, vext_batched, rhist, m, slot_new, c1_out, beta, lam3):
+        check(self.lib.cmm_finish_anderson(self.h, _ptr(rho), _ptr(vext), int(vext_batched), _ptr(rhist), int(m),
+                                           int(slot_new), _ptr(c1_out), float(beta), float(lam3)))
+
+    def anderson_init(self, damping, cf):
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(damping, dtype=np.float64), (self.nbatch,)))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(cf, dtype=np.float64), (self.nbatch,)))
+        check(self.lib.cmm_anderson_init(self.h, d.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p)))
+
+    def anderson_coef(self, m, order, slot_new, slot_prev, adaptive, fac_up, fac_down, dmin, dmax):
+        check(self.lib.cmm_anderson_coef(self.h, int(m), len(order), _ints(order), int(slot_new), int(slot_prev),
+                                         int(adaptive), float(fac_up), float(fac_down), float(dmin), float(dmax)))
+
+    def anderson_mix(self, rhohist, order_rho, rhist, order, rho_cur, rho_out, vext, vext_batched, lam3):
+        check(self.lib.cmm_anderson_mix(self.h, _ptr(rhohist), _ints(order_rho), _ptr(rhist), _ints(order),
+                                        len(order), _ptr(rho_cur), _ptr(rho_out), _ptr(vext), int(vext_batched),
+                                        float(lam3)))
+
+    def anderson_state(self):
+        d = np.zeros(self.nbatch)
+        a = np.zeros(self.nbatch * MMAX)
+        check(self.lib.cmm_anderson_state(self.h, d.ctypes.data_as(C.c_void_p), a.ctypes.data_as(C.c_void_p)))
+        return d, a.reshape(self.nbatch, MMAX)
+
+    def trial(self, rho, rhist, slot, rho_out, vext, vext_batched, lam3, clip):
+        check(self.lib.cmm_trial(self.h, _ptr(rho), _ptr(rhist), int(slot), _ptr(rho_out), _ptr(vext),
+                                 int(vext_batched), float(lam3), int(clip)))
+
+    def scalars(self):
+        check(self.lib.cmm_get_scalars(self.h, self._scal.ctypes.data_as(C.c_void_p)))
+        return self._scal.reshape(self.nbatch, NSCAL).copy()
+
+    def synchronize(self):
+        check(self.lib.cmm_synchronize(self.h))
+
+
+def lj_grid(npoints, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff, images, limit, out, stream=None):
+    torch = _torch()
+    lib = load()
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_lj_grid(int(npoints[0]), int(npoints[1]), int(npoints[2]), rv.ctypes.data_as(C.c_void_p),
+                          _ptr(atoms_dev), int(natom), float(sigma_f), float(epsilon_f), float(cutoff), int(images),
+                          float(limit), _ptr(out), C.c_void_p(s.cuda_stream)))
+
+
+def lj_points(points_dev, M, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff, images, limit, out, stream=None):
+    torch = _torch()
+    lib = load()
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_lj_points(_ptr(points_dev), int(M), rv.ctypes.data_as(C.c_void_p), _ptr(atoms_dev), int(natom),
+                            float(sigma_f), float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(out),
+                            C.c_void_p(s.cuda_stream)))

@@ -1,0 +1,632 @@
+// C ABI of libcmmdft_hip.so: plans, launchers and error handling.
+// Declared in include/cmmdft_hip.h.  One translation unit: the kernels are
+// included here so the launch templates are instantiated next to their use.
+#include "cmm_kernels.hip"
+#include "cmm_lj.hip"
+#include "../../include/cmmdft_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace cmm;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t e_ = (expr);                                                                           \
+    if (e_ != hipSuccess) return fail(CMM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK(what)                                                                           \
+  do {                                                                                            \
+    hipError_t e_ = hipGetLastError();                                                            \
+    if (e_ != hipSuccess) return fail(CMM_E_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct cmm_plan {
+  int nx, ny, nz, nzh, nb;
+  double rvecs[9];
+  hipStream_t stream;
+  double2* tw[3];
+  double* ktab[3];
+  double* ltab[3];
+  GridDev g;
+  FuncDev fp;
+  int has_func;
+  double2* rz;
+  double2* F;
+  long long nhalf;   // complex elements per half spectrum
+  long long npts;
+  double* partials;
+  long long pstride;
+  unsigned* counters;
+  double* scal;
+  InstDev* inst;
+  double* gram;
+  double* damping;
+  double* cf;
+  double* coef;
+  double* alpha_out;
+  const int* active;
+  int tbx, tby, pz, pphi;
+  size_t ws_bytes;
+};
+
+namespace {
+
+bool radix_plan(int n, Fft1D& f) {
+  f.n = n;
+  f.nst = 0;
+  int m = n;
+  while (m % 5 == 0) { f.rad[f.nst++] = 5; m /= 5; }
+  while (m % 3 == 0) { f.rad[f.nst++] = 3; m /= 3; }
+  while (m % 8 == 0) { f.rad[f.nst++] = 8; m /= 8; }
+  while (m % 4 == 0) { f.rad[f.nst++] = 4; m /= 4; }
+  while (m % 2 == 0) { f.rad[f.nst++] = 2; m /= 2; }
+  if (m != 1 || f.nst > CMM_MAX_STAGES) return false;
+  if (f.nst == 0) { f.rad[0] = 2; f.nst = 0; }
+  return true;
+}
+
+double sinc_np(double x) {   // numpy.sinc: sin(pi x)/(pi x), 1 at 0
+  if (x == 0.0) return 1.0;
+  const double y = M_PI * x;
+  return std::sin(y) / y;
+}
+
+int largest_even_le(int v) { return std::max(2, v - (v & 1)); }
+
+void default_tiles(cmm_plan* p) {
+  const int budget = VPT * NT;               // complex values a block stages
+  // x pass keeps two tiles (64 KB LDS budget): nx*TB*2*16 <= 65536
+  int tbx = std::min(32, std::min(budget / p->nx, 2048 / p->nx));
+  p->tbx = std::max(1, tbx);
+  int tby = std::min(32, std::min(budget / p->ny, 4096 / p->ny));
+  p->tby = std::max(1, tby);
+  // simple z pass: (P/2)*nz <= budget and P*nz*8 <= 64 KB
+  int pz = std::min(32, std::min(2 * budget / p->nz, 8192 / p->nz));
+  p->pz = largest_even_le(pz);
+  // 6-field z pass: 6*(P/2)*nz <= budget, 6*P*nz*8 <= 64 KB
+  int pphi = std::min(16, std::min(budget / (3 * p->nz), 1365 / p->nz));
+  p->pphi = largest_even_le(pphi);
+}
+
+RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->counters, p->scal}; }
+
+long long zblocks(cmm_plan* p, int P) { return ((long long)p->nx * p->ny + P - 1) / P; }
+
+size_t zlds(cmm_plan* p, int P, int nf) { return (size_t)nf * (P / 2) * p->nz * sizeof(double2); }
+
+}  // namespace
+
+// The exported functions take C linkage from their declarations in cmmdft_hip.h.
+
+int cmm_version(void) { return 10000; }
+
+const char* cmm_last_error(void) { return g_err.c_str(); }
+
+int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm_plan** out) {
+  if (!out || !rvecs) return fail(CMM_E_ARG, "null argument");
+  if (nx < 2 || ny < 2 || nz < 2 || nbatch < 1 || nbatch > 4096) return fail(CMM_E_ARG, "bad grid or batch size");
+  if (nx > 512 || ny > 512 || nz > 512) return fail(CMM_E_ARG, "grid axis longer than 512 is not supported");
+  cmm_plan* p = new cmm_plan();
+  std::memset(p, 0, sizeof(*p));
+  p->nx = nx; p->ny = ny; p->nz = nz; p->nzh = nz / 2 + 1; p->nb = nbatch;
+  std::memcpy(p->rvecs, rvecs, 9 * sizeof(double));
+  p->stream = 0;
+  int ns[3] = {nx, ny, nz};
+  Fft1D* fs[3] = {&p->g.fx, &p->g.fy, &p->g.fz};
+  KAxis* ks[3] = {&p->g.kx, &p->g.ky, &p->g.kz};
+  for (int a = 0; a < 3; ++a) {
+    const int n = ns[a];
+    if (!radix_plan(n, *fs[a])) { delete p; return fail(CMM_E_ARG, "axis length must be 2^a 3^b 5^c"); }
+    std::vector<double2> tw(n);
+    for (int j = 0; j < n; ++j) {
+      long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)j / (long double)n;
+      tw[j] = make_double2((double)cosl(ang), (double)(-sinl(ang)));
+    }
+    // spacing from the column norm (cell vectors are the rows)
+    const double colnorm = std::sqrt(rvecs[0 * 3 + a] * rvecs[0 * 3 + a] + rvecs[1 * 3 + a] * rvecs[1 * 3 + a] +
+                                     rvecs[2 * 3 + a] * rvecs[2 * 3 + a]);
+    const double h = colnorm / n;
+    const double val = 1.0 / (n * h);
+    std::vector<double> kt(n), lt(n);
+    const double kcut = 2.0 * M_PI / h;
+    for (int m = 0; m < n; ++m) {
+      const int fm = (m <= (n - 1) / 2) ? m : m - n;
+      kt[m] = 2.0 * M_PI * (fm * val);
+      lt[m] = sinc_np(kt[m] / kcut);
+    }
+    if (hipMalloc(&p->tw[a], n * sizeof(double2)) != hipSuccess || hipMalloc(&p->ktab[a], n * sizeof(double)) != hipSuccess ||
+        hipMalloc(&p->ltab[a], n * sizeof(double)) != hipSuccess) {
+      cmm_plan_destroy(p);
+      return fail(CMM_E_HIP, "hipMalloc tables");
+    }
+    (void)hipMemcpy(p->tw[a], tw.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+    (void)hipMemcpy(p->ktab[a], kt.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    (void)hipMemcpy(p->ltab[a], lt.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    fs[a]->tw = p->tw[a];
+    ks[a]->k = p->ktab[a];
+    ks[a]->lanczos = p->ltab[a];
+    ks[a]->nyq = (n % 2 == 0) ? n / 2 : -1;
+  }
+  p->g.nx = nx; p->g.ny = ny; p->g.nz = nz; p->g.nzh = p->nzh;
+  p->npts = (long long)nx * ny * nz;
+  p->nhalf = (long long)nx * ny * p->nzh;
+  p->g.inv_npts = 1.0 / (double)p->npts;
+  default_tiles(p);
+  // workspace: rz (1) + F (6) half spectra per instance
+  size_t half_bytes = (size_t)p->nhalf * sizeof(double2) * nbatch;
+  long long maxblocks = zblocks(p, 2);
+  p->pstride = 16 * maxblocks;
+  hipError_t e = hipSuccess;
+  e = (e == hipSuccess) ? hipMalloc(&p->rz, half_bytes) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->F, 6 * half_bytes) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->partials, (size_t)p->pstride * nbatch * sizeof(double)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->counters, (size_t)RED_SITES * nbatch * sizeof(unsigned)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->scal, (size_t)NSCAL * nbatch * sizeof(double)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->inst, (size_t)nbatch * sizeof(InstDev)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->gram, (size_t)nbatch * CMM_MMAX * CMM_MMAX * sizeof(double)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->damping, (size_t)nbatch * sizeof(double)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->cf, (size_t)nbatch * sizeof(double)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->coef, (size_t)nbatch * 2 * CMM_MMAX * sizeof(double)) : e;
+  e = (e == hipSuccess) ? hipMalloc(&p->alpha_out, (size_t)nbatch * CMM_MMAX * sizeof(double)) : e;
+  if (e != hipSuccess) {
+    cmm_plan_destroy(p);
+    return fail(CMM_E_HIP, std::string("workspace allocation: ") + hipGetErrorString(e));
+  }
+  (void)hipMemset(p->counters, 0, (size_t)RED_SITES * nbatch * sizeof(unsigned));
+  (void)hipMemset(p->scal, 0, (size_t)NSCAL * nbatch * sizeof(double));
+  (void)hipMemset(p->gram, 0, (size_t)nbatch * CMM_MMAX * CMM_MMAX * sizeof(double));
+  (void)hipMemset(p->coef, 0, (size_t)nbatch * 2 * CMM_MMAX * sizeof(double));
+  std::vector<InstDev> iv(nbatch, InstDev{0.0, 0.0});
+  (void)hipMemcpy(p->inst, iv.data(), nbatch * sizeof(InstDev), hipMemcpyHostToDevice);
+  p->ws_bytes = 7 * half_bytes;
+  if (hipDeviceSynchronize() != hipSuccess) {
+    cmm_plan_destroy(p);
+    return fail(CMM_E_HIP, "plan initialisation");
+  }
+  *out = p;
+  return CMM_OK;
+}
+
+int cmm_plan_destroy(cmm_plan* p) {
+  if (!p) return CMM_OK;
+  for (int a = 0; a < 3; ++a) {
+    if (p->tw[a]) (void)hipFree(p->tw[a]);
+    if (p->ktab[a]) (void)hipFree(p->ktab[a]);
+    if (p->ltab[a]) (void)hipFree(p->ltab[a]);
+  }
+  void* bufs[] = {p->rz, p->F, p->partials, p->counters, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef, p->alpha_out};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete p;
+  return CMM_OK;
+}
+
+int cmm_plan_set_stream(cmm_plan* p, void* stream) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  p->stream = (hipStream_t)stream;
+  return CMM_OK;
+}
+
+int cmm_plan_set_tiles(cmm_plan* p, int tbx, int tby, int pz, int pphi) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  const int budget = VPT * NT;
+  if (tbx > 0) {
+    if (tbx * p->nx > budget || tbx * p->nx * 32 > 65536) return fail(CMM_E_ARG, "tb_x too large");
+    p->tbx = tbx;
+  }
+  if (tby > 0) {
+    if (tby * p->ny > budget || tby * p->ny * 16 > 65536) return fail(CMM_E_ARG, "tb_y too large");
+    p->tby = tby;
+  }
+  if (pz > 0) {
+    if ((pz & 1) || (pz / 2) * p->nz > budget || pz * p->nz * 8 > 65536) return fail(CMM_E_ARG, "p_z invalid");
+    p->pz = pz;
+  }
+  if (pphi > 0) {
+    if ((pphi & 1) || 3 * pphi * p->nz > budget || 6 * pphi * p->nz * 8 > 65536) return fail(CMM_E_ARG, "p_phi invalid");
+    p->pphi = pphi;
+  }
+  return CMM_OK;
+}
+
+int cmm_plan_get_info(cmm_plan* p, long long* info, int n) {
+  if (!p || !info) return fail(CMM_E_ARG, "null argument");
+  long long v[10] = {p->nx, p->ny, p->nz, p->nb, p->tbx, p->tby, p->pz, p->pphi, (long long)p->ws_bytes, p->nzh};
+  for (int i = 0; i < n && i < 10; ++i) info[i] = v[i];
+  return CMM_OK;
+}
+
+int cmm_plan_set_active(cmm_plan* p, const int* mask) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  p->active = mask;
+  return CMM_OK;
+}
+
+int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
+  if (!p || !f) return fail(CMM_E_ARG, "null argument");
+  if (f->hs_base < -1 || f->hs_base > 2) return fail(CMM_E_ARG, "hs_base must be -1, 0, 1 or 2");
+  if (f->n_eos < 0 || f->n_eos > 4) return fail(CMM_E_ARG, "n_eos must be 0..4");
+  if (f->hs_base >= 0 && !(f->R > 0.0)) return fail(CMM_E_ARG, "hard-sphere radius must be positive");
+  if (!(f->kT > 0.0)) return fail(CMM_E_ARG, "kT must be positive");
+  FuncDev& d = p->fp;
+  std::memset(&d, 0, sizeof(d));
+  d.use_hs = f->hs_base >= 0;
+  d.base = std::max(0, f->hs_base);
+  d.anti = f->hs_antisym ? 1 : 0;
+  d.use_wda = f->use_wda && f->n_eos > 0;
+  d.neos = f->n_eos;
+  for (int t = 0; t < 4; ++t) {
+    d.eos_kind[t] = f->eos_kind[t];
+    d.eos_fac[t] = f->eos_factor[t];
+    for (int k = 0; k < 16; ++k) d.eos_par[t][k] = f->eos_par[t][k];
+    if (t < f->n_eos && (f->eos_kind[t] < 0 || f->eos_kind[t] > 2)) return fail(CMM_E_ARG, "unknown EOS kind");
+  }
+  d.R = d.use_hs ? f->R : 1.0;
+  d.D = f->D;
+  d.kT = f->kT;
+  d.c_w2 = 4.0 * M_PI * d.R * d.R;
+  d.c_w3 = 4.0 * M_PI * d.R * d.R * d.R / 3.0;
+  d.inv4piR2 = 1.0 / (4.0 * M_PI * d.R * d.R);
+  d.inv4piR = 1.0 / (4.0 * M_PI * d.R);
+  p->has_func = 1;
+  return CMM_OK;
+}
+
+int cmm_set_instances(cmm_plan* p, const double* fug, const double* alpha) {
+  if (!p || !fug || !alpha) return fail(CMM_E_ARG, "null argument");
+  std::vector<InstDev> iv(p->nb);
+  for (int i = 0; i < p->nb; ++i) iv[i] = InstDev{fug[i], alpha[i]};
+  HIPCHK(hipMemcpyAsync(p->inst, iv.data(), p->nb * sizeof(InstDev), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
+// --------------------------------------------------------------------------
+// launch helpers
+// --------------------------------------------------------------------------
+static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
+  const int ntk = (p->nzh + p->tby - 1) / p->tby;
+  dim3 grid(p->nx * ntk, nf, p->nb);
+  size_t lds = (size_t)p->ny * p->tby * sizeof(double2);
+  long long is = (nf == 1 && F == p->rz) ? p->nhalf : 6 * p->nhalf;
+  if (sgn < 0) hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, p->active);
+  else hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, p->active);
+  LAUNCHCHK("k_ypass");
+  return CMM_OK;
+}
+
+static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
+  const long long nb = (long long)p->ny * p->nzh;
+  dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
+  size_t lds = (size_t)p->nx * p->tbx * sizeof(double2);
+  if (sgn < 0) hipLaunchKernelGGL(k_xpass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tbx, p->active);
+  else hipLaunchKernelGGL(k_xpass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tbx, p->active);
+  LAUNCHCHK("k_xpass");
+  return CMM_OK;
+}
+
+int cmm_fft_r2c(cmm_plan* p, const double* x, double* X) {
+  if (!p || !x || !X) return fail(CMM_E_ARG, "null argument");
+  double2* Xc = reinterpret_cast<double2*>(X);
+  dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
+  hipLaunchKernelGGL(k_zr2c, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, x, p->npts, Xc, p->nhalf, p->g, p->pz,
+                     p->active);
+  LAUNCHCHK("k_zr2c");
+  // y and x passes on the caller's buffer (instance stride nhalf)
+  {
+    const int ntk = (p->nzh + p->tby - 1) / p->tby;
+    dim3 gy(p->nx * ntk, 1, p->nb);
+    hipLaunchKernelGGL(k_ypass<-1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, Xc, p->nhalf,
+                       p->nhalf, p->g, p->tby, p->active);
+    LAUNCHCHK("k_ypass");
+  }
+  return launch_xpass(p, Xc, p->nhalf, -1);
+}
+
+int cmm_fft_c2r(cmm_plan* p, const double* X, double* x) {
+  if (!p || !x || !X) return fail(CMM_E_ARG, "null argument");
+  // work on a copy in the workspace (rz) so the caller's spectrum survives
+  HIPCHK(hipMemcpyAsync(p->rz, X, (size_t)p->nhalf * p->nb * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
+  int rc = launch_xpass(p, p->rz, p->nhalf, 1);
+  if (rc) return rc;
+  {
+    const int ntk = (p->nzh + p->tby - 1) / p->tby;
+    dim3 gy(p->nx * ntk, 1, p->nb);
+    hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->rz, p->nhalf,
+                       p->nhalf, p->g, p->tby, p->active);
+    LAUNCHCHK("k_ypass");
+  }
+  dim3 grid((unsigned)zblocks(p, p->pz), 1, p->nb);
+  hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->rz, p->nhalf, 0LL, x, p->npts, 0LL,
+                     p->g, p->pz, p->active);
+  LAUNCHCHK("k_zc2r");
+  return CMM_OK;
+}
+
+static ElDev el_base(cmm_plan* p, const double* rho, const double* vext, int vb, double lam3) {
+  ElDev el;
+  std::memset(&el, 0, sizeof(el));
+  el.rho = rho;
+  el.vext = vext;
+  el.vext_batched = vb;
+  el.fstride = p->npts;
+  el.inst = p->inst;
+  el.lam3 = lam3;
+  el.rz = p->rz;
+  el.rzstride = p->nhalf;
+  return el;
+}
+
+template <int MODE>
+static int launch_produce(cmm_plan* p, const ElDev& el, const char* name) {
+  dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
+  hipLaunchKernelGGL(k_zproduce<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, el, p->pz, p->active,
+                     red_of(p));
+  LAUNCHCHK(name);
+  return CMM_OK;
+}
+
+int cmm_seed(cmm_plan* p, const double* rho, const double* vext, int vb, double lam3) {
+  if (!p || !rho) return fail(CMM_E_ARG, "null argument");
+  ElDev el = el_base(p, rho, vext, vb, lam3);
+  return launch_produce<MODE_SEED>(p, el, "k_zproduce(seed)");
+}
+
+static int forward_chain(cmm_plan* p) {
+  if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
+  int rc = launch_ypass(p, p->rz, 1, -1);
+  if (rc) return rc;
+  const long long nb = (long long)p->ny * p->nzh;
+  dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
+  size_t lds = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
+  hipLaunchKernelGGL(k_xweights, grid, dim3(NT), lds, p->stream, p->rz, p->nhalf, p->F, 6 * p->nhalf, p->nhalf, p->g,
+                     p->fp, p->tbx, p->active);
+  LAUNCHCHK("k_xweights");
+  return launch_ypass(p, p->F, 6, 1);
+}
+
+static int launch_zphi(cmm_plan* p, int store) {
+  dim3 grid((unsigned)zblocks(p, p->pphi), p->nb);
+  hipLaunchKernelGGL(k_zphi, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g,
+                     p->fp, p->pphi, store, p->active, red_of(p));
+  LAUNCHCHK("k_zphi");
+  return CMM_OK;
+}
+
+int cmm_derive(cmm_plan* p) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  int rc = forward_chain(p);
+  if (rc) return rc;
+  rc = launch_zphi(p, 1);
+  if (rc) return rc;
+  rc = launch_ypass(p, p->F, 6, -1);
+  if (rc) return rc;
+  const long long nb = (long long)p->ny * p->nzh;
+  dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
+  size_t lds = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
+  hipLaunchKernelGGL(k_xback, grid, dim3(NT), lds, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
+                     p->active);
+  LAUNCHCHK("k_xback");
+  // inverse y of the sum (field 0 of F)
+  const int ntk = (p->nzh + p->tby - 1) / p->tby;
+  dim3 gy(p->nx * ntk, 1, p->nb);
+  hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->F,
+                     6 * p->nhalf, p->nhalf, p->g, p->tby, p->active);
+  LAUNCHCHK("k_ypass");
+  return CMM_OK;
+}
+
+int cmm_value(cmm_plan* p) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  int rc = forward_chain(p);
+  if (rc) return rc;
+  return launch_zphi(p, 0);
+}
+
+int cmm_weighted_densities(cmm_plan* p, double* out) {
+  if (!p || !out) return fail(CMM_E_ARG, "null argument");
+  int rc = forward_chain(p);
+  if (rc) return rc;
+  dim3 grid((unsigned)zblocks(p, p->pz), 6, p->nb);
+  hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->nhalf, out,
+                     6 * p->npts, p->npts, p->g, p->pz, p->active);
+  LAUNCHCHK("k_zc2r");
+  return CMM_OK;
+}
+
+template <int MODE>
+static int launch_final(cmm_plan* p, const ElDev& el, const char* name) {
+  dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
+  hipLaunchKernelGGL(k_zfinal<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, el, p->pz,
+                     p->active, red_of(p));
+  LAUNCHCHK(name);
+  return CMM_OK;
+}
+
+int cmm_finish_c1(cmm_plan* p, const double* rho, const double* vext, int vb, double* c1, double lam3) {
+  if (!p || !rho || !c1) return fail(CMM_E_ARG, "null argument");
+  ElDev el = el_base(p, rho, vext, vb, lam3);
+  el.c1out = c1;
+  return launch_final<MODE_C1>(p, el, "k_zfinal(c1)");
+}
+
+int cmm_finish_picard(cmm_plan* p, const double* rho, const double* vext, int vb, double* rho_out, double* c1_out,
+                      double beta, double lam3, double zero_thr) {
+  if (!p || !rho || !rho_out) return fail(CMM_E_ARG, "null argument");
+  ElDev el = el_base(p, rho, vext, vb, lam3);
+  el.rho_out = rho_out;
+  el.c1out = c1_out;
+  el.beta = beta;
+  el.zero_thr = zero_thr;
+  return launch_final<MODE_PICARD>(p, el, "k_zfinal(picard)");
+}
+
+int cmm_finish_anderson(cmm_plan* p, const double* rho, const double* vext, int vb, double* rhist, int m, int slot_new,
+                        double* c1_out, double beta, double lam3) {
+  if (!p || !rho || !rhist) return fail(CMM_E_ARG, "null argument");
+  if (m < 1 || m > CMM_MMAX || slot_new < 0 || slot_new >= m) return fail(CMM_E_ARG, "bad history slot");
+  ElDev el = el_base(p, rho, vext, vb, lam3);
+  el.rhist = rhist;
+  el.hstride = (long long)p->nb * p->npts;
+  el.m = m;
+  el.slot_new = slot_new;
+  el.c1out = c1_out;
+  el.beta = beta;
+  return launch_final<MODE_ANDRES>(p, el, "k_zfinal(anderson)");
+}
+
+int cmm_anderson_init(cmm_plan* p, const double* damping, const double* cf) {
+  if (!p || !damping || !cf) return fail(CMM_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(p->damping, damping, p->nb * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemcpyAsync(p->cf, cf, p->nb * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemsetAsync(p->gram, 0, (size_t)p->nb * CMM_MMAX * CMM_MMAX * sizeof(double), p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
+int cmm_anderson_coef(cmm_plan* p, int m, int mk, const int* order, int slot_new, int slot_prev, int adaptive,
+                      double fac_up, double fac_down, double dmin, double dmax) {
+  if (!p || !order) return fail(CMM_E_ARG, "null argument");
+  if (m < 1 || m > CMM_MMAX || mk < 1 || mk > m) return fail(CMM_E_ARG, "bad history length");
+  AndDev a;
+  std::memset(&a, 0, sizeof(a));
+  a.gram = p->gram;
+  a.m = m;
+  a.mk = mk;
+  for (int i = 0; i < mk; ++i) {
+    if (order[i] < 0 || order[i] >= m) return fail(CMM_E_ARG, "bad history order");
+    a.order[i] = order[i];
+  }
+  a.slot_new = slot_new;
+  a.slot_prev = slot_prev;
+  a.damping = p->damping;
+  a.cf = p->cf;
+  a.adaptive = adaptive;
+  a.fac_up = fac_up;
+  a.fac_down = fac_down;
+  a.dmin = dmin;
+  a.dmax = dmax;
+  a.coef = p->coef;
+  a.alpha_out = p->alpha_out;
+  hipLaunchKernelGGL(k_anderson_coef, dim3(p->nb), dim3(64), 0, p->stream, a, p->active, red_of(p));
+  LAUNCHCHK("k_anderson_coef");
+  return CMM_OK;
+}
+
+int cmm_anderson_mix(cmm_plan* p, const double* rhohist, const int* order_rho, const double* rhist, const int* order,
+                     int mk, const double* rho_cur, double* rho_out, const double* vext, int vb, double lam3) {
+  if (!p || !rhohist || !rhist || !order || !order_rho || !rho_cur || !rho_out) return fail(CMM_E_ARG, "null argument");
+  if (mk < 1 || mk > CMM_MMAX) return fail(CMM_E_ARG, "bad history length");
+  ElDev el = el_base(p, rho_cur, vext, vb, lam3);
+  el.rho_out = rho_out;
+  el.rhohist = rhohist;
+  el.hstride2 = (long long)p->nb * p->npts;
+  el.rhist = const_cast<double*>(rhist);
+  el.hstride = (long long)p->nb * p->npts;
+  el.coef = p->coef;
+  el.mk = mk;
+  for (int i = 0; i < mk; ++i) {
+    el.order[i] = order[i];
+    el.order_rho[i] = order_rho[i];
+  }
+  return launch_produce<MODE_MIX>(p, el, "k_zproduce(mix)");
+}
+
+int cmm_anderson_state(cmm_plan* p, double* damping, double* alpha) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (damping) HIPCHK(hipMemcpyAsync(damping, p->damping, p->nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  if (alpha)
+    HIPCHK(hipMemcpyAsync(alpha, p->alpha_out, p->nb * CMM_MMAX * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
+int cmm_trial(cmm_plan* p, const double* rho, const double* rhist, int slot, double* rho_out, const double* vext, int vb,
+              double lam3, int clip) {
+  if (!p || !rho || !rhist || !rho_out) return fail(CMM_E_ARG, "null argument");
+  ElDev el = el_base(p, rho, vext, vb, lam3);
+  el.rho_out = rho_out;
+  el.rhist = const_cast<double*>(rhist);
+  el.hstride = (long long)p->nb * p->npts;
+  el.slot_new = slot;
+  el.clip_trial = clip;
+  return launch_produce<MODE_TRIAL>(p, el, "k_zproduce(trial)");
+}
+
+int cmm_get_scalars(cmm_plan* p, double* out) {
+  if (!p || !out) return fail(CMM_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(out, p->scal, (size_t)NSCAL * p->nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
+int cmm_synchronize(cmm_plan* p) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
+static int lj_setup(int nx, int ny, int nz, const double* rvecs, double sigma_f, double epsilon_f, double cutoff,
+                    int images, double limit, LJDev& d) {
+  if (!rvecs) return fail(CMM_E_ARG, "null argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(CMM_E_ARG, "bad sizes");
+  if (!(cutoff > 0.0)) return fail(CMM_E_ARG, "cutoff must be positive");
+  std::memset(&d, 0, sizeof(d));
+  d.nx = nx; d.ny = ny; d.nz = nz;
+  std::memcpy(d.rv, rvecs, 9 * sizeof(double));
+  for (int a = 0; a < 3; ++a) {
+    d.L[a] = std::sqrt(rvecs[3 * a] * rvecs[3 * a] + rvecs[3 * a + 1] * rvecs[3 * a + 1] + rvecs[3 * a + 2] * rvecs[3 * a + 2]);
+    const int n = a == 0 ? nx : (a == 1 ? ny : nz);
+    d.inv_n[a] = 1.0 / n;
+  }
+  if (images) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        if (a != b && rvecs[3 * a + b] != 0.0) return fail(CMM_E_UNSUPPORTED, "image sum needs an orthorhombic cell");
+    for (int a = 0; a < 3; ++a) d.nimg[a] = (int)std::ceil(cutoff / d.L[a]) + 1;
+  }
+  d.rc = cutoff;
+  d.sigf = sigma_f;
+  d.epsf = epsilon_f;
+  d.images = images ? 1 : 0;
+  d.limit = limit;
+  return CMM_OK;
+}
+
+int cmm_lj_grid(int nx, int ny, int nz, const double* rvecs, const double* atoms, int natom, double sigma_f,
+                double epsilon_f, double cutoff, int images, double limit, double* V, void* stream) {
+  if (!V || (natom > 0 && !atoms) || natom < 0) return fail(CMM_E_ARG, "null argument");
+  LJDev d;
+  int rc = lj_setup(nx, ny, nz, rvecs, sigma_f, epsilon_f, cutoff, images, limit, d);
+  if (rc) return rc;
+  dim3 grid((nz + LJ_BZ - 1) / LJ_BZ, (ny + LJ_BY - 1) / LJ_BY, (nx + LJ_BX - 1) / LJ_BX);
+  hipLaunchKernelGGL(k_lj, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, natom, d, V);
+  LAUNCHCHK("k_lj");
+  return CMM_OK;
+}
+
+int cmm_lj_points(const double* points, long long M, const double* rvecs, const double* atoms, int natom,
+                  double sigma_f, double epsilon_f, double cutoff, int images, double limit, double* V, void* stream) {
+  if (!V || !points || (natom > 0 && !atoms) || natom < 0 || M < 0) return fail(CMM_E_ARG, "null argument");
+  if (M == 0) return CMM_OK;
+  LJDev d;
+  int rc = lj_setup(1, 1, 1, rvecs, sigma_f, epsilon_f, cutoff, images, limit, d);
+  if (rc) return rc;
+  dim3 grid((unsigned)((M + LJ_NT - 1) / LJ_NT));
+  hipLaunchKernelGGL(k_lj_points, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, natom, d, points, M, V);
+  LAUNCHCHK("k_lj_points");
+  return CMM_OK;
+}

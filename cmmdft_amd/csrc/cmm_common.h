@@ -1,0 +1,272 @@
+// Shared device-side building blocks of the cDFT hot path for CDNA4 (gfx950):
+// complex FP64 helpers, the in-LDS Stockham FFT (radix 2/3/4/5/8, mixed,
+// runtime stage plan), and deterministic block reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CMM_MAX_STAGES 10
+#define CMM_MAX_INST 256
+
+// One 1-D transform length: radix schedule and twiddle table
+// tw[j] = exp(-2*pi*i*j/n), j in [0, n).
+struct Fft1D {
+  int n;
+  int nst;
+  int rad[CMM_MAX_STAGES];
+  const double2* tw;
+};
+
+// Per-axis k-space tables of one grid: k values (2*pi*fftfreq(n, h)), the
+// Lanczos sigma factor sinc(k h / 2 pi) and the Nyquist index (-1 if n odd).
+struct KAxis {
+  const double* k;
+  const double* lanczos;
+  int nyq;
+};
+
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// multiply by (sgn * i)
+__device__ __forceinline__ double2 cmul_i(double2 a, double sgn) { return make_double2(-sgn * a.y, sgn * a.x); }
+
+// ---------------------------------------------------------------------------
+// Radix-r DFT butterflies. SGN = -1: forward (exp(-2 pi i tu/r)); +1 inverse.
+// ---------------------------------------------------------------------------
+template <int SGN>
+__device__ __forceinline__ void dft2(double2* v) {
+  double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <int SGN>
+__device__ __forceinline__ void dft3(double2* v) {
+  const double s3 = 0.86602540378443864676;  // sin(2 pi / 3)
+  double2 t1 = cadd(v[1], v[2]);
+  double2 t2 = csub(v[1], v[2]);
+  double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+  double2 r = cmul_i(cscale(t2, s3), (double)SGN);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <int SGN>
+__device__ __forceinline__ void dft4(double2* v) {
+  double2 a = cadd(v[0], v[2]);
+  double2 b = csub(v[0], v[2]);
+  double2 c = cadd(v[1], v[3]);
+  double2 d = cmul_i(csub(v[1], v[3]), (double)SGN);
+  v[0] = cadd(a, c);
+  v[2] = csub(a, c);
+  v[1] = cadd(b, d);
+  v[3] = csub(b, d);
+}
+
+template <int SGN>
+__device__ __forceinline__ void dft5(double2* v) {
+  const double c1 = 0.30901699437494742410;   // cos(2 pi / 5)
+  const double c2 = -0.80901699437494742410;  // cos(4 pi / 5)
+  const double s1 = 0.95105651629515357212;   // sin(2 pi / 5)
+  const double s2 = 0.58778525229247312917;   // sin(4 pi / 5)
+  double2 t1 = cadd(v[1], v[4]);
+  double2 t2 = cadd(v[2], v[3]);
+  double2 t3 = csub(v[1], v[4]);
+  double2 t4 = csub(v[2], v[3]);
+  double2 a1 = make_double2(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+  double2 a2 = make_double2(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+  double2 b1 = cmul_i(make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y), (double)SGN);
+  double2 b2 = cmul_i(make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y), (double)SGN);
+  v[0] = cadd(v[0], cadd(t1, t2));
+  v[1] = cadd(a1, b1);
+  v[4] = csub(a1, b1);
+  v[2] = cadd(a2, b2);
+  v[3] = csub(a2, b2);
+}
+
+template <int SGN>
+__device__ __forceinline__ void dft8(double2* v) {
+  const double h = 0.70710678118654752440;  // 1/sqrt(2)
+  // two radix-4 on even/odd, then combine with W8 twiddles
+  double2 e[4] = {v[0], v[2], v[4], v[6]};
+  double2 o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<SGN>(e);
+  dft4<SGN>(o);
+  // o[k] *= exp(SGN 2 pi i k / 8)
+  double2 o1 = make_double2(h * (o[1].x - SGN * o[1].y), h * (o[1].y + SGN * o[1].x));
+  double2 o2 = cmul_i(o[2], (double)SGN);
+  double2 o3 = make_double2(-h * (o[3].x + SGN * o[3].y), h * (SGN * o[3].x - o[3].y));
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+template <int R, int SGN>
+__device__ __forceinline__ void dftr(double2* v) {
+  if constexpr (R == 2) dft2<SGN>(v);
+  else if constexpr (R == 3) dft3<SGN>(v);
+  else if constexpr (R == 4) dft4<SGN>(v);
+  else if constexpr (R == 5) dft5<SGN>(v);
+  else dft8<SGN>(v);
+}
+
+// ---------------------------------------------------------------------------
+// In-place Stockham stage over `batch` sequences stored in LDS.
+// Element e of sequence s lives at buf[s*bs + e*es].  EM (element-major) picks
+// the butterfly -> thread mapping that keeps consecutive lanes on consecutive
+// LDS addresses: sequences fastest when es==batch-stride layout [n][batch],
+// elements fastest for [batch][n].  Each thread stages at most NB butterflies
+// in registers: all lanes load, barrier, all lanes store, barrier.
+// ---------------------------------------------------------------------------
+template <int R, int SGN, bool EM, int NB>
+__device__ __forceinline__ void stockham_stage(double2* buf, int batch, int bs, int es, const Fft1D& p, int Ns) {
+  const int n = p.n;
+  const int nq = n / R;                 // butterflies per sequence
+  const int total = batch * nq;
+  const int twstride = n / (Ns * R);
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  double2 v[NB][R];
+  int dst[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    int b = tid + i * nt;
+    dst[i] = -1;
+    if (b < total) {
+      int s, j;
+      if (EM) { s = b % batch; j = b / batch; }
+      else { j = b % nq; s = b / nq; }
+      const int k = j % Ns;
+      double2* base = buf + s * bs;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[i][t] = base[(j + t * nq) * es];
+      if (Ns > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+          double2 w = p.tw[(t * k * twstride) % n];
+          if (SGN > 0) w.y = -w.y;
+          v[i][t] = cmul(v[i][t], w);
+        }
+      }
+      dftr<R, SGN>(v[i]);
+      dst[i] = s * bs + ((j / Ns) * Ns * R + k) * es;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (dst[i] >= 0) {
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[dst[i] + t * Ns * es] = v[i][t];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform of `batch` sequences of length p.n, in place in LDS.
+// VPT bounds the complex values one thread holds per stage: the caller must
+// guarantee batch * n <= VPT * blockDim.x, and that a barrier separates the
+// data's producers from this call.  Ends with a barrier.
+template <int SGN, bool EM, int VPT>
+__device__ void lds_fft(double2* buf, int batch, int bs, int es, const Fft1D& p) {
+  int Ns = 1;
+  for (int s = 0; s < p.nst; ++s) {
+    const int r = p.rad[s];
+    switch (r) {
+      case 2: stockham_stage<2, SGN, EM, (VPT + 1) / 2>(buf, batch, bs, es, p, Ns); break;
+      case 3: stockham_stage<3, SGN, EM, (VPT + 2) / 3>(buf, batch, bs, es, p, Ns); break;
+      case 4: stockham_stage<4, SGN, EM, (VPT + 3) / 4>(buf, batch, bs, es, p, Ns); break;
+      case 5: stockham_stage<5, SGN, EM, (VPT + 4) / 5>(buf, batch, bs, es, p, Ns); break;
+      default: stockham_stage<8, SGN, EM, (VPT + 7) / 8>(buf, batch, bs, es, p, Ns); break;
+    }
+    Ns *= r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic block + grid reductions.  Each block reduces NQ doubles
+// (op 0 = sum, 1 = max), writes them to partials[q*nblocks + bid]; the last
+// block to arrive (global counter) sums the partials in index order and writes
+// out[q].  Order of summation is fixed, so results are bitwise reproducible.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int NQ>
+__device__ void block_reduce(double (&v)[NQ], const int (&ops)[NQ], double* sh /* >= 16*NQ */) {
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double x = ops[q] ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0) sh[q * 16 + wid] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      double a = sh[q * 16];
+      for (int w = 1; w < nw; ++w) a = ops[q] ? fmax(a, sh[q * 16 + w]) : a + sh[q * 16 + w];
+      v[q] = a;
+    }
+  }
+  __syncthreads();
+}
+
+// Called by every block with its block-reduced values valid in thread 0.
+// partials: [NQ][nblocks]; counter: one unsigned per reduction instance.
+template <int NQ>
+__device__ void grid_reduce_finish(const double (&v)[NQ], const int (&ops)[NQ], double* partials, int nblocks,
+                                   int bid, unsigned* counter, double* out, double* sh) {
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) partials[q * nblocks + bid] = v[q];
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    is_last = (prev == (unsigned)(nblocks - 1));
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  // fixed-order two-level sum: lanes stride over partials, then tree
+  for (int q = 0; q < NQ; ++q) {
+    double a = ops[q] ? -INFINITY : 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+      double x = __hip_atomic_load(&partials[q * nblocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      a = ops[q] ? fmax(a, x) : a + x;
+    }
+    double r = ops[q] ? wave_max(a) : wave_sum(a);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    if (lane == 0) sh[wid] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = sh[0];
+      for (int w = 1; w < nw; ++w) t = ops[q] ? fmax(t, sh[w]) : t + sh[w];
+      out[q] = t;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *counter = 0u;
+}

@@ -1,0 +1,94 @@
+// Device-side parameter blocks shared by the kernels and their launchers.
+#pragma once
+
+#include "cmm_common.h"
+
+#define CMM_MMAX 8   // longest Anderson history supported on the device
+
+namespace cmm {
+
+enum { MODE_C1 = 0, MODE_PICARD = 1, MODE_ANDRES = 2 };
+enum { MODE_MIX = 0, MODE_TRIAL = 1, MODE_SEED = 2 };
+
+// Per-instance reduction results (doubles, on the device), NSCAL per instance.
+constexpr int NSCAL = 32;
+constexpr int RED_SITES = 4;
+constexpr int S_PHI = 0;    // sum Phi, sum f_ex(rho_bar), max n3            (functional pass)
+constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (final pass)
+                            //   picard: N', sum|d|, max|d|, nonfinite
+                            //   anderson: nonfinite, Gram row[CMM_MMAX]
+constexpr int S_GRAM = S_NCUR + 4;
+constexpr int S_PN = 16;    // N', sum|d|, max|d|, nonfinite, ideal', ext'  (density producers)
+
+struct GridDev {
+  int nx, ny, nz, nzh;
+  double inv_npts;
+  Fft1D fx, fy, fz;
+  KAxis kx, ky, kz;
+};
+
+// Functional: FMT hard spheres (base 0 Rosenfeld, 1 modified, 2 White-Bear
+// II; anti = the 'a' variant) and WDA with a sum of EOS terms (kind 0 mean
+// field, 1 Carnahan-Starling, 2 MBWR).
+struct FuncDev {
+  int use_hs, use_wda, base, anti, neos;
+  int eos_kind[4];
+  double eos_fac[4];
+  double eos_par[4][16];
+  double R, D, kT;
+  double c_w2, c_w3, inv4piR2, inv4piR;
+};
+
+struct InstDev {
+  double fug;     // fugacity exp(beta mu) / beta / Lambda^3
+  double alpha;   // mixing parameter (times correction factor) / trial step
+};
+
+struct RedDev {
+  double* partials;
+  long long pstride;
+  unsigned* counters;
+  double* scal;
+};
+
+// Density-update passes.  Real fields are [inst][npts] with instance stride
+// fstride; history rings are [slot][inst][npts] with slot stride hstride
+// (residuals) / hstride2 (densities).
+struct ElDev {
+  const double* rho;
+  const double* vext;
+  int vext_batched;
+  long long fstride;
+  const InstDev* inst;
+  double beta, lam3, zero_thr;
+  double* c1out;
+  double* rho_out;
+  double2* rz;
+  long long rzstride;
+  double* rhist;
+  long long hstride;
+  int slot_new;
+  int m;
+  const double* rhohist;
+  long long hstride2;
+  const double* coef;
+  int mk;
+  int order[CMM_MMAX];
+  int order_rho[CMM_MMAX];
+  int clip_trial;
+};
+
+struct AndDev {
+  double* gram;
+  int m, mk;
+  int order[CMM_MMAX];
+  int slot_new, slot_prev;
+  double* damping;
+  const double* cf;
+  int adaptive;
+  double fac_up, fac_down, dmin, dmax;
+  double* coef;
+  double* alpha_out;
+};
+
+}  // namespace cmm

@@ -1,0 +1,808 @@
+// HIP kernels of the cDFT hot path for CDNA4 (gfx950), all FP64.
+//
+// Data layout in HBM (C order, z fastest, per instance of a batch):
+//   real field      double [nx][ny][nz]
+//   half spectrum   double2[nx][ny][nz/2+1]   (R2C along z, then complex y, x)
+// A 3-D transform is three pencil passes (z, y, x).  The z pass packs two real
+// pencils into one complex sequence (two-for-one), so odd lengths need no
+// special case.  Every pass stages a tile of pencils in LDS and runs an
+// in-place Stockham FFT there; the functional's k-space weights, the
+// pointwise derivatives of the FMT/WDA free-energy density and the
+// Euler-Lagrange update are fused into the passes that own the data.
+#include "cmm_common.h"
+#include "cmm_kernels.h"
+
+#include <math.h>
+
+namespace cmm {
+
+constexpr int NT = 256;   // threads per block of the FFT passes
+constexpr int VPT = 16;   // complex values per thread per Stockham stage
+constexpr int EPTX = 8;   // x-pass tile elements per thread (nx * TB <= EPTX * NT)
+
+// ---------------------------------------------------------------------------
+// z-pass helpers (pencils along the contiguous axis, two real pencils per
+// complex sequence).  LDS holds (P/2) sequences of length n: seq q = pencils
+// p0+2q (real part) and p0+2q+1 (imag part).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void z_load_real(double2* lds, const double* src, long long p0, int P, long long npen, int n) {
+  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
+    const int pl = i / n, e = i - pl * n;
+    const long long p = p0 + pl;
+    const double v = (p < npen) ? src[p * n + e] : 0.0;
+    reinterpret_cast<double*>(&lds[(pl >> 1) * n + e])[pl & 1] = v;
+  }
+}
+
+// forward-transformed pairs -> two half spectra
+__device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, long long p0, int P, long long npen, int n,
+                                             int nzh) {
+  for (int i = threadIdx.x; i < (P >> 1) * nzh; i += blockDim.x) {
+    const int q = i / nzh, k = i - q * nzh;
+    const double2 zk = lds[q * n + k];
+    const double2 zm = lds[q * n + (k == 0 ? 0 : n - k)];
+    const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+    const double2 B = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    const long long pa = p0 + 2 * q, pb = pa + 1;
+    if (pa < npen) dst[pa * nzh + k] = A;
+    if (pb < npen) dst[pb * nzh + k] = B;
+  }
+}
+
+// two half spectra -> complex sequence whose inverse FFT is (a + i b).
+// The k = 0 and k = n/2 coefficients are projected onto their real parts,
+// i.e. the Hermitian part is taken, which is what the real part of a full
+// complex inverse transform does.
+__device__ __forceinline__ void z_load_half(double2* lds, const double2* src, long long p0, int P, long long npen, int n,
+                                            int nzh) {
+  for (int i = threadIdx.x; i < (P >> 1) * nzh; i += blockDim.x) {
+    const int q = i / nzh, k = i - q * nzh;
+    const long long pa = p0 + 2 * q, pb = pa + 1;
+    double2 A = (pa < npen) ? src[pa * nzh + k] : make_double2(0.0, 0.0);
+    double2 B = (pb < npen) ? src[pb * nzh + k] : make_double2(0.0, 0.0);
+    const bool selfconj = (k == 0) || (2 * k == n);
+    if (selfconj) { A.y = 0.0; B.y = 0.0; }
+    lds[q * n + k] = make_double2(A.x - B.y, A.y + B.x);
+    if (!selfconj) lds[q * n + n - k] = make_double2(A.x + B.y, B.x - A.y);
+  }
+}
+
+__device__ __forceinline__ double& z_val(double2* lds, int pl, int e, int n) {
+  return reinterpret_cast<double*>(&lds[(pl >> 1) * n + e])[pl & 1];
+}
+
+// ---------------------------------------------------------------------------
+// Plain transform passes (3-D FFT entry points and building blocks)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ x, long long xs, double2* __restrict__ X, long long Xs,
+                                            GridDev g, int P, const int* __restrict__ active) {
+  extern __shared__ double2 sm[];
+  const int inst = blockIdx.y;
+  if (active && !active[inst]) return;
+  const long long npen = (long long)g.nx * g.ny;
+  const long long p0 = (long long)blockIdx.x * P;
+  z_load_real(sm, x + inst * xs, p0, P, npen, g.nz);
+  __syncthreads();
+  lds_fft<-1, false, VPT>(sm, P >> 1, g.nz, 1, g.fz);
+  z_store_half(sm, X + inst * Xs, p0, P, npen, g.nz, g.nzh);
+}
+
+// nf fields, field stride fs (in complex / real elements)
+__global__ void __launch_bounds__(NT) k_zc2r(const double2* __restrict__ X, long long Xs, long long Xfs, double* __restrict__ x,
+                                            long long xs, long long xfs, GridDev g, int P, const int* __restrict__ active) {
+  extern __shared__ double2 sm[];
+  const int inst = blockIdx.z;
+  const int f = blockIdx.y;
+  if (active && !active[inst]) return;
+  const long long npen = (long long)g.nx * g.ny;
+  const long long p0 = (long long)blockIdx.x * P;
+  z_load_half(sm, X + inst * Xs + f * Xfs, p0, P, npen, g.nz, g.nzh);
+  __syncthreads();
+  lds_fft<1, false, VPT>(sm, P >> 1, g.nz, 1, g.fz);
+  double* out = x + inst * xs + f * xfs;
+  for (int i = threadIdx.x; i < P * g.nz; i += blockDim.x) {
+    const int pl = i / g.nz, e = i - pl * g.nz;
+    const long long p = p0 + pl;
+    if (p < npen) out[p * g.nz + e] = z_val(sm, pl, e, g.nz);
+  }
+}
+
+// y pass: tile = one x plane, TB consecutive kz columns, all y.
+template <int SGN>
+__global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
+                                             const int* __restrict__ active) {
+  extern __shared__ double2 sm[];
+  const int inst = blockIdx.z;
+  if (active && !active[inst]) return;
+  const int ntk = (g.nzh + TB - 1) / TB;
+  const int x = blockIdx.x / ntk, tk = blockIdx.x - x * ntk;
+  const int k0 = tk * TB;
+  const int tbv = min(TB, g.nzh - k0);
+  double2* base = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh + k0;
+  for (int i = threadIdx.x; i < g.ny * TB; i += blockDim.x) {
+    const int y = i / TB, b = i - y * TB;
+    sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft<SGN, true, VPT>(sm, TB, 1, TB, g.fy);
+  for (int i = threadIdx.x; i < g.ny * TB; i += blockDim.x) {
+    const int y = i / TB, b = i - y * TB;
+    if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
+  }
+}
+
+// x pass: tile = TB consecutive (ky, kz) columns of the flattened plane, all x.
+template <int SGN>
+__global__ void __launch_bounds__(NT) k_xpass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
+                                             const int* __restrict__ active) {
+  extern __shared__ double2 sm[];
+  const int inst = blockIdx.z;
+  if (active && !active[inst]) return;
+  const long long nb = (long long)g.ny * g.nzh;
+  const long long b0 = (long long)blockIdx.x * TB;
+  const int tbv = (int)min((long long)TB, nb - b0);
+  double2* base = F + inst * is + blockIdx.y * fs + b0;
+  for (int i = threadIdx.x; i < g.nx * TB; i += blockDim.x) {
+    const int x = i / TB, b = i - x * TB;
+    sm[i] = (b < tbv) ? base[(long long)x * nb + b] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft<SGN, true, VPT>(sm, TB, 1, TB, g.fx);
+  for (int i = threadIdx.x; i < g.nx * TB; i += blockDim.x) {
+    const int x = i / TB, b = i - x * TB;
+    if (b < tbv) base[(long long)x * nb + b] = sm[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Analytic k-space weights (FMT: Rosenfeld weights with the Lanczos sigma
+// factor; WDA: normalised ball of diameter D).  For one k point, returns
+// w2 (= 4 pi R^2 sinc(kR)), w3 (= 4 pi R^3/3 * 3(sin-w cos)/w^3), wD and the
+// k components.  omega <= 1e-8 uses the k -> 0 limits.
+// ---------------------------------------------------------------------------
+struct KW {
+  double w2, w3, wD;
+  bool k0;   // omega = |k| R <= 1e-8: the k -> 0 limits apply, vector weights vanish
+};
+
+__device__ __forceinline__ double j_ball(double w, double s, double c) {
+  return 3.0 * (s - w * c) / (w * w * w);
+}
+
+__device__ __forceinline__ KW kweights(const GridDev& g, const FuncDev& fp, int ix, int iy, int iz) {
+  KW r;
+  const double kx = g.kx.k[ix], ky = g.ky.k[iy], kz = g.kz.k[iz];
+  const double sl = g.kx.lanczos[ix] * g.ky.lanczos[iy] * g.kz.lanczos[iz];
+  const double kn = sqrt(kx * kx + ky * ky + kz * kz);
+  const double w = kn * fp.R;
+  double s, c;
+  r.k0 = (w <= 1e-8);
+  if (r.k0) {
+    r.w2 = fp.c_w2 * sl;
+    r.w3 = fp.c_w3 * sl;
+  } else {
+    sincos(w, &s, &c);
+    r.w2 = fp.c_w2 * (s / w) * sl;
+    r.w3 = fp.c_w3 * j_ball(w, s, c) * sl;
+  }
+  const double wd = kn * fp.D;
+  if (wd <= 1e-8) {
+    r.wD = sl;
+  } else {
+    sincos(wd, &s, &c);
+    r.wD = j_ball(wd, s, c) * sl;
+  }
+  return r;
+}
+
+// Weight of field f at grid indices (ix, iy, iz).  Forward (DIR = -1): the
+// weights that produce n2, n3, n_v2 (x,y,z) and the WDA density from rho^.
+// Back (DIR = +1): the weights that convolve the derivative fields (vector
+// weights change sign).  Each vector component is zero on the Nyquist plane of
+// its own axis: there its spectrum has no Hermitian part, so the real part of
+// the inverse transform does not see it.  At k = 0 the vector weights vanish.
+__device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix, int iy, int iz, int f, double2 v,
+                                           int dir) {
+  const bool k0 = w.k0;
+  switch (f) {
+    case 0: return cscale(v, w.w2);
+    case 1: return cscale(v, w.w3);
+    case 2: return (k0 || ix == g.kx.nyq) ? make_double2(0.0, 0.0) : cmul_i(cscale(v, w.w3 * g.kx.k[ix]), (double)dir);
+    case 3: return (k0 || iy == g.ky.nyq) ? make_double2(0.0, 0.0) : cmul_i(cscale(v, w.w3 * g.ky.k[iy]), (double)dir);
+    case 4: return (k0 || iz == g.kz.nyq) ? make_double2(0.0, 0.0) : cmul_i(cscale(v, w.w3 * g.kz.k[iz]), (double)dir);
+    default: return cscale(v, w.wD);
+  }
+}
+
+// x pass of the forward chain: rho (z,y transformed) -> x FFT -> 6 weighted
+// copies -> inverse x FFT each -> F[0..5].  The 1/N of the inverse 3-D
+// transform is folded into the weights.
+__global__ void __launch_bounds__(NT) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
+                                                long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
+                                                const int* __restrict__ active) {
+  extern __shared__ double2 sm[];
+  const int inst = blockIdx.z;
+  if (active && !active[inst]) return;
+  const int tile = g.nx * TB;
+  double2* A = sm;
+  double2* B = sm + tile;
+  const long long nb = (long long)g.ny * g.nzh;
+  const long long b0 = (long long)blockIdx.x * TB;
+  const int tbv = (int)min((long long)TB, nb - b0);
+  const double2* src = rz + inst * rzs + b0;
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    const int x = i / TB, b = i - x * TB;
+    A[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft<-1, true, VPT>(A, TB, 1, TB, g.fx);
+  KW wr[EPTX];
+  int wy[EPTX], wz[EPTX];
+#pragma unroll
+  for (int e = 0; e < EPTX; ++e) {
+    const int i = threadIdx.x + e * NT;
+    if (i < tile) {
+      const int x = i / TB, b = i - x * TB;
+      const long long bb = min(b0 + b, nb - 1);
+      wy[e] = (int)(bb / g.nzh);
+      wz[e] = (int)(bb - (long long)wy[e] * g.nzh);
+      wr[e] = kweights(g, fp, x, wy[e], wz[e]);
+    }
+  }
+  for (int f = 0; f < 6; ++f) {
+#pragma unroll
+    for (int e = 0; e < EPTX; ++e) {
+      const int i = threadIdx.x + e * NT;
+      if (i < tile) B[i] = cscale(apply_w(g, wr[e], i / TB, wy[e], wz[e], f, A[i], -1), g.inv_npts);
+    }
+    __syncthreads();
+    lds_fft<1, true, VPT>(B, TB, 1, TB, g.fx);
+    double2* dst = F + inst * Fs + f * Ffs + b0;
+    for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+      const int x = i / TB, b = i - x * TB;
+      if (b < tbv) dst[(long long)x * nb + b] = B[i];
+    }
+    __syncthreads();
+  }
+}
+
+// x pass of the back-convolution: 6 derivative fields -> x FFT each ->
+// weighted sum -> inverse x FFT -> F[0] (in place: every field's tile is read
+// before the sum is written).
+__global__ void __launch_bounds__(NT) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
+                                             int TB, const int* __restrict__ active) {
+  extern __shared__ double2 sm[];
+  const int inst = blockIdx.z;
+  if (active && !active[inst]) return;
+  const int tile = g.nx * TB;
+  double2* A = sm;
+  double2* B = sm + tile;
+  const long long nb = (long long)g.ny * g.nzh;
+  const long long b0 = (long long)blockIdx.x * TB;
+  const int tbv = (int)min((long long)TB, nb - b0);
+  KW wr[EPTX];
+  int wy[EPTX], wz[EPTX];
+#pragma unroll
+  for (int e = 0; e < EPTX; ++e) {
+    const int i = threadIdx.x + e * NT;
+    if (i < tile) {
+      const int x = i / TB, b = i - x * TB;
+      const long long bb = min(b0 + b, nb - 1);
+      wy[e] = (int)(bb / g.nzh);
+      wz[e] = (int)(bb - (long long)wy[e] * g.nzh);
+      wr[e] = kweights(g, fp, x, wy[e], wz[e]);
+      A[i] = make_double2(0.0, 0.0);
+    }
+  }
+  for (int f = 0; f < 6; ++f) {
+    const double2* src = F + inst * Fs + f * Ffs + b0;
+    for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+      const int x = i / TB, b = i - x * TB;
+      B[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    lds_fft<-1, true, VPT>(B, TB, 1, TB, g.fx);
+#pragma unroll
+    for (int e = 0; e < EPTX; ++e) {
+      const int i = threadIdx.x + e * NT;
+      if (i < tile) A[i] = cadd(A[i], cscale(apply_w(g, wr[e], i / TB, wy[e], wz[e], f, B[i], 1), g.inv_npts));
+    }
+    __syncthreads();
+  }
+  lds_fft<1, true, VPT>(A, TB, 1, TB, g.fx);
+  double2* dst = F + inst * Fs + b0;
+  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    const int x = i / TB, b = i - x * TB;
+    if (b < tbv) dst[(long long)x * nb + b] = A[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pointwise free-energy density: FMT Phi and its partial derivatives
+// (Rosenfeld 'FMT', 'MFMT', White-Bear II 'WBII', optional 'a' variant), and
+// the WDA excess free energy per volume of a sum of EOS terms.
+// ---------------------------------------------------------------------------
+struct PhiOut {
+  double phi, g0, g1, gv[3];
+};
+
+__device__ __forceinline__ void phi23(int base, double n3, double ln3, double& p2, double& p3, double& d2,
+                                      double& d3) {
+  const double n3_2 = n3 * n3, n3_3 = n3_2 * n3;
+  const double q = 1.0 - 2.0 * n3 + n3_2;
+  const double c = 1.0 - 3.0 * n3 + 3.0 * n3_2 - n3_3;
+  const double pi = M_PI;
+  if (base == 0) {                       // Rosenfeld
+    p2 = 1.0 / (1.0 - n3);
+    d2 = 1.0 / q;
+    p3 = 1.0 / (24.0 * pi * q);
+    d3 = 1.0 / (12.0 * pi * c);
+  } else if (base == 1) {                // modified FMT
+    p2 = 1.0 / (1.0 - n3);
+    d2 = 1.0 / q;
+    if (n3 <= 1e-8) {
+      p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) / (24.0 * pi * q);
+      d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) / (36.0 * pi * c);
+    } else {
+      p3 = (n3 + q * ln3) / (36.0 * pi * n3_2 * q);
+      d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) / (36.0 * pi * n3_3 * c);
+    }
+  } else {                               // White-Bear II
+    if (n3 <= 1e-8) {
+      p2 = (1.0 + n3_2 / 9.0) / (1.0 - n3);
+      d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) / q;
+      p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) / (24.0 * pi * q);
+      d3 = (7.0 / 3.0 - n3 / 2.0 + n3_2 / 10.0) / (36.0 * pi * c);
+    } else {
+      p2 = (5.0 * n3 - n3_2 + 2.0 * (1.0 - n3) * ln3) / (3.0 * (n3 - n3_2));
+      d2 = -2.0 * (n3 - 3.0 * n3_2 + q * ln3) / (3.0 * n3_2 * q);
+      p3 = -2.0 * (n3 - 3.0 * n3_2 + n3_3 + ln3 * q) / ((3.0 * n3_2) * 24.0 * pi * q);
+      d3 = (2.0 * n3 - 5.0 * n3_2 + 6.0 * n3_3 - n3_2 * n3_2 + 2.0 * c * ln3) / (36.0 * pi * n3_3 * c);
+    }
+  }
+}
+
+// n2, n3 (unclipped), nv2: weighted densities of rho.  Outputs Phi and the
+// combined derivative fields (times kT) that the back-convolution weights:
+//   g0 = kT (dPhi/dn0 / 4piR^2 + dPhi/dn1 / 4piR + dPhi/dn2)    with w2
+//   g1 = kT dPhi/dn3                                           with w3
+//   gv = kT (dPhi/dnv1 / 4piR + dPhi/dnv2)                      with -wv2
+__device__ __forceinline__ PhiOut fmt_point(const FuncDev& fp, double n2, double n3raw, double vx, double vy, double vz) {
+  PhiOut o;
+  const double R = fp.R;
+  const double i4pR2 = fp.inv4piR2, i4pR = fp.inv4piR;
+  const double n0 = n2 * i4pR2, n1 = n2 * i4pR;
+  const double ux = vx * i4pR, uy = vy * i4pR, uz = vz * i4pR;   // n_v1
+  const double n3 = fmin(fmax(n3raw, 1e-30), 0.99);
+  const double ln3 = log(1.0 - n3);
+  double p2, p3, d2, d3;
+  phi23(fp.base, n3, ln3, p2, p3, d2, d3);
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double dot12 = ux * vx + uy * vy + uz * vz;
+  double q3, dq3_n2, fv;
+  if (fp.anti) {
+    double xi = v2 / (n2 * n2 + 1e-16);
+    xi = fmin(fmax(xi, 0.0), 1.0);
+    const double om = 1.0 - xi;
+    q3 = n2 * n2 * n2 * om * om * om;
+    dq3_n2 = 3.0 * n2 * n2 * (1.0 + xi) * om * om;
+    fv = -6.0 * n2 * om * om * p3;
+  } else {
+    q3 = n2 * n2 * n2 - 3.0 * n2 * v2;
+    dq3_n2 = 3.0 * (n2 * n2 - v2);
+    fv = -6.0 * n2 * p3;
+  }
+  o.phi = n0 * (-ln3) + (n1 * n2 - dot12) * p2 + q3 * p3;
+  const double dn0 = -ln3;
+  const double dn1 = n2 * p2;
+  const double dn2 = n1 * p2 + dq3_n2 * p3;
+  const double dn3 = n0 / (1.0 - n3) + (n1 * n2 - dot12) * d2 + q3 * d3;
+  const double kT = fp.kT;
+  o.g0 = kT * (dn0 * i4pR2 + dn1 * i4pR + dn2);
+  o.g1 = kT * dn3;
+  // dPhi/dnv1 = -nv2 p2 ; dPhi/dnv2 = -nv1 p2 + nv2 fv
+  o.gv[0] = kT * ((-vx * p2) * i4pR + (-ux * p2 + vx * fv));
+  o.gv[1] = kT * ((-vy * p2) * i4pR + (-uy * p2 + vy * fv));
+  o.gv[2] = kT * ((-vz * p2) * i4pR + (-uz * p2 + vz * fv));
+  (void)R;
+  return o;
+}
+
+// EOS term: excess free energy per volume f(rho) and df/drho.
+__device__ __forceinline__ void eos_term(int kind, const double* p, double rho, double& f, double& df) {
+  if (kind == 0) {            // mean field: a_ex = a rho
+    const double a = p[0];
+    f = rho * (a * rho);
+    df = rho * a + a * rho;
+  } else if (kind == 1) {     // Carnahan-Starling, p = {eta (=4 pi R^3/3), kT}
+    const double eta = p[0], kT = p[1];
+    const double x = eta * rho;
+    const double om = 1.0 - x;
+    const double ap = kT * (4.0 * x - 3.0 * x * x) / (om * om);
+    const double dap = 2.0 * kT * eta * (2.0 - x) / (om * om * om);
+    f = rho * ap;
+    df = rho * dap + ap;
+  } else {                    // MBWR (Johnson et al. 1993): p = {a1..a8, b1..b6, sigma^3, eps}
+    const double s3 = p[14], eps = p[15], gam = 3.0;
+    const double rr = rho * s3;
+    const double F = exp(-gam * rr * rr);
+    const double ig = 1.0 / (2.0 * gam);
+    double A = 0.0, dA = 0.0, pw = 1.0;
+    for (int i = 0; i < 8; ++i) {      // sum a_i/(i+1) rr^(i+1) ; d: a_i rr^i s3
+      dA += p[i] * pw * s3;
+      pw *= rr;
+      A += p[i] / (i + 1) * pw;
+    }
+    double G = ig * (1.0 - F);
+    A += p[8] * G;
+    double rr2 = rr * rr, pw2 = rr2;
+    for (int t = 1; t < 6; ++t) {      // G_{t+1} = -ig (F rr^(2t) - 2t G_t)
+      G = -ig * (F * pw2 - 2.0 * t * G);
+      A += p[8 + t] * G;
+      pw2 *= rr2;
+    }
+    double po = rr;                   // rr^(2t+1)
+    for (int t = 0; t < 6; ++t) {
+      dA += p[8 + t] * s3 * po * F;
+      po *= rr2;
+    }
+    const double ap = A * eps, dap = dA * eps;
+    f = rho * ap;
+    df = rho * dap + ap;
+  }
+}
+
+__device__ __forceinline__ void wda_point(const FuncDev& fp, double rb, double& f, double& df) {
+  f = 0.0;
+  df = 0.0;
+  for (int t = 0; t < fp.neos; ++t) {
+    double ft, dft;
+    eos_term(fp.eos_kind[t], fp.eos_par[t], rb, ft, dft);
+    f += fp.eos_fac[t] * ft;
+    df += fp.eos_fac[t] * dft;
+  }
+}
+
+// z pass of the functional: 6 weighted densities (x,y inverse-transformed)
+// -> C2R -> pointwise Phi / WDA derivatives -> R2C -> F[0..5] in place.
+// Reduces sum Phi, sum f_ex(rho_bar), max n3 (unclipped) per instance.
+__global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
+                                            int P, int store, const int* __restrict__ active, RedDev red) {
+  extern __shared__ double2 sm[];
+  __shared__ double sh[64];
+  const int inst = blockIdx.y;
+  if (active && !active[inst]) return;
+  const int n = g.nz;
+  const long long npen = (long long)g.nx * g.ny;
+  const long long p0 = (long long)blockIdx.x * P;
+  const int seg = (P >> 1) * n;
+  for (int f = 0; f < 6; ++f) z_load_half(sm + f * seg, F + inst * Fs + f * Ffs, p0, P, npen, n, g.nzh);
+  __syncthreads();
+  lds_fft<1, false, VPT>(sm, 6 * (P >> 1), n, 1, g.fz);
+  double acc[3] = {0.0, 0.0, -INFINITY};
+  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
+    const int pl = i / n, e = i - pl * n;
+    if (p0 + pl >= npen) continue;
+    double& r0 = z_val(sm, pl, e, n);
+    double& r1 = z_val(sm + seg, pl, e, n);
+    double& r2 = z_val(sm + 2 * seg, pl, e, n);
+    double& r3 = z_val(sm + 3 * seg, pl, e, n);
+    double& r4 = z_val(sm + 4 * seg, pl, e, n);
+    double& r5 = z_val(sm + 5 * seg, pl, e, n);
+    double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
+    if (fp.use_hs) {
+      const PhiOut o = fmt_point(fp, r0, r1, r2, r3, r4);
+      acc[0] += o.phi;
+      acc[2] = fmax(acc[2], r1);
+      g0 = o.g0;
+      g1 = o.g1;
+      gv0 = o.gv[0];
+      gv1 = o.gv[1];
+      gv2 = o.gv[2];
+    }
+    if (fp.use_wda) {
+      double fw, dfw;
+      wda_point(fp, r5, fw, dfw);
+      acc[1] += fw;
+      g5 = dfw;
+    }
+    r0 = g0;
+    r1 = g1;
+    r2 = gv0;
+    r3 = gv1;
+    r4 = gv2;
+    r5 = g5;
+  }
+  if (store) {
+    __syncthreads();
+    lds_fft<-1, false, VPT>(sm, 6 * (P >> 1), n, 1, g.fz);
+    for (int f = 0; f < 6; ++f) z_store_half(sm + f * seg, F + inst * Fs + f * Ffs, p0, P, npen, n, g.nzh);
+  }
+  const int ops[3] = {0, 0, 1};
+  block_reduce<3>(acc, ops, sh);
+  grid_reduce_finish<3>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
+                        red.counters + inst * RED_SITES + 0, red.scal + inst * NSCAL + S_PHI, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Final z pass: C2R of the back-convolved sum -> C1 = c_ex + V, then one of
+//   MODE_C1       write C1
+//   MODE_PICARD   rho' = (1-a) rho + a G, G = beta exp(-beta C1) z ;
+//                 zero below the threshold; write rho', R2C(rho') -> rz
+//   MODE_ANDRES   r = G - rho into the residual ring + Gram row
+// Always reduces (for the density rho entering this pass) N, the ideal-gas
+// sum and the external-potential sum.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, long long Fs, GridDev g, ElDev el,
+                                              int P, const int* __restrict__ active, RedDev red) {
+  extern __shared__ double2 sm[];
+  __shared__ double sh[64];
+  const int inst = blockIdx.y;
+  if (active && !active[inst]) return;
+  const int n = g.nz;
+  const long long npen = (long long)g.nx * g.ny;
+  const long long p0 = (long long)blockIdx.x * P;
+  z_load_half(sm, F + inst * Fs, p0, P, npen, n, g.nzh);
+  __syncthreads();
+  lds_fft<1, false, VPT>(sm, P >> 1, n, 1, g.fz);
+  const double* rho = el.rho + inst * el.fstride;
+  const double* V = el.vext ? el.vext + (el.vext_batched ? inst * el.fstride : 0) : nullptr;
+  const InstDev ip = el.inst[inst];
+  constexpr int NQ = (MODE == MODE_ANDRES) ? (4 + CMM_MMAX) : 8;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  double* rnew_hist = nullptr;
+  if (MODE == MODE_ANDRES) rnew_hist = el.rhist + (long long)el.slot_new * el.hstride + inst * el.fstride;
+  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
+    const int pl = i / n, e = i - pl * n;
+    const long long p = p0 + pl;
+    if (p >= npen) continue;
+    const long long idx = p * n + e;
+    double& cv = z_val(sm, pl, e, n);
+    const double vv = V ? V[idx] : 0.0;
+    const double C1 = cv + vv;
+    const double r = rho[idx];
+    // ideal-gas and external terms of the grand potential of rho
+    const double rc = (r < 1e-12) ? 1e-30 : r;
+    acc[0] += r;
+    acc[1] += rc * (log(el.lam3 * rc) - 1.0);
+    acc[2] += r * vv;
+    if (MODE == MODE_C1) {
+      el.c1out[inst * el.fstride + idx] = C1;
+    } else {
+      if (el.c1out) el.c1out[inst * el.fstride + idx] = C1;
+      const double G = el.beta * exp(-el.beta * C1) * ip.fug;
+      if (MODE == MODE_PICARD) {
+        double rn = (1.0 - ip.alpha) * r + ip.alpha * G;
+        if (rn < el.zero_thr) rn = 0.0;
+        el.rho_out[inst * el.fstride + idx] = rn;
+        cv = rn;
+        const double d = fabs(rn - r);
+        acc[3] += rn;
+        acc[4] += d;
+        acc[5] = fmax(acc[5], d);
+        acc[6] += isfinite(rn) ? 0.0 : 1.0;
+      } else {
+        const double rr = G - r;
+        rnew_hist[idx] = rr;
+        acc[3] += isfinite(rr) ? 0.0 : 1.0;
+#pragma unroll
+        for (int j = 0; j < CMM_MMAX; ++j) {
+          if (j < el.m) {
+            const double other = (j == el.slot_new) ? rr : el.rhist[(long long)j * el.hstride + inst * el.fstride + idx];
+            acc[4 + j] += rr * other;
+          }
+        }
+      }
+    }
+  }
+  if (MODE == MODE_PICARD) {
+    __syncthreads();
+    lds_fft<-1, false, VPT>(sm, P >> 1, n, 1, g.fz);
+    z_store_half(sm, el.rz + inst * el.rzstride, p0, P, npen, n, g.nzh);
+  }
+  int ops[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) ops[q] = 0;
+  if (MODE == MODE_PICARD) ops[5] = 1;
+  block_reduce<NQ>(acc, ops, sh);
+  grid_reduce_finish<NQ>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
+                         red.counters + inst * RED_SITES + 1, red.scal + inst * NSCAL + S_NCUR, sh);
+}
+
+// Density producer passes (fused with the z R2C of the produced density):
+//   MODE_MIX    Anderson mixing  rho' = clip(sum_i c_i rho_i + d_i r_i)
+//   MODE_TRIAL  line-search point rho' = rho + a r   (hybrid Picard)
+//   MODE_SEED   rho' = rho (just transform an initial density)
+// Reduces N', sum|rho'-rho_cur|, max|rho'-rho_cur|, non-finite count, and the
+// ideal-gas / external sums of rho'.
+template <int MODE>
+__global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, int P, const int* __restrict__ active,
+                                                RedDev red) {
+  extern __shared__ double2 sm[];
+  __shared__ double sh[64];
+  const int inst = blockIdx.y;
+  if (active && !active[inst]) return;
+  const int n = g.nz;
+  const long long npen = (long long)g.nx * g.ny;
+  const long long p0 = (long long)blockIdx.x * P;
+  const double* V = el.vext ? el.vext + (el.vext_batched ? inst * el.fstride : 0) : nullptr;
+  const double* rcur = el.rho + inst * el.fstride;
+  double* rout = el.rho_out + inst * el.fstride;
+  const double* coef = el.coef + inst * 2 * CMM_MMAX;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
+    const int pl = i / n, e = i - pl * n;
+    const long long p = p0 + pl;
+    double rn = 0.0;
+    if (p < npen) {
+      const long long idx = p * n + e;
+      const double r0 = rcur[idx];
+      if (MODE == MODE_MIX) {
+        double s = 0.0;
+        for (int j = 0; j < el.mk; ++j) {
+          const int so = el.order[j];
+          s += coef[j] * el.rhohist[(long long)el.order_rho[j] * el.hstride2 + inst * el.fstride + idx];
+          s += coef[CMM_MMAX + j] * el.rhist[(long long)so * el.hstride + inst * el.fstride + idx];
+        }
+        rn = (s < 1e-12) ? 1e-30 : s;
+      } else if (MODE == MODE_TRIAL) {
+        const double rr = el.rhist[(long long)el.slot_new * el.hstride + inst * el.fstride + idx];
+        rn = r0 + el.inst[inst].alpha * rr;
+        if (el.clip_trial && rn < 1e-12) rn = 1e-30;
+      } else {
+        rn = r0;
+      }
+      if (MODE != MODE_SEED) rout[idx] = rn;
+      const double vv = V ? V[idx] : 0.0;
+      const double rc = (rn < 1e-12) ? 1e-30 : rn;
+      const double d = fabs(rn - r0);
+      acc[0] += rn;
+      acc[1] += d;
+      acc[2] = fmax(acc[2], d);
+      acc[3] += isfinite(rn) ? 0.0 : 1.0;
+      acc[4] += rc * (log(el.lam3 * rc) - 1.0);
+      acc[5] += rn * vv;
+    }
+    z_val(sm, pl, e, n) = rn;
+  }
+  __syncthreads();
+  lds_fft<-1, false, VPT>(sm, P >> 1, n, 1, g.fz);
+  z_store_half(sm, el.rz + inst * el.rzstride, p0, P, npen, n, g.nzh);
+  const int ops[7] = {0, 0, 1, 0, 0, 0, 0};
+  block_reduce<7>(acc, ops, sh);
+  grid_reduce_finish<7>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
+                        red.counters + inst * RED_SITES + 2, red.scal + inst * NSCAL + S_PN, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Anderson coefficients: minimise a^T G a over the simplex {a >= 0, sum a = 1}
+// exactly, by solving the equality-constrained problem on every support and
+// keeping the best feasible one (a convex QP attains its minimum on the
+// relative interior of some face, where it equals that face's affine
+// minimiser).  One wavefront per instance.  Also updates the Gram matrix ring
+// from this step's row and the adaptive damping.
+// ---------------------------------------------------------------------------
+__device__ bool kkt_solve(const double* Gm, int mk, unsigned mask, double* alpha) {
+  // unknowns: alpha_S (|S| of them) and lambda; rows: G_S a - lambda 1 = 0, 1^T a = 1
+  int idx[CMM_MMAX];
+  int s = 0;
+  for (int i = 0; i < mk; ++i)
+    if (mask & (1u << i)) idx[s++] = i;
+  const int dim = s + 1;
+  double M[CMM_MMAX + 1][CMM_MMAX + 2];
+  for (int r = 0; r < s; ++r) {
+    for (int c = 0; c < s; ++c) M[r][c] = Gm[idx[r] * CMM_MMAX + idx[c]];
+    M[r][s] = -1.0;
+    M[r][s + 1] = 0.0;
+  }
+  for (int c = 0; c < s; ++c) M[s][c] = 1.0;
+  M[s][s] = 0.0;
+  M[s][s + 1] = 1.0;
+  for (int c = 0; c < dim; ++c) {
+    int piv = c;
+    double best = fabs(M[c][c]);
+    for (int r = c + 1; r < dim; ++r)
+      if (fabs(M[r][c]) > best) { best = fabs(M[r][c]); piv = r; }
+    if (best < 1e-13) return false;
+    if (piv != c)
+      for (int k = 0; k <= dim; ++k) { double t = M[c][k]; M[c][k] = M[piv][k]; M[piv][k] = t; }
+    for (int r = c + 1; r < dim; ++r) {
+      const double f = M[r][c] / M[c][c];
+      for (int k = c; k <= dim; ++k) M[r][k] -= f * M[c][k];
+    }
+  }
+  double x[CMM_MMAX + 1];
+  for (int r = dim - 1; r >= 0; --r) {
+    double t = M[r][dim];
+    for (int k = r + 1; k < dim; ++k) t -= M[r][k] * x[k];
+    x[r] = t / M[r][r];
+  }
+  for (int i = 0; i < mk; ++i) alpha[i] = 0.0;
+  double sum = 0.0;
+  for (int r = 0; r < s; ++r) {
+    if (x[r] < -1e-12) return false;
+    const double a = fmax(x[r], 0.0);
+    alpha[idx[r]] = a;
+    sum += a;
+  }
+  if (!(sum > 0.0)) return false;
+  for (int i = 0; i < mk; ++i) alpha[i] /= sum;
+  return true;
+}
+
+__global__ void __launch_bounds__(64) k_anderson_coef(AndDev ad, const int* __restrict__ active, RedDev red) {
+  const int inst = blockIdx.x;
+  if (active && !active[inst]) return;
+  double* Gring = ad.gram + inst * CMM_MMAX * CMM_MMAX;   // by ring slot
+  const double* row = red.scal + inst * NSCAL + S_GRAM;
+  const int lane = threadIdx.x;
+  // this step's Gram row (slot_new against every ring slot)
+  if (lane < ad.m) {
+    Gring[ad.slot_new * CMM_MMAX + lane] = row[lane];
+    Gring[lane * CMM_MMAX + ad.slot_new] = row[lane];
+  }
+  __syncthreads();
+  const int mk = ad.mk;
+  // ordered sub-matrix (oldest -> newest), normalised by its largest diagonal
+  __shared__ double Gm[CMM_MMAX * CMM_MMAX];
+  __shared__ double scale;
+  if (lane == 0) {
+    double mx = 0.0;
+    for (int i = 0; i < mk; ++i) mx = fmax(mx, Gring[ad.order[i] * CMM_MMAX + ad.order[i]]);
+    scale = (mx > 0.0 && isfinite(mx)) ? 1.0 / mx : 0.0;
+  }
+  __syncthreads();
+  for (int t = lane; t < mk * mk; t += 64) {
+    const int i = t / mk, j = t - i * mk;
+    Gm[i * CMM_MMAX + j] = Gring[ad.order[i] * CMM_MMAX + ad.order[j]] * scale;
+  }
+  __syncthreads();
+  double best = INFINITY;
+  unsigned bestmask = 0;
+  double alpha[CMM_MMAX];
+  const unsigned nsub = (1u << mk) - 1u;
+  if (scale > 0.0) {
+    for (unsigned m = 1 + lane; m <= nsub; m += 64) {
+      if (!kkt_solve(Gm, mk, m, alpha)) continue;
+      double f = 0.0;
+      for (int i = 0; i < mk; ++i) {
+        double t = 0.0;
+        for (int j = 0; j < mk; ++j) t += Gm[i * CMM_MMAX + j] * alpha[j];
+        f += alpha[i] * t;
+      }
+      if (f < best || (f == best && m < bestmask)) { best = f; bestmask = m; }
+    }
+  }
+  // wave argmin (value, then mask)
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ob = __shfl_xor(best, o, 64);
+    const unsigned om = __shfl_xor(bestmask, o, 64);
+    if (ob < best || (ob == best && om < bestmask && om != 0)) { best = ob; bestmask = om; }
+  }
+  if (lane == 0) {
+    if (bestmask == 0 || !kkt_solve(Gm, mk, bestmask, alpha)) {
+      for (int i = 0; i < mk; ++i) alpha[i] = 1.0 / mk;   // degenerate: uniform start point
+    }
+    // adaptive damping from the newest and previous residual norms
+    double d = ad.damping[inst];
+    if (ad.adaptive) {
+      const double rn = Gring[ad.slot_new * CMM_MMAX + ad.slot_new];
+      const double rp = (ad.slot_prev >= 0) ? Gring[ad.slot_prev * CMM_MMAX + ad.slot_prev] : 0.0;
+      if (sqrt(rn) < sqrt(rp)) d = fmin(d * ad.fac_up, ad.dmax);
+      else d = fmax(d * ad.fac_down, ad.dmin);
+      ad.damping[inst] = d;
+    }
+    const double dc = d * ad.cf[inst];
+    double* coef = ad.coef + inst * 2 * CMM_MMAX;
+    for (int i = 0; i < CMM_MMAX; ++i) {
+      coef[i] = (i < mk) ? alpha[i] : 0.0;
+      coef[CMM_MMAX + i] = (i < mk) ? dc * alpha[i] : 0.0;
+      ad.alpha_out[inst * CMM_MMAX + i] = (i < mk) ? alpha[i] : 0.0;
+    }
+  }
+}
+
+}  // namespace cmm

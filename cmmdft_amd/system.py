@@ -1,0 +1,281 @@
+"""Grid, guest and host objects with the reference's interface
+(cmmdft/system.py), backed by the HIP plan for transforms.
+
+``Grid`` keeps the reference attributes (npoints, spacings, dr, points,
+kpoints, scalprod, sigma_lanczos; system.py:231-295) -- the big (N,N,N,4)
+arrays are built on first access -- and its transforms follow the reference
+convention fft = fftn(x) P / N, ifft = Re ifftn(X conj(P)) N with
+P = exp(i pi scalprod) (system.py:310-332), computed by the HIP R2C/C2R.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import units as U
+from ._device import torch_mod, as_device
+
+__all__ = ['System', 'Host', 'EmptyHost', 'NanoporousHost', 'Guest', 'SphericalLJGuest', 'Grid', 'Cell']
+
+
+class Cell:
+    """Minimal stand-in for yaff's Cell: ``rvecs`` rows are the cell vectors."""
+
+    def __init__(self, rvecs):
+        self.rvecs = np.asarray(rvecs, dtype=np.float64).reshape(3, 3)
+        self.nvec = 3
+
+    @property
+    def volume(self):
+        return abs(float(np.linalg.det(self.rvecs)))
+
+    @property
+    def parameters(self):
+        lengths = np.linalg.norm(self.rvecs, axis=1)
+        return lengths, None
+
+
+class System:
+    def __init__(self, host, guest):
+        self.host = host
+        self.guest = guest
+
+    def copy(self):
+        return System(self.host, self.guest.copy())
+
+
+class Host:
+    def __init__(self, name, cell):
+        self.name = name
+        self.cell = cell if hasattr(cell, 'rvecs') else Cell(cell)
+
+
+class EmptyHost(Host):
+    """Host without atoms (system.py:79-96)."""
+
+    def __init__(self, name, cell=None, volume=None):
+        if cell is None:
+            assert volume is not None, 'Either cell or volume keyword argument must be defined'
+            cell = Cell(np.eye(3) * volume ** (1.0 / 3.0))
+        super().__init__(name, cell)
+
+
+class NanoporousHost(Host):
+    """Host framework given as a duck-typed molecular system ``mol`` with
+    ``pos``, ``ffatype_ids``, ``ffatypes`` and ``cell.rvecs`` (the attributes
+    the reference reads from a yaff System, system.py:53-76) and a force-field
+    dict {ffatype: (sigma, epsilon)}.  Positions are centred on their mean
+    (system.py:68-70)."""
+
+    def __init__(self, name, mol, ff):
+        self.mol = mol
+        pos = np.asarray(mol.pos, dtype=np.float64)
+        mol.pos = pos - pos.sum(axis=0) / len(pos)
+        self.ff = dict(ff)
+        super().__init__(name, mol.cell)
+
+
+class Guest:
+    def __init__(self, name, mass):
+        self.name = name
+        self.mass = mass
+        self.preset_Rhs = None
+        self.preset_Rhs_zero = None
+        self.Rhs = None
+        self.Rhs_zero = None
+
+    def copy(self):
+        return type(self)(self.name, self.mass)
+
+    def wavelength(self, temperature):                 # system.py:115-117
+        kT = U.boltzmann * temperature
+        return U.planck / np.sqrt(2 * np.pi * self.mass * kT)
+
+    def set_fixed_rhs(self, Rhs, Rhs_zero):
+        self.preset_Rhs = Rhs
+        self.preset_Rhs_zero = Rhs_zero
+
+    def _calculate_hardsphere_radius(self, temperature, **kwargs):
+        raise NotImplementedError
+
+    def compute_hardsphere_radius(self, temperature, **kwargs):
+        if self.preset_Rhs_zero is not None:
+            self.Rhs, self.Rhs_zero = self.preset_Rhs, self.preset_Rhs_zero
+        else:
+            self.Rhs, self.Rhs_zero = self._calculate_hardsphere_radius(temperature, **kwargs)
+
+
+class SphericalLJGuest(Guest):
+    """Spherical LJ guest; hard-sphere radius from the Barker-Henderson fit
+    R = sigma (1 + 0.2977 T*) / (1 + 0.33163 T* + 0.0010477 T*^2) / 2 with
+    T* = kT / epsilon (system.py:141-158)."""
+
+    def __init__(self, name, mass, sigma, epsilon):
+        super().__init__(name, mass)
+        self.sigma = sigma
+        self.epsilon = epsilon
+        self.natom = 1
+
+    def copy(self):
+        return type(self)(self.name, self.mass, self.sigma, self.epsilon)
+
+    def _calculate_hardsphere_radius(self, temperature, **kwargs):
+        Tt = U.boltzmann * temperature / self.epsilon
+        Rhs = self.sigma * (1 + 0.2977 * Tt) / (1 + 0.33163 * Tt + 0.0010477 * Tt ** 2) / 2
+        return Rhs, self.sigma
+
+
+class Grid:
+    """Periodic real-space grid centred at the origin (system.py:231-332)."""
+
+    def __init__(self, cell, npoints=None, spacing=0.25 * U.angstrom, silent=False):
+        self.cell = cell if hasattr(cell, 'rvecs') else Cell(cell)
+        rv = np.asarray(self.cell.rvecs, dtype=np.float64)
+        if npoints is None:
+            lengths = np.linalg.norm(rv, axis=1)
+            self.npoints = [int(np.ceil(l / spacing)) for l in lengths]
+        elif isinstance(npoints, (int, np.integer)):
+            self.npoints = [int(npoints)] * 3
+        else:
+            self.npoints = [int(n) for n in npoints]
+        self.suffix = '_'.join('%d' % n for n in self.npoints)
+        self.spacings = [np.linalg.norm(rv[:, a]) / self.npoints[a] for a in range(3)]
+        self.dr = self.cell.volume / np.prod(self.npoints)
+        self.dk = 1.0 / self.dr
+        self.npts = int(np.prod(self.npoints))
+        self._plans = {}
+        self._cache = {}
+
+    # -- reference attributes, built on demand ---------------------------------
+    def _frac(self):
+        return [np.linspace(-0.5, 0.5, num=n, endpoint=False) for n in self.npoints]
+
+    @property
+    def points(self):
+        if 'points' not in self._cache:
+            fg = np.asarray(np.meshgrid(*self._frac(), indexing='ij'))
+            pts = np.zeros(self.npoints + [4])
+            pts[..., :3] = np.einsum('ab,aijk->ijkb', self.cell.rvecs, fg)
+            pts[..., 3] = np.sqrt(pts[..., 0] ** 2 + pts[..., 1] ** 2 + pts[..., 2] ** 2)
+            self._cache['points'] = pts
+        return self._cache['points']
+
+    def k1d(self, axis):
+        return 2 * np.pi * np.fft.fftfreq(self.npoints[axis], d=self.spacings[axis])
+
+    @property
+    def kpoints(self):
+        if 'kpoints' not in self._cache:
+            kg = np.meshgrid(*[self.k1d(a) for a in range(3)], indexing='ij')
+            kp = np.zeros(self.npoints + [4])
+            for a in range(3):
+                kp[..., a] = kg[a]
+            kp[..., 3] = np.sqrt(kp[..., 0] ** 2 + kp[..., 1] ** 2 + kp[..., 2] ** 2)
+            self._cache['kpoints'] = kp
+        return self._cache['kpoints']
+
+    @property
+    def scalprod(self):
+        kp = self.kpoints
+        return sum(kp[..., a] * self.spacings[a] * self.npoints[a] for a in range(3))
+
+    @property
+    def sigma_lanczos(self):
+        kcut = 2 * np.pi / np.array(self.spacings)
+        kp = self.kpoints
+        return np.sinc(kp[..., 0] / kcut[0]) * np.sinc(kp[..., 1] / kcut[1]) * np.sinc(kp[..., 2] / kcut[2])
+
+    def supercell(self, supercell):
+        supercell = np.asarray(supercell)
+        return Grid(Cell(self.cell.rvecs * supercell), npoints=list(np.asarray(self.npoints) * supercell))
+
+    def copy(self):
+        return Grid(self.cell, npoints=self.npoints)
+
+    # -- device plan ------------------------------------------------------------
+    def plan(self, nbatch=1):
+        from ._lib import Plan
+        key = int(nbatch)
+        if key not in self._plans:
+            self._plans[key] = Plan(self.npoints, self.cell.rvecs, nbatch=key)
+        p = self._plans[key]
+        p.bind_stream()
+        return p
+
+    def phase_half(self):
+        """P = exp(i pi scalprod) on the half spectrum, as a device tensor."""
+        torch = torch_mod()
+        if 'phase_half' not in self._cache:
+            nzh = self.npoints[2] // 2 + 1
+            sp = [self.k1d(a) * self.spacings[a] * self.npoints[a] for a in range(3)]
+            s = sp[0][:, None, None] + sp[1][None, :, None] + sp[2][None, None, :nzh]
+            self._cache['phase_half'] = torch.from_numpy(np.exp(1j * np.pi * s)).to('cuda')
+        return self._cache['phase_half']
+
+    # -- transforms (reference convention) --------------------------------------
+    def integrate(self, data):                          # system.py:307-308
+        if hasattr(data, 'is_cuda'):
+            return float(data.sum()) * self.dr
+        return np.sum(data) * self.dr
+
+    def rfft_native(self, rho):
+        """Unnormalised R2C half spectrum of a device density (complex128)."""
+        torch = torch_mod()
+        nzh = self.npoints[2] // 2 + 1
+        x = as_device(rho).reshape(self.npoints).contiguous()
+        X = torch.empty(tuple(self.npoints[:2]) + (nzh,), dtype=torch.complex128, device=x.device)
+        self.plan().fft_r2c(x, torch.view_as_real(X))
+        return X
+
+    def irfft_native(self, X):
+        """Unnormalised C2R of a device half spectrum (real part semantics)."""
+        torch = torch_mod()
+        X = X.contiguous()
+        out = torch.empty(tuple(self.npoints), dtype=torch.float64, device=X.device)
+        self.plan().fft_c2r(torch.view_as_real(X), out)
+        return out
+
+    def fft(self, rdata):                               # system.py:310-311
+        torch = torch_mod()
+        X = self.rfft_native(rdata) * self.phase_half() / self.npts
+        full = _hermitian_full(X, self.npoints, self)
+        return full.cpu().numpy() if isinstance(rdata, np.ndarray) else full
+
+    def ifft(self, fdata):                              # system.py:322-323
+        torch = torch_mod()
+        nzh = self.npoints[2] // 2 + 1
+        F = as_device(fdata)[..., :nzh].to(torch.complex128)
+        rho = self.irfft_native((F * torch.conj(self.phase_half())).contiguous())
+        return rho.cpu().numpy() if isinstance(fdata, np.ndarray) else rho
+
+    def fftn(self, rdata):                              # system.py:313-320
+        comps = [self.fft(rdata[..., i]) for i in range(rdata.shape[-1])]
+        return np.stack(comps, axis=-1) if isinstance(rdata, np.ndarray) else torch_mod().stack(comps, dim=-1)
+
+    def ifftn(self, fdata):                             # system.py:325-332
+        comps = [self.ifft(fdata[..., i]) for i in range(fdata.shape[-1])]
+        return np.stack(comps, axis=-1) if isinstance(fdata, np.ndarray) else torch_mod().stack(comps, dim=-1)
+
+
+def _hermitian_full(Xh, npoints, grid):
+    """Full spectrum from the half spectrum of a real field (device).  The
+    phase-weighted half spectrum is extended by X(-k) = conj(X(k)) P(k)^2 ...
+    computed here directly: full(k) = conj(Xu(-k)) P(k) / N for kz beyond the
+    half, Xu being the unphased spectrum."""
+    torch = torch_mod()
+    nx, ny, nz = npoints
+    nzh = nz // 2 + 1
+    P = grid.phase_half()
+    Xu = Xh / P * grid.npts                  # unphased, unnormalised half spectrum
+    full = torch.empty((nx, ny, nz), dtype=torch.complex128, device=Xh.device)
+    full[..., :nzh] = Xh
+    if nz - nzh > 0:
+        kz = torch.arange(nzh, nz, device=Xh.device)
+        mz = (nz - kz) % nz
+        ix = (-torch.arange(nx, device=Xh.device)) % nx
+        iy = (-torch.arange(ny, device=Xh.device)) % ny
+        Xm = torch.conj(Xu[ix][:, iy][:, :, mz])
+        sp = [grid.k1d(a) * grid.spacings[a] * grid.npoints[a] for a in range(3)]
+        s = sp[0][:, None, None] + sp[1][None, :, None] + sp[2][None, None, nzh:]
+        Pf = torch.from_numpy(np.exp(1j * np.pi * s)).to(Xh.device)
+        full[..., nzh:] = Xm * Pf / grid.npts
+    return full

@@ -1,0 +1,158 @@
+/*
+ * C ABI of libcmmdft_hip.so: the MI355X (gfx950) implementation of the cDFT
+ * hot path of CmmDFT (FMT/WDA weighted densities and their back-convolution,
+ * the Picard/Anderson Euler-Lagrange update, the LJ external-potential grid).
+ *
+ * Conventions
+ *  - All arrays are device pointers (e.g. torch.Tensor.data_ptr()), FP64,
+ *    C-contiguous.  A real field is double[nbatch][nx][ny][nz]; a half
+ *    spectrum is double2 (interleaved re, im) [nbatch][nx][ny][nz/2+1].
+ *  - Lengths in bohr, energies in hartree (atomic units).
+ *  - Every call returns a status: CMM_OK, or an error whose text is given by
+ *    cmm_last_error() (thread-local).  CMM_E_ARG maps to ValueError,
+ *    CMM_E_NONFINITE to FloatingPointError, CMM_E_HIP to RuntimeError.
+ *  - Work is enqueued on the plan's stream (cmm_plan_set_stream); only
+ *    cmm_get_scalars and cmm_synchronize wait for it.  The library never
+ *    frees caller memory.  A plan is not thread-safe: one per stream.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *  cmm_plan_create       Grid.__init__                 cmmdft/system.py:231-295
+ *  cmm_fft_r2c/_c2r      Grid.fft / Grid.ifft          cmmdft/system.py:310-332
+ *  cmm_set_functional    HardSphereFunctional.__init__/set_temperature,
+ *                        WDAVFunctional.set_temperature
+ *                                                      cmmdft/functionals.py:384-487, 1133-1155
+ *  cmm_derive            HardSphereFunctional.derive + WDAVFunctional.derive
+ *                        (+ get_phi / EOS f_ex for value())
+ *                                                      cmmdft/functionals.py:489-618, 620-767, 1157-1184
+ *  cmm_weighted_densities HardSphereFunctional._get_density_functions,
+ *                        WDAVFunctional._get_weighted_density
+ *                                                      cmmdft/functionals.py:489-532, 1157-1158
+ *  cmm_finish_c1         Solver._get_C1                cmmdft/solver.py:137-144
+ *  cmm_finish_picard     Picard.update_rho_static      cmmdft/solver.py:392-401 (+ get_new_rho :128-129)
+ *  cmm_finish_anderson   Anderson.update_rho (residual + history)  cmmdft/solver.py:527-535, 546-557
+ *  cmm_anderson_coef     Anderson.update_rho_Anderson (SLSQP simplex LSQ) and
+ *                        _get_damping_coefficient      cmmdft/solver.py:537-544, 581-597
+ *  cmm_anderson_mix      Anderson.update_rho_Anderson (mixing, clip)  cmmdft/solver.py:594-604
+ *  cmm_trial             Picard.update_rho_hybrid trial densities     cmmdft/solver.py:414-441, 451-452
+ *  cmm_seed              Grid.fft of a density + Solver._get_Omega ideal/N terms
+ *                                                      cmmdft/solver.py:103-119
+ *  cmm_lj_grid/_points   get_external_potential / lennard_jones
+ *                                                      cmmdft/extpot_calculator.py:156-222
+ */
+#ifndef CMMDFT_HIP_H
+#define CMMDFT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CMM_OK 0
+#define CMM_E_ARG 1
+#define CMM_E_HIP 2
+#define CMM_E_NONFINITE 3
+#define CMM_E_UNSUPPORTED 4
+
+#define CMM_NSCAL 32      /* doubles of reduction results per instance */
+#define CMM_MMAX_HIST 8   /* longest Anderson history */
+
+typedef struct cmm_plan cmm_plan;
+
+/* Functional definition.  hs_base: -1 none, 0 Rosenfeld FMT, 1 MFMT,
+ * 2 White-Bear II; hs_antisym: the 'a' variant.  WDA EOS terms: kind 0 mean
+ * field (par[0] = a), 1 Carnahan-Starling (par = {eta, kT}), 2 MBWR
+ * (par = {a1..a8, b1..b6, sigma^3, epsilon}); factor multiplies the term. */
+typedef struct {
+  int hs_base;
+  int hs_antisym;
+  int use_wda;
+  int n_eos;
+  int eos_kind[4];
+  double eos_factor[4];
+  double eos_par[4][16];
+  double R;    /* hard-sphere radius */
+  double D;    /* WDA weight diameter */
+  double kT;   /* 1 / beta */
+} cmm_functional;
+
+int cmm_version(void);
+const char* cmm_last_error(void);
+
+/* Grid plan for nbatch instances of an nx*ny*nz grid in the cell whose rows
+ * are rvecs[0..2], rvecs[3..5], rvecs[6..8].  Owns the FFT tables, the
+ * k-space tables and the spectral workspace (7 half spectra per instance). */
+int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm_plan** out);
+int cmm_plan_destroy(cmm_plan* plan);
+int cmm_plan_set_stream(cmm_plan* plan, void* stream);
+/* tile shapes: x-pass columns, y-pass columns, z-pass pencils (simple),
+ * z-pass pencils (6-field functional pass); 0 keeps the current value */
+int cmm_plan_set_tiles(cmm_plan* plan, int tb_x, int tb_y, int p_z, int p_phi);
+/* info[0..9] = nx, ny, nz, nbatch, tb_x, tb_y, p_z, p_phi, workspace bytes, nzh */
+int cmm_plan_get_info(cmm_plan* plan, long long* info, int n);
+/* device int[nbatch] mask of instances to process (NULL = all) */
+int cmm_plan_set_active(cmm_plan* plan, const int* dev_mask);
+int cmm_set_functional(cmm_plan* plan, const cmm_functional* f);
+/* per-instance fugacity and mixing parameter (host arrays of nbatch) */
+int cmm_set_instances(cmm_plan* plan, const double* fug, const double* alpha);
+
+/* unnormalised 3-D transforms over the batch (X: interleaved complex) */
+int cmm_fft_r2c(cmm_plan* plan, const double* x, double* X);
+int cmm_fft_c2r(cmm_plan* plan, const double* X, double* x);
+
+/* R2C of a density into the plan workspace; reduces N, ideal-gas and
+ * external sums of rho (scalars S_PN..).  vext may be NULL; vext_batched
+ * says whether vext has one field per instance. */
+int cmm_seed(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double lam3);
+/* functional chain on the seeded density: weighted densities, pointwise
+ * Phi/WDA derivatives (reduces sum Phi, sum f_ex, max n3), back-convolution;
+ * leaves the x,y-transformed derivative sum in the workspace */
+int cmm_derive(cmm_plan* plan);
+/* value-only chain (no back-convolution): reduces sum Phi, sum f_ex, max n3 */
+int cmm_value(cmm_plan* plan);
+/* weighted densities n2, n3, nv2x, nv2y, nv2z, rho_bar of the seeded density
+ * into out[nbatch][6][npts] (n3 unclipped) */
+int cmm_weighted_densities(cmm_plan* plan, double* out);
+
+/* final passes after cmm_derive; rho is the density the derivative belongs to */
+int cmm_finish_c1(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double* c1,
+                  double lam3);
+int cmm_finish_picard(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double* rho_out,
+                      double* c1_out, double beta, double lam3, double zero_thr);
+int cmm_finish_anderson(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double* rhist,
+                        int m, int slot_new, double* c1_out, double beta, double lam3);
+
+/* Anderson coefficients on the device for the mk newest residual slots
+ * order[0..mk) (oldest first); reads this step's Gram row from the scalars. */
+int cmm_anderson_init(cmm_plan* plan, const double* damping, const double* cf);
+int cmm_anderson_coef(cmm_plan* plan, int m, int mk, const int* order, int slot_new, int slot_prev, int adaptive,
+                      double fac_up, double fac_down, double dmin, double dmax);
+int cmm_anderson_mix(cmm_plan* plan, const double* rhohist, const int* order_rho, const double* rhist,
+                     const int* order, int mk, const double* rho_cur, double* rho_out, const double* vext,
+                     int vext_batched, double lam3);
+/* host copies of the per-instance damping and the last coefficients
+ * (alpha[nbatch][CMM_MMAX_HIST]) */
+int cmm_anderson_state(cmm_plan* plan, double* damping, double* alpha);
+/* rho_out = rho + alpha_inst * r_slot (clip to 1e-30 below 1e-12 if clip) */
+int cmm_trial(cmm_plan* plan, const double* rho, const double* rhist, int slot, double* rho_out,
+              const double* vext, int vext_batched, double lam3, int clip);
+
+/* copy scalars [nbatch][CMM_NSCAL] to host (waits for the stream) */
+int cmm_get_scalars(cmm_plan* plan, double* host_out);
+int cmm_synchronize(cmm_plan* plan);
+
+/* LJ external potential on the nx*ny*nz grid of the cell rvecs (points
+ * frac . rvecs, frac in [-1/2, 1/2)).  atoms: natom x {x, y, z, sigma_i,
+ * epsilon_i} (host parameters before mixing); guest sigma_f, epsilon_f;
+ * Lorentz-Berthelot mixing on the device.  images = 0: minimum image with row
+ * norms (the reference rule); 1: every periodic image within the cutoff
+ * (orthorhombic cells).  limit > 0 clamps non-finite and larger values to
+ * limit.  Writes V[nx*ny*nz]. */
+int cmm_lj_grid(int nx, int ny, int nz, const double* rvecs, const double* atoms, int natom, double sigma_f,
+                double epsilon_f, double cutoff, int images, double limit, double* V, void* stream);
+/* the same sum at M explicit points (points: M x 3, device), V[M] */
+int cmm_lj_points(const double* points, long long M, const double* rvecs, const double* atoms, int natom,
+                  double sigma_f, double epsilon_f, double cutoff, int images, double limit, double* V, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

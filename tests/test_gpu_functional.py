@@ -1,0 +1,125 @@
+"""Fused HIP functional chain (weighted densities, Phi derivatives,
+back-convolution) against the oracle's restatement of
+HardSphereFunctional / WDAVFunctional (cmmdft/functionals.py)."""
+import numpy as np
+import pytest
+
+from cmmdft_amd import units as U, workloads as W
+from oracle import cdft_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+T = 300.0
+
+
+def _density(og, host, seed=3):
+    """A structured density: Boltzmann factor of the host potential times the
+    bulk density, with a seeded smooth perturbation."""
+    sig, eps = host.atom_params()
+    V = O.external_potential(og.points[..., :3], sig, eps, host.pos, host.rvecs, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    V = V.reshape(og.npoints)
+    rb = 2.4e-4 / U.angstrom ** 3
+    beta = 1 / (U.boltzmann * T)
+    rho = rb * np.exp(-beta * np.clip(V, -50 * U.kjmol, 50 * U.kjmol))
+    rng = np.random.default_rng(seed)
+    rho *= 1 + 0.1 * rng.random(og.npoints)
+    return rho, V
+
+
+def _product(shape, cell):
+    from cmmdft_amd.system import Grid
+    return Grid(cell, shape)
+
+
+SHAPES = [(24, 24, 24), (20, 18, 25), (16, 15, 12)]
+
+
+@pytest.mark.parametrize('version', ['FMT', 'WBII', 'aWBII', 'MFMT', 'aFMT'])
+@pytest.mark.parametrize('shape', SHAPES)
+def test_hard_sphere_derive_and_value(version, shape):
+    import torch
+    from cmmdft_amd.functionals import HardSphereFunctional
+    host = W.host_case1()
+    og = O.Grid(host.rvecs, shape)
+    rho, _ = _density(og, host)
+    R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
+    ohs = O.HardSphere(og, R, version, T)
+    krho = og.fft(rho)
+    c_ref = ohs.derive(krho)
+    F_ref = ohs.value(krho)
+    g = _product(shape, host.rvecs)
+    hs = HardSphereFunctional(g, R, version)
+    hs.set_temperature(T, R)
+    c = hs.derive(torch.from_numpy(rho).cuda()).cpu().numpy()
+    scale = np.abs(c_ref).max()
+    # The WBII/MFMT phi_2, phi_3 forms cancel to O(n3^2), O(n3^3) in their
+    # numerators (functionals.py:730-767), so both sides carry a rounding error
+    # ~ eps / n3^2 there; 1e-9 of max|c| bounds it with margin (observed ~1e-11).
+    tol = 1e-12 if version in ('FMT', 'aFMT') else 1e-9
+    assert np.abs(c - c_ref).max() <= tol * scale, (np.abs(c - c_ref).max(), scale)
+    # reference-convention spectrum as input
+    c2 = hs.derive(krho)
+    assert np.abs(c2 - c_ref).max() <= tol * scale
+    F = hs.value(torch.from_numpy(rho).cuda())
+    assert abs(F - F_ref) <= 1e-11 * abs(F_ref)
+
+
+@pytest.mark.parametrize('shape', SHAPES)
+def test_weighted_densities(shape):
+    import torch
+    from cmmdft_amd.functionals import HardSphereFunctional
+    host = W.host_case1()
+    og = O.Grid(host.rvecs, shape)
+    rho, _ = _density(og, host)
+    R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
+    ohs = O.HardSphere(og, R, 'WBII', T)
+    krho = og.fft(rho)
+    n3_ref = og.ifft(krho * ohs.w['scalar'][3])
+    n2_ref = og.ifft(krho * ohs.w['scalar'][2])
+    nv2_ref = og.ifftn(krho[..., None] * ohs.w['vector'][1])
+    hs = HardSphereFunctional(_product(shape, host.rvecs), R, 'WBII')
+    hs.set_temperature(T, R)
+    n = hs.weighted_densities_dev(torch.from_numpy(rho).cuda()).cpu().numpy()
+    for got, ref in [(n[0], n2_ref), (n[1], n3_ref), (n[2], nv2_ref[..., 0]), (n[3], nv2_ref[..., 1]),
+                     (n[4], nv2_ref[..., 2])]:
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _eos(kind, R):
+    m, s, e = W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON
+    from cmmdft_amd import eos as E
+    if kind == 'MFA':
+        return O.MFAEOS(m, s, e), E.MFAEOS(m, s, e)
+    if kind == 'CS':
+        return O.CarnahanStarlingEOS(m, R), E.CarnahanStarlingEOS(m, R)
+    if kind == 'MBWR':
+        return O.MBWREOS(m, s, e), E.ModifiedBenedictWebbRubinEOS(m, s, e)
+    o = O.SumOfEOS(m, [O.MBWREOS(m, s, e), O.CarnahanStarlingEOS(m, R), O.MFAEOS(m, s, e)], [1, -1, -1])
+    p = E.SumOfEOS(m, [E.ModifiedBenedictWebbRubinEOS(m, s, e), E.CarnahanStarlingEOS(m, R), E.MFAEOS(m, s, e)],
+                   [1, -1, -1])
+    return o, p
+
+
+@pytest.mark.parametrize('kind', ['MFA', 'CS', 'MBWR', 'corr'])
+def test_wda_derive_and_value(kind):
+    import torch
+    from cmmdft_amd.functionals import WDAVFunctional
+    shape = (24, 20, 18)
+    host = W.host_case1()
+    og = O.Grid(host.rvecs, shape)
+    rho, _ = _density(og, host)
+    R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
+    oe, pe = _eos(kind, R)
+    ow = O.WDAV(og, R, oe, T)
+    krho = og.fft(rho)
+    c_ref = ow.derive(krho)
+    F_ref = ow.value(krho)
+    w = WDAVFunctional(_product(shape, host.rvecs), R, pe)
+    w.set_temperature(T, R)
+    c = w.derive(torch.from_numpy(rho).cuda()).cpu().numpy()
+    assert np.abs(c - c_ref).max() <= 1e-11 * np.abs(c_ref).max()
+    F = w.value(torch.from_numpy(rho).cuda())
+    assert abs(F - F_ref) <= 1e-11 * abs(F_ref)
+    wd_ref = ow.weighted_density(krho)
+    wd = w._get_weighted_density(torch.from_numpy(rho).cuda()).cpu().numpy()
+    assert np.abs(wd - wd_ref).max() <= 1e-12 * np.abs(wd_ref).max()
