@@ -67,9 +67,10 @@ class NanoporousHost(Host):
     (system.py:68-70)."""
 
     def __init__(self, name, mol, ff):
-        self.mol = mol
+        import copy
+        self.mol = copy.copy(mol)
         pos = np.asarray(mol.pos, dtype=np.float64)
-        mol.pos = pos - pos.sum(axis=0) / len(pos)
+        self.mol.pos = pos - pos.sum(axis=0) / len(pos)
         self.ff = dict(ff)
         super().__init__(name, mol.cell)
 

@@ -52,14 +52,20 @@ def test_hard_sphere_derive_and_value(version, shape):
     hs.set_temperature(T, R)
     c = hs.derive(torch.from_numpy(rho).cuda()).cpu().numpy()
     scale = np.abs(c_ref).max()
-    # The WBII/MFMT phi_2, phi_3 forms cancel to O(n3^2), O(n3^3) in their
-    # numerators (functionals.py:730-767), so both sides carry a rounding error
-    # ~ eps / n3^2 there; 1e-9 of max|c| bounds it with margin (observed ~1e-11).
-    tol = 1e-12 if version in ('FMT', 'aFMT') else 1e-9
-    assert np.abs(c - c_ref).max() <= tol * scale, (np.abs(c - c_ref).max(), scale)
+    # Rosenfeld/aFMT: well conditioned, bound relative to max|c|.  WBII/MFMT:
+    # the reference's phi_2/phi_3 forms cancel to O(n3^2)/O(n3^3) in their
+    # numerators (functionals.py:730-767), so at low n3 a roundoff-level change
+    # of n3 moves c by ~1e-9 relative (test_oracle_conditioning shows it on the
+    # CPU).  There the bound is on beta*|dc|, the relative error it puts on the
+    # Boltzmann factor exp(-beta c) of the Euler-Lagrange update.
+    if version in ('FMT', 'aFMT'):
+        tol = 1e-12 * scale
+    else:
+        tol = 1e-8 * U.boltzmann * T
+    assert np.abs(c - c_ref).max() <= tol, (np.abs(c - c_ref).max(), scale)
     # reference-convention spectrum as input
     c2 = hs.derive(krho)
-    assert np.abs(c2 - c_ref).max() <= tol * scale
+    assert np.abs(c2 - c_ref).max() <= tol
     F = hs.value(torch.from_numpy(rho).cuda())
     assert abs(F - F_ref) <= 1e-11 * abs(F_ref)
 
