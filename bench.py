@@ -1,0 +1,207 @@
+#!/usr/bin/env python
+"""Benchmark: converged isotherm points per second at 128^3 on one MI355X.
+
+Workload: the 128^3 host (cubic a = 25.6 A, 276 LJ atoms of 4 types, seeded),
+united-atom methane at 300 K, White-Bear II FMT + mean-field WDA + LJ external
+potential, Anderson mixing (m = 5, damping 0.1, RIUE < 1e-8), over the
+isotherm of 32 pressures log-spaced 1e-2..1e2 bar (mu from the MBWR EOS).
+A "step" is one converged isotherm point (one full Euler-Lagrange solve from
+the ideal-gas initial density); the external potential is computed before the
+timed region and reported separately.
+
+Prints one JSON line (rank 0).  Default: N = 1, 8 timed points, 1 warm-up.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'isotherm points/s at 128^3, batched vs plain; HBM GB/s; adjoint/fwd time'
+PEAK_HBM_GBPS = 8000.0
+NSIDE = 128
+
+
+def build_program(shape):
+    from cmmdft_amd import units as U, workloads as W
+    from cmmdft_amd.program import Program
+    from cmmdft_amd.system import NanoporousHost, SphericalLJGuest
+    from cmmdft_amd.eos import MFAEOS
+    host = W.host_case2()
+    prog = Program(write_files=False)
+    prog.set_system(NanoporousHost('case2', host, host.ff),
+                    SphericalLJGuest('CH4_UA', W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON))
+    prog.set_grid(npoints=list(shape))
+    prog.init_free_energy(W.TEMPERATURE)
+    prog.fener.add_hard_sphere('WBII')
+    prog.fener.add_wdav(MFAEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON))
+    return host, prog
+
+
+def chempots(n=32):
+    from cmmdft_amd import workloads as W
+    from cmmdft_amd.eos import ModifiedBenedictWebbRubinEOS
+    eos = ModifiedBenedictWebbRubinEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    return np.array([eos.calculate_mu(W.TEMPERATURE, p) for p in W.pressures_isotherm(n)])
+
+
+PASS_NAMES = {0: 'k_zphi', 1: 'k_xweights', 2: 'k_xback', 3: 'k_ypass(6 fields)'}
+
+
+def pass_bytes(nx, ny, nz):
+    """Algorithmic HBM bytes per launch of each functional pass (one
+    instance): every complex half-spectrum element read or written once."""
+    nh = nx * ny * (nz // 2 + 1) * 16
+    return {0: 12 * nh, 1: 7 * nh, 2: 7 * nh, 3: 12 * nh}
+
+
+def time_passes(plan, reps=20):
+    """Average duration of each pass, HIP events on the plan's stream."""
+    import torch
+    out = {}
+    for which in PASS_NAMES:
+        plan.launch_pass(which)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(plan.stream)
+        for _ in range(reps):
+            plan.launch_pass(which)
+        e1.record(plan.stream)
+        e1.synchronize()
+        out[which] = e0.elapsed_time(e1) / reps
+    return out
+
+
+def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample):
+    """The oracle (NumPy/SciPy restatement of the reference path, scipy.fft
+    with one worker) timed for ``nsample`` Anderson iterations at the same
+    grid and potential; extrapolated to points/s with the GPU's mean
+    iteration count per point."""
+    from threadpoolctl import threadpool_limits
+    from cmmdft_amd import units as U, workloads as W
+    from oracle import cdft_oracle as O
+    host = W.host_case2()
+    T = W.TEMPERATURE
+    with threadpool_limits(limits=1):
+        og = O.Grid(host.rvecs, shape)
+        R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
+        lam = O.thermal_wavelength(W.GUEST_MASS, T)
+        beta = 1 / (U.boltzmann * T)
+        parts = [O.HardSphere(og, R, 'WBII', T),
+                 O.WDAV(og, R, O.MFAEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON), T), O.ExtPot(og, V)]
+        stamps = []
+
+        class Timed(O.Anderson):
+            def converged(self, *a, **k):
+                stamps.append(time.perf_counter())
+                return super().converged(*a, **k)
+
+        s = Timed(og, parts, beta, lam, nsteps=nsample + 1, method='anderson', m=5, damping=0.1,
+                  criterion='RIUE', threshold=1e-30)
+        rho0 = O.initial_density(og, mu, beta, lam, V=parts[2].potential)
+        s.solve(mu, rho0)
+    per_iter = float(np.median(np.diff(stamps))) if len(stamps) > 1 else float('nan')
+    pts = 1.0 / (per_iter * gpu_iters_per_point)
+    return {'value': pts, 'unit': 'points/s', 'cores': 1, 'kind': 'port',
+            'sample': ('oracle Anderson (SLSQP mixing, scipy.fft 1 worker) at %dx%dx%d, %d iterations timed, median '
+                       '%.3f s/iteration, extrapolated with the GPU mean of %.1f iterations/point'
+                       % (shape + (nsample, per_iter, gpu_iters_per_point))),
+            'seconds_per_iteration': per_iter}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=8)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--grid', type=int, default=NSIDE)
+    ap.add_argument('--cpu-sample', type=int, default=4, help='oracle iterations timed (0 skips the CPU leg)')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported'
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from cmmdft_amd import units as U, workloads as W
+    from cmmdft_amd.solver import Anderson
+    shape = (args.grid,) * 3
+    mus = chempots(32)
+    host, prog = build_program(shape)
+    torch.cuda.synchronize()
+    tv = time.perf_counter()
+    prog.fener.add_external_potential(cutoff=W.CUTOFF)
+    torch.cuda.synchronize()
+    vext_ms = (time.perf_counter() - tv) * 1e3
+    solver = Anderson(prog, nsteps=500, method='anderson', m=5, damping=0.1, criterion='RIUE', threshold=1e-8)
+    prog.set_solver(solver)
+
+    def solve_point(i):
+        mu = mus[i % len(mus)]
+        rho0 = prog._set_initial_density(chempot=mu, temperature=W.TEMPERATURE)
+        N, rho, conv = solver.solve(mu, rho0)
+        return solver.curr_step, conv, N
+
+    for i in range(args.warmup):
+        solve_point(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    iters, convs, loads = [], [], []
+    for k in range(args.steps):
+        it, cv, N = solve_point(k)
+        iters.append(it)
+        convs.append(cv)
+        loads.append(N)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    points_per_s = args.steps / dt
+    iters_per_point = float(np.mean(iters))
+
+    # dominant pass: per-launch time x launches per iteration (2 y passes of 6 fields)
+    plan = prog.grid.plan()
+    tp = time_passes(plan)
+    per_iter_weight = {0: 1, 1: 1, 2: 1, 3: 2}
+    dom = max(tp, key=lambda w: tp[w] * per_iter_weight[w])
+    pb = pass_bytes(*shape)
+    achieved = pb[dom] / (tp[dom] * 1e-3) / 1e9
+    npts = int(np.prod(shape))
+    b_iter = npts * (16 * 14 + 24) + npts * 8 * (3 * 5 + 2)
+    iter_gbps = b_iter * sum(iters) / dt / 1e9
+
+    cpu = None
+    if args.cpu_sample > 0:
+        V = prog.fener.part_dict['ExtPot'].potential
+        cpu = cpu_baseline(V, shape, mus[0], iters_per_point, args.cpu_sample)
+
+    line = {
+        'metric': METRIC, 'value': points_per_s, 'unit': 'points/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': dt / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
+        'config': {'workload': 'plain GPU drop-in: case-2 host (a=25.6 A, 276 LJ atoms) at %dx%dx%d, WBII FMT + '
+                               'MFA-WDA + LJ ext. potential, Anderson m=5 damping 0.1 RIUE<1e-8, isotherm of 32 '
+                               'pressures 1e-2..1e2 bar, one point per step' % shape,
+                   'grid': list(shape), 'solver': 'anderson', 'mode': 'plain'},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': PEAK_HBM_GBPS, 'unit': 'GB/s',
+                     'frac': achieved / PEAK_HBM_GBPS, 'traffic': None, 'kernel': PASS_NAMES[dom],
+                     'avg_launch_ms': tp[dom], 'algorithmic_bytes_per_launch': pb[dom]},
+        'cpu_baseline': cpu,
+        'plain_points_per_s': points_per_s,
+        'iterations_per_point': iters_per_point,
+        'ms_per_iteration': dt * 1e3 / sum(iters),
+        'iteration_algorithmic_GBps': iter_gbps,
+        'all_converged': bool(all(convs)),
+        'vext_ms': vext_ms,
+        'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

@@ -51,6 +51,7 @@ EXPORTS = {
     'cmm_derive': ([C.c_void_p], C.c_int),
     'cmm_value': ([C.c_void_p], C.c_int),
     'cmm_weighted_densities': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_launch_pass': ([C.c_void_p, C.c_int], C.c_int),
     'cmm_finish_c1': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_double], C.c_int),
     'cmm_finish_picard': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                            C.c_double, C.c_double], C.c_int),
@@ -190,6 +191,9 @@ class Plan:
 
     def value(self):
         check(self.lib.cmm_value(self.h))
+
+    def launch_pass(self, which):
+        check(self.lib.cmm_launch_pass(self.h, int(which)))
 
     def weighted_densities(self, out):
         check(self.lib.cmm_weighted_densities(self.h, _ptr(out)))

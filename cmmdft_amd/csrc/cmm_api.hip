@@ -436,6 +436,30 @@ int cmm_value(cmm_plan* p) {
   return launch_zphi(p, 0);
 }
 
+int cmm_launch_pass(cmm_plan* p, int which) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
+  const long long nb = (long long)p->ny * p->nzh;
+  dim3 gx((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
+  size_t ldsx = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
+  switch (which) {
+    case 0: return launch_zphi(p, 1);
+    case 1:
+      hipLaunchKernelGGL(k_xweights, gx, dim3(NT), ldsx, p->stream, p->rz, p->nhalf, p->F, 6 * p->nhalf, p->nhalf,
+                         p->g, p->fp, p->tbx, p->active);
+      LAUNCHCHK("k_xweights");
+      return CMM_OK;
+    case 2:
+      hipLaunchKernelGGL(k_xback, gx, dim3(NT), ldsx, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
+                         p->active);
+      LAUNCHCHK("k_xback");
+      return CMM_OK;
+    case 3: return launch_ypass(p, p->F, 6, -1);
+    case 4: return launch_ypass(p, p->rz, 1, -1);
+    default: return fail(CMM_E_ARG, "unknown pass");
+  }
+}
+
 int cmm_weighted_densities(cmm_plan* p, double* out) {
   if (!p || !out) return fail(CMM_E_ARG, "null argument");
   int rc = forward_chain(p);

@@ -1,0 +1,369 @@
+"""Pins the oracle (oracle/cdft_oracle.py) independently of its own formulas:
+brute-force DFTs, independent library routines (scipy.special spherical Bessel
+functions), closed-form bulk limits of the FMT/WDA functionals (Percus-Yevick
+and Carnahan-Starling), finite-difference consistency of derive() with
+value(), analytic LJ values, the ideal-gas and bulk Euler-Lagrange fixed
+points, and the EOS root.  The reference ships no tests or golden vectors for
+this path, so these checks are what fixes the oracle.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+import scipy.special as sp
+from scipy.optimize import brentq
+
+from cmmdft_amd import units as U, workloads as W
+from oracle import cdft_oracle as O
+
+T = 300.0
+R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
+LAM = O.thermal_wavelength(W.GUEST_MASS, T)
+BETA = 1 / (U.boltzmann * T)
+
+
+def brute_dft(x):
+    """O(N^2) 3-D DFT, sum_n x_n exp(-2 pi i k.n / N)."""
+    shp = x.shape
+    idx = [np.arange(n) for n in shp]
+    out = np.zeros(shp, dtype=complex)
+    g = np.meshgrid(*idx, indexing='ij')
+    for k in np.ndindex(*shp):
+        ph = sum(k[a] * g[a] / shp[a] for a in range(3))
+        out[k] = np.sum(x * np.exp(-2j * np.pi * ph))
+    return out
+
+
+@pytest.mark.parametrize('shape', [(4, 3, 5), (6, 4, 2)])
+def test_grid_fft_is_phased_dft(shape):
+    g = O.Grid(np.diag([7.0, 5.0, 9.0]), shape)
+    x = np.random.default_rng(0).standard_normal(shape)
+    ref = brute_dft(x) * g._phase / g.npts
+    assert np.allclose(g.fft(x), ref, atol=1e-13)
+    assert np.allclose(g.ifft(g.fft(x)), x, atol=1e-13)
+
+
+def test_phase_is_not_sign_alternation():
+    """P = exp(i 2 pi^2 m), not (-1)^m (the reference's scalprod convention)."""
+    g = O.Grid(np.eye(3) * 10.0, (6, 6, 6))
+    m = np.fft.fftfreq(6, 1 / 6)
+    assert np.allclose(g.scalprod[:, 0, 0], 2 * np.pi * m)
+    assert not np.allclose(g._phase[:, 0, 0], (-1.0) ** m)
+
+
+def test_convolution_phase_cancels():
+    """ifft(fft(x) w) is the plain circular convolution with the kernel whose
+    unphased DFT is w: brute force on a small grid."""
+    shape = (5, 4, 6)
+    g = O.Grid(np.diag([4.0, 3.0, 5.0]), shape)
+    rng = np.random.default_rng(1)
+    x, k = rng.standard_normal(shape), rng.standard_normal(shape)
+    w = np.fft.fftn(k)
+    got = g.ifft(g.fft(x) * w)
+    ref = np.zeros(shape)
+    for i in np.ndindex(*shape):
+        s = 0.0
+        for j in np.ndindex(*shape):
+            s += x[j] * k[tuple((i[a] - j[a]) % shape[a] for a in range(3))]
+        ref[i] = s
+    assert np.allclose(got, ref, atol=1e-12)
+
+
+def test_vector_weight_nyquist_plane_is_dropped():
+    """Re(ifft(rho^ w_v)) equals the real inverse transform with the vector
+    weight zeroed on its own axis' Nyquist plane (what the HIP path does)."""
+    shape = (8, 6, 10)
+    g = O.Grid(np.diag([12.0, 9.0, 15.0]), shape)
+    rho = np.random.default_rng(2).random(shape)
+    w = O.hs_weights(g, 1.3, 'WBII')
+    kr = g.fft(rho)
+    for a in range(3):
+        got = g.ifft(kr * w['vector'][1][..., a])
+        wz = w['vector'][1][..., a].copy()
+        sl = [slice(None)] * 3
+        sl[a] = shape[a] // 2
+        wz[tuple(sl)] = 0.0
+        X = np.fft.fftn(rho) * wz
+        ref = np.fft.ifftn(X)
+        assert np.abs(ref.imag).max() < 1e-12          # zeroed weight is Hermitian-consistent
+        assert np.allclose(got, ref.real, atol=1e-13)
+
+
+def test_integrate_matches_fsum():
+    g = O.Grid(np.eye(3) * 11.0, (9, 8, 7))
+    x = np.random.default_rng(3).random(g.npoints)
+    assert abs(g.integrate(x) - math.fsum(x.ravel()) * g.dr) < 1e-12 * abs(g.integrate(x))
+
+
+def test_hs_weights_closed_forms():
+    g = O.Grid(np.eye(3) * 20.0, (12, 10, 14))
+    w = O.hs_weights(g, R, 'WBII')
+    k = g.kpoints[..., 3]
+    om = k * R
+    sl = g.sigma_lanczos
+    nz = om > 0
+    # independent spherical Bessel routines
+    assert np.allclose(w['scalar'][0][nz], sp.spherical_jn(0, om[nz]) * sl[nz], atol=1e-14)
+    assert np.allclose(w['scalar'][3][nz], 4 * np.pi * R ** 3 * sp.spherical_jn(1, om[nz]) / om[nz] * sl[nz],
+                       rtol=1e-10, atol=1e-12)
+    # k = 0 limits
+    assert w['scalar'][0][0, 0, 0] == 1.0
+    assert np.isclose(w['scalar'][1][0, 0, 0], R)
+    assert np.isclose(w['scalar'][2][0, 0, 0], 4 * np.pi * R ** 2)
+    assert np.isclose(w['scalar'][3][0, 0, 0], 4 * np.pi * R ** 3 / 3)
+    assert np.all(w['vector'][1][0, 0, 0] == 0)
+    # Lanczos: product of sinc(k_a h_a / 2 pi)
+    xh = g.kpoints[:, 0, 0, 0] * g.spacings[0] / 2
+    ref = np.ones_like(xh)
+    ref[xh != 0] = np.sin(xh[xh != 0]) / xh[xh != 0]
+    assert np.allclose(sl[:, 0, 0], ref, atol=1e-15)
+
+
+def _uniform(version, rho_b, shape=(8, 8, 8)):
+    g = O.Grid(np.eye(3) * 30.0, shape)
+    hs = O.HardSphere(g, R, version, T)
+    rho = np.full(shape, rho_b)
+    return g, hs, rho
+
+
+@pytest.mark.parametrize('eta', [1e-3, 0.05, 0.3])
+def test_bulk_limits_fmt(eta):
+    rho_b = eta / (4 * np.pi * R ** 3 / 3)
+    # Rosenfeld FMT -> Percus-Yevick compressibility route
+    g, hs, rho = _uniform('FMT', rho_b)
+    phi = hs.value(g.fft(rho), local=True) * BETA
+    f_py = -np.log(1 - eta) + 1.5 * eta * (2 - eta) / (1 - eta) ** 2
+    assert np.allclose(phi, rho_b * f_py, rtol=1e-12)
+    mu_py = -np.log(1 - eta) + (14 * eta - 13 * eta ** 2 + 5 * eta ** 3) / (2 * (1 - eta) ** 3)
+    assert np.allclose(hs.derive(g.fft(rho)) * BETA, mu_py, rtol=1e-11)
+    # White-Bear II and MFMT -> Carnahan-Starling
+    for v in ['WBII', 'MFMT', 'aWBII']:
+        g, hs, rho = _uniform(v, rho_b)
+        phi = hs.value(g.fft(rho), local=True) * BETA
+        f_cs = (4 * eta - 3 * eta ** 2) / (1 - eta) ** 2
+        assert np.allclose(phi, rho_b * f_cs, rtol=1e-9), v
+        mu_cs = (8 * eta - 9 * eta ** 2 + 3 * eta ** 3) / (1 - eta) ** 3
+        assert np.allclose(hs.derive(g.fft(rho)) * BETA, mu_cs, rtol=1e-9), v
+
+
+def _wbii_published(n):
+    """White-Bear II (Hansen-Goos and Roth, J. Phys.: Condens. Matter 18,
+    8413 (2006)), written with log1p: phi2/(1-n) and phi3/(24 pi (1-n)^2)."""
+    L = np.log1p(-n)
+    f2 = 1 + (2 * n - n ** 2 + 2 * (1 - n) * L) / (3 * n)
+    f3 = 1 - (2 * n - 3 * n ** 2 + 2 * n ** 3 + 2 * (1 - n) ** 2 * L) / (3 * n ** 2)
+    return f2 / (1 - n), f3 / (24 * np.pi * (1 - n) ** 2)
+
+
+@pytest.mark.parametrize('n3', [1e-3, 1e-2, 0.1, 0.3, 0.6])
+def test_wbii_matches_published_form(n3):
+    """The oracle's WBII phi2, phi3 (full branch) equal the published closed
+    form; their n3-derivatives equal central differences of it."""
+    n = np.array([n3])
+    ln = np.log(1 - n)
+    p2, p3 = _wbii_published(n3)
+    # conditioning of the reference's own form ~ eps / n3^2
+    tol = 1e-15 / n3 ** 2 + 1e-12
+    assert np.isclose(O.phi2(n, ln, n * n, 'WBII')[0], p2, rtol=tol)
+    assert np.isclose(O.phi3(n, ln, n * n, n ** 3, 'WBII')[0], p3, rtol=tol)
+    # fourth-order central differences of the published form
+    h = 1e-3 * n3
+    fd = lambda i: (-_wbii_published(n3 + 2 * h)[i] + 8 * _wbii_published(n3 + h)[i]
+                    - 8 * _wbii_published(n3 - h)[i] + _wbii_published(n3 - 2 * h)[i]) / (12 * h)
+    assert np.isclose(O.dphi2(n, ln, n * n, 'WBII')[0], fd(0), rtol=1e-8)
+    assert np.isclose(O.dphi3(n, ln, n * n, n ** 3, 'WBII')[0], fd(1), rtol=1e-7)
+
+
+def test_wbii_small_branch_is_the_series():
+    """Below n3 = 1e-8 the reference switches to the leading terms of the
+    series of the published form (functionals.py:730-767)."""
+    for n3 in [1e-12, 1e-10, 9e-9]:
+        n = np.array([n3])
+        ln = np.log(1 - n)
+        assert np.isclose(O.phi2(n, ln, n * n, 'WBII')[0], 1.0 / (1 - n3), rtol=1e-14)
+        assert np.isclose(O.phi3(n, ln, n * n, n ** 3, 'WBII')[0], 1 / (24 * np.pi), rtol=1e-7)
+        assert np.isclose(O.dphi2(n, ln, n * n, 'WBII')[0], 1.0, rtol=1e-7)
+        assert np.isclose(O.dphi3(n, ln, n * n, n ** 3, 'WBII')[0], 7 / 3 / (36 * np.pi), rtol=1e-7)
+    # the series' own n3 coefficients agree with central differences of the
+    # published form at moderate n3 (dphi3 -> 7/(108 pi) as n3 -> 0)
+    h = 1e-4
+    d3 = (_wbii_published(2 * h)[1] - _wbii_published(h)[1]) / h
+    assert np.isclose(d3, 7 / 3 / (36 * np.pi), rtol=1e-2)
+
+
+def _smooth_density(shape, seed=4):
+    g = O.Grid(np.eye(3) * 14.0 * U.angstrom, shape)
+    rng = np.random.default_rng(seed)
+    x = g.points[..., :3]
+    rho = 3e-3 / U.angstrom ** 3 * (1 + 0.3 * np.cos(2 * np.pi * x[..., 0] / (14 * U.angstrom))
+                                    * np.cos(4 * np.pi * x[..., 1] / (14 * U.angstrom))) * (1 + 0.05 * rng.random(shape))
+    return g, rho
+
+
+@pytest.mark.parametrize('version', ['FMT', 'WBII', 'aWBII'])
+def test_derive_is_functional_derivative(version):
+    """[F(rho + h e_j) - F(rho - h e_j)] / (2 h dr) = c_j."""
+    g, rho = _smooth_density((12, 12, 12))
+    hs = O.HardSphere(g, R, version, T)
+    c = hs.derive(g.fft(rho))
+    for j in [(0, 0, 0), (3, 5, 7), (11, 2, 9)]:
+        h = 1e-4 * rho[j]
+        rp, rm = rho.copy(), rho.copy()
+        rp[j] += h
+        rm[j] -= h
+        fd = (hs.value(g.fft(rp)) - hs.value(g.fft(rm))) / (2 * h * g.dr)
+        assert np.isclose(fd, c[j], rtol=1e-6), (j, fd, c[j])
+
+
+def test_wda_bulk_and_derivative():
+    g, rho = _smooth_density((10, 10, 10))
+    eos = O.MFAEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    w = O.WDAV(g, R, eos, T)
+    rb = np.full(g.npoints, 2e-3 / U.angstrom ** 3)
+    assert np.allclose(w.weighted_density(g.fft(rb)), rb, rtol=1e-12)
+    assert np.allclose(w.derive(g.fft(rb)), 2 * eos.a * rb, rtol=1e-10)
+    c = w.derive(g.fft(rho))
+    for j in [(1, 2, 3), (7, 7, 0)]:
+        h = 1e-4 * rho[j]
+        rp, rm = rho.copy(), rho.copy()
+        rp[j] += h
+        rm[j] -= h
+        fd = (w.value(g.fft(rp)) - w.value(g.fft(rm))) / (2 * h * g.dr)
+        assert np.isclose(fd, c[j], rtol=1e-6)
+
+
+@pytest.mark.parametrize('kind', ['MFA', 'CS', 'MBWR'])
+def test_eos_derivative_by_finite_differences(kind):
+    m, s, e = W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON
+    eos = {'MFA': O.MFAEOS(m, s, e), 'CS': O.CarnahanStarlingEOS(m, R), 'MBWR': O.MBWREOS(m, s, e)}[kind]
+    eos.set_temperature(T)
+    for r in np.array([1e-4, 3e-3, 1e-2]) / U.angstrom ** 3:
+        h = 1e-6 * r
+        fd = (eos.fex_particle(r + h) - eos.fex_particle(r - h)) / (2 * h)
+        assert np.isclose(fd, eos.dfex_particle(r), rtol=1e-6)
+
+
+def test_mbwr_matches_cs_like_limits_and_mu_root():
+    """mu(p) from the rough-grid + brentq search satisfies p(rho) = p; at low
+    density the MBWR fluid approaches the ideal gas."""
+    eos = O.MBWREOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    for p in [0.01, 1.0, 10.0, 100.0]:
+        mu = eos.calculate_mu(T, p * U.bar)
+        rho = eos.densities_from_pressures([p * U.bar])[0][0]
+        # brentq's default absolute xtol (2e-12 bohr^-3) bounds the root to ~1e-6
+        assert abs(eos.pressure(rho) - p * U.bar) <= 1e-6 * p * U.bar
+        assert np.isclose(eos.chempot(rho), mu)
+    rho_id = 0.01 * U.bar / (U.boltzmann * T)
+    rho = eos.densities_from_pressures([0.01 * U.bar])[0][0]
+    assert np.isclose(rho, rho_id, rtol=1e-3)
+
+
+def test_lj_single_atom_and_image_sum():
+    L = 26.0 * U.angstrom
+    sig_i, eps_i = 3.1 * U.angstrom, 45 * U.boltzmann
+    pos = np.array([[1.0, -2.0, 0.5]]) * U.angstrom
+    g = O.Grid(np.eye(3) * L, (10, 10, 10))
+    V = O.external_potential(g.points[..., :3], [sig_i], [eps_i], pos, np.eye(3) * L, W.GUEST_SIGMA,
+                             W.GUEST_EPSILON)
+    sm = 0.5 * (sig_i + W.GUEST_SIGMA)
+    em = math.sqrt(eps_i * W.GUEST_EPSILON)
+    rc = 12 * U.angstrom
+    # explicit image sum (L/2 > rc, so the minimum image is the only one in range)
+    p = g.points[..., :3].reshape(-1, 3)
+    ref = np.zeros(len(p))
+    for n in np.ndindex(3, 3, 3):
+        d = p - pos[0] + (np.array(n) - 1) * L
+        r = np.sqrt((d ** 2).sum(1) + 1e-16)
+        ref += np.where(r < rc, 4 * em * ((sm / r) ** 12 - (sm / r) ** 6) - 4 * em * ((sm / rc) ** 12 - (sm / rc) ** 6),
+                        0.0)
+    assert np.allclose(V, ref, rtol=1e-12, atol=1e-18)
+
+
+def test_ideal_gas_fixed_point():
+    """No excess functional: rho = Lambda^-3 exp(beta (mu - V)) in closed form."""
+    host = W.host_case1()
+    g = O.Grid(host.rvecs, (16, 16, 16))
+    sig, eps = host.atom_params()
+    V = O.external_potential(g.points[..., :3], sig, eps, host.pos, host.rvecs, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    ext = O.ExtPot(g, V)
+    mu = -30 * U.kjmol
+    exact = np.exp(BETA * (mu - ext.potential)) / LAM ** 3
+    s = O.Picard(g, [ext], BETA, LAM, nsteps=2000, alpha_mix=0.3, method='static', criterion='RIUE',
+                 threshold=1e-14)
+    rho0 = exact * (1 + 0.2 * np.random.default_rng(6).random(g.npoints))
+    N, rho, conv = s.solve(mu, rho0)
+    assert conv
+    m = exact > 1e-10 / U.angstrom ** 3
+    assert np.max(np.abs(rho[m] - exact[m]) / exact[m]) < 1e-12
+
+
+def test_bulk_fixed_point():
+    """V = 0, WBII + MFA-WDA: the uniform fixed point solves
+    mu = kT ln(Lambda^3 rho) + mu_CS(rho) + 2 a rho (closed forms)."""
+    g = O.Grid(np.eye(3) * 20 * U.angstrom, (10, 10, 10))
+    eos = O.MFAEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    parts = [O.HardSphere(g, R, 'WBII', T), O.WDAV(g, R, eos, T), O.ExtPot(g, np.zeros(g.npoints))]
+    mu = -25 * U.kjmol
+    v = 4 * np.pi * R ** 3 / 3
+
+    def f(r):
+        e = v * r
+        return (np.log(LAM ** 3 * r) + (8 * e - 9 * e ** 2 + 3 * e ** 3) / (1 - e) ** 3) / BETA + 2 * eos.a * r - mu
+    rb = brentq(f, 1e-12, 0.5 / v)
+    s = O.Anderson(g, parts, BETA, LAM, nsteps=500, method='anderson', m=5, damping=0.1, criterion='RIUE',
+                   threshold=1e-13)
+    N, rho, conv = s.solve(mu, np.full(g.npoints, 0.5 * rb))
+    assert conv
+    assert np.allclose(rho, rb, rtol=1e-9)
+
+
+def test_simplex_lsq_slsqp_vs_exact():
+    """The reference's SLSQP coefficients reach the exact simplex-constrained
+    minimum (computed here by enumerating the KKT systems of every support)."""
+    rng = np.random.default_rng(5)
+    for trial in range(6):
+        m = 2 + trial % 4
+        Rm = rng.standard_normal((m, 40)) + (0.8 if trial % 2 else 0.0) * rng.standard_normal(40)
+        G = Rm @ Rm.T
+        best = np.inf
+        for mask in range(1, 2 ** m):
+            S = [i for i in range(m) if mask >> i & 1]
+            K = np.zeros((len(S) + 1, len(S) + 1))
+            K[:len(S), :len(S)] = G[np.ix_(S, S)]
+            K[:len(S), -1] = -1
+            K[-1, :len(S)] = 1
+            try:
+                x = np.linalg.solve(K, np.r_[np.zeros(len(S)), 1.0])[:len(S)]
+            except np.linalg.LinAlgError:
+                continue
+            if np.all(x >= -1e-12):
+                a = np.zeros(m)
+                a[S] = x
+                best = min(best, a @ G @ a)
+        a = O.simplex_lsq_slsqp(Rm)
+        assert np.isclose(a.sum(), 1) and np.all(a >= -1e-9)
+        assert a @ G @ a <= best * (1 + 1e-6) + 1e-12
+
+
+def test_wbii_formulas_are_ill_conditioned_at_low_n3():
+    """Documents why the GPU-vs-oracle bound for WBII c1 is on beta*|dc|: a
+    roundoff-size change of n3 (1e-17 of max n3) moves the reference's c1 by
+    up to ~1e-9 of max|c1| where n3 is small."""
+    host = W.host_case1()
+    shape = (24, 24, 24)
+    og = O.Grid(host.rvecs, shape)
+    sig, eps = host.atom_params()
+    V = O.external_potential(og.points[..., :3], sig, eps, host.pos, host.rvecs, W.GUEST_SIGMA,
+                             W.GUEST_EPSILON).reshape(shape)
+    rho = 2.4e-4 / U.angstrom ** 3 * np.exp(-BETA * np.clip(V, -50 * U.kjmol, 50 * U.kjmol))
+    hs = O.HardSphere(og, R, 'WBII', T)
+    kr = og.fft(rho)
+    c0 = hs.derive(kr)
+    d = dict(hs.densities(kr))
+    n3 = og.ifft(kr * hs.w['scalar'][3])
+    n3p = np.clip(n3 + 1e-17 * np.abs(n3).max() * np.random.default_rng(1).standard_normal(shape), 1e-30, 0.99)
+    d.update(n3=n3p, ln3=np.log(1 - n3p), n3_2=n3p ** 2, n3_3=n3p ** 3)
+    hs._key, hs._d = kr, d
+    c1 = hs.derive(kr)
+    rel = np.abs(c1 - c0).max() / np.abs(c0).max()
+    assert rel > 1e-12            # far above the 1e-17 perturbation
+    assert np.abs(c1 - c0).max() < 1e-8 * U.boltzmann * T
