@@ -66,6 +66,14 @@ EXPORTS = {
     'cmm_trial': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                    C.c_int], C.c_int),
     'cmm_get_scalars': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_batch_reset': ([C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int], C.c_int),
+    'cmm_batch_check': ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    'cmm_batch_state': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_batch_release': ([C.c_void_p], C.c_int),
+    'cmm_graph_begin': ([C.c_void_p], C.c_int),
+    'cmm_graph_end': ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    'cmm_graph_launch': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_graph_destroy': ([C.c_void_p], C.c_int),
     'cmm_synchronize': ([C.c_void_p], C.c_int),
     'cmm_lj_grid': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                      C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
@@ -232,6 +240,42 @@ class Plan:
     def trial(self, rho, rhist, slot, rho_out, vext, vext_batched, lam3, clip):
         check(self.lib.cmm_trial(self.h, _ptr(rho), _ptr(rhist), int(slot), _ptr(rho_out), _ptr(vext),
                                  int(vext_batched), float(lam3), int(clip)))
+
+    # ---- batched solves and graphs -----------------------------------------
+    def batch_reset(self, mu, beta, lam3, dr, tol, nsteps):
+        mu = np.ascontiguousarray(np.asarray(mu, dtype=np.float64))
+        assert mu.shape == (self.nbatch,)
+        check(self.lib.cmm_batch_reset(self.h, mu.ctypes.data_as(C.c_void_p), float(beta), float(lam3), float(dr),
+                                       float(tol), int(nsteps)))
+
+    def batch_check(self, new_slot, init=False):
+        check(self.lib.cmm_batch_check(self.h, int(new_slot), int(init)))
+
+    def batch_state(self):
+        nb = self.nbatch
+        a, it, st, fs = (np.zeros(nb, dtype=np.int32) for _ in range(4))
+        om = np.zeros(nb)
+        check(self.lib.cmm_batch_state(self.h, a.ctypes.data_as(C.c_void_p), it.ctypes.data_as(C.c_void_p),
+                                       st.ctypes.data_as(C.c_void_p), fs.ctypes.data_as(C.c_void_p),
+                                       om.ctypes.data_as(C.c_void_p)))
+        return dict(active=a, iters=it, status=st, final_slot=fs, omega=om)
+
+    def batch_release(self):
+        check(self.lib.cmm_batch_release(self.h))
+
+    def graph_begin(self):
+        check(self.lib.cmm_graph_begin(self.h))
+
+    def graph_end(self):
+        ex = C.c_void_p()
+        check(self.lib.cmm_graph_end(self.h, C.byref(ex)))
+        return ex
+
+    def graph_launch(self, ex):
+        check(self.lib.cmm_graph_launch(self.h, ex))
+
+    def graph_destroy(self, ex):
+        check(self.lib.cmm_graph_destroy(ex))
 
     def scalars(self):
         check(self.lib.cmm_get_scalars(self.h, self._scal.ctypes.data_as(C.c_void_p)))

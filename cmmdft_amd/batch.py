@@ -1,0 +1,186 @@
+"""Batched Euler-Lagrange solves: many state points of one host and grid
+(the chemical potentials of an isotherm) iterated together on the device.
+
+The reference loops over state points one by one
+(examples/methane_ZIF8.ipynb, the cascade_solver loop).  Here every kernel
+of the pipeline runs over the whole batch at once (one grid dimension of every
+launch is the instance), each instance carries its own Anderson history,
+damping, grand potential and convergence state on the device, and a device
+kernel retires converged instances from an activity mask that every later
+kernel reads.  The instances advance in lockstep, so their history-ring slots
+are common; after the first m steps the slot pattern is periodic with period
+m (m + 1), and that many iterations are captured once into a HIP graph and
+replayed until every instance has finished.  The host only reads the status
+block between graph launches.
+
+Per instance the arithmetic is that of the plain ``Anderson(method='anderson')``
+solver (same kernels, same tiles, same reduction order), so a state point that
+converges without the grand-potential safeguard firing gives the same result
+bit for bit.  Instances whose safeguard fires (the reference would take a
+hybrid Picard step there) or that stop on non-finite values are re-solved
+with the plain solver.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import units as U
+from ._device import torch_mod, as_device
+from . import _lib as L
+from .functionals import build_functional
+from .solver import Anderson
+
+__all__ = ['BatchedAnderson']
+
+STATUS = {0: 'running', 1: 'converged', 2: 'step limit', 3: 'fallback', 4: 'non-finite'}
+
+
+class BatchedAnderson:
+    def __init__(self, program, chempots, m=5, damping=0.1, threshold=1e-8, nsteps=500, use_graph=True,
+                 adaptive_damping=True, damping_max=0.8, damping_min=0.01, damping_factors=(1.5, 0.5),
+                 resolve_fallbacks=True):
+        assert 1 <= m <= L.MMAX
+        self.program = program
+        self.grid = program.grid
+        self.fener = program.fener
+        self.chempots = np.asarray(chempots, dtype=np.float64).ravel()
+        self.m = m
+        self.damping = damping
+        self.threshold = threshold
+        self.nsteps = nsteps
+        self.use_graph = use_graph
+        self.adaptive = adaptive_damping
+        self.dmax, self.dmin = damping_max, damping_min
+        self.factors = damping_factors
+        self.resolve_fallbacks = resolve_fallbacks
+        self._plan = None
+        self._graph = None
+
+    # ------------------------------------------------------------------
+    def _setup(self):
+        torch = torch_mod()
+        fener, grid = self.fener, self.grid
+        B = len(self.chempots)
+        hs = wda = None
+        V = None
+        for part in fener.parts:
+            if part.name == 'HardSphere':
+                hs = part
+            elif part.name == 'WDA-V':
+                wda = part
+            elif part.name == 'ExtPot':
+                V = part.potential_dev.reshape(-1) if V is None else V + part.potential_dev.reshape(-1)
+            else:
+                raise NotImplementedError('functional part %r has no device implementation' % part.name)
+        self.beta = fener.beta
+        self.lam3 = fener.wavelength ** 3
+        self.V = V.contiguous() if V is not None else None
+        if self._plan is None or self._plan.nbatch != B:
+            from ._lib import Plan
+            self.stream = torch.cuda.Stream()
+            self._plan = Plan(grid.npoints, grid.cell.rvecs, nbatch=B)
+            self._plan.bind_stream(self.stream)
+            n = grid.npts
+            self.rho_ring = torch.empty((self.m + 1, B, n), dtype=torch.float64, device='cuda')
+            self.res_ring = torch.zeros((self.m, B, n), dtype=torch.float64, device='cuda')
+            if self._graph is not None:
+                self._plan.graph_destroy(self._graph)
+                self._graph = None
+        self._plan.set_functional(build_functional(hs=hs, wda=wda, kT=1.0 / self.beta))
+        fug = np.exp(self.beta * self.chempots) / self.beta / self.lam3
+        self._plan.set_instances(fug, np.zeros(B))
+        return self._plan
+
+    def _iteration(self, s):
+        """Enqueue batched Anderson step s (1-based)."""
+        m, p = self.m, self._plan
+        cur, rslot, new = (s - 1) % (m + 1), (s - 1) % m, s % (m + 1)
+        mk = min(s, m)
+        ts = range(s - mk, s)
+        order_r = [t % m for t in ts]
+        order_rho = [t % (m + 1) for t in ts]
+        slot_prev = (s - 2) % m if s >= 2 else -1
+        p.finish_anderson(self.rho_ring[cur], self.V, False, self.res_ring, m, rslot, None, self.beta, self.lam3)
+        p.anderson_coef(m, order_r, rslot, slot_prev, int(self.adaptive), self.factors[0], self.factors[1],
+                        self.dmin, self.dmax)
+        p.anderson_mix(self.rho_ring, order_rho, self.res_ring, order_r, self.rho_ring[cur], self.rho_ring[new],
+                       self.V, False, self.lam3)
+        p.derive()
+        p.batch_check(new)
+
+    def solve(self, rho0=None):
+        """Returns dict(N, rho (B, nx, ny, nz) device tensor, converged, iters,
+        status, omega)."""
+        torch = torch_mod()
+        p = self._setup()
+        B, m = len(self.chempots), self.m
+        self.stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self.stream):
+            if rho0 is None:
+                # ideal-gas initial densities (program.py:296-305), one per state point
+                Vd = self.V if self.V is not None else torch.zeros(self.grid.npts, dtype=torch.float64, device='cuda')
+                mu = torch.from_numpy(self.chempots).to('cuda')[:, None]
+                self.rho_ring[0].copy_(torch.exp(self.beta * (mu - Vd[None, :])) / self.lam3)
+            else:
+                self.rho_ring[0].copy_(as_device(rho0).reshape(B, -1))
+            self.res_ring.zero_()
+            p.anderson_init(np.full(B, self.damping), np.ones(B))
+            p.batch_reset(self.chempots, self.beta, self.lam3, self.grid.dr, self.threshold, self.nsteps)
+            p.seed(self.rho_ring[0], self.V, False, self.lam3)
+            p.derive()
+            p.batch_check(0, init=True)
+            s = 0
+            # warm-up: the first m steps have growing history lengths
+            while s < m and s < self.nsteps:
+                s += 1
+                self._iteration(s)
+            st = p.batch_state()
+            period = m * (m + 1)
+            if self.use_graph and st['active'].any() and s < self.nsteps:
+                if self._graph is None:
+                    p.graph_begin()
+                    try:
+                        for k in range(period):
+                            self._iteration(m + 1 + k)
+                    finally:
+                        self._graph = p.graph_end()
+                while st['active'].any() and s < self.nsteps:
+                    p.graph_launch(self._graph)
+                    s += period
+                    st = p.batch_state()
+            else:
+                while st['active'].any() and s < self.nsteps:
+                    s += 1
+                    self._iteration(s)
+                    if s % 8 == 0:
+                        st = p.batch_state()
+                st = p.batch_state()
+            scal = p.scalars()
+            rho = torch.empty((B,) + tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
+            for i in range(B):
+                rho[i].copy_(self.rho_ring[st['final_slot'][i], i].reshape(self.grid.npoints))
+        self.stream.synchronize()
+        p.batch_release()
+        N = scal[:, L.S_PN] * self.grid.dr
+        status = st['status'].copy()
+        iters = st['iters'].copy()
+        redo = [i for i in range(B) if status[i] in (2, 3, 4)] if self.resolve_fallbacks else []
+        for i in redo:
+            solver = Anderson(self.program, nsteps=self.nsteps, method='anderson', m=m, damping=self.damping,
+                              criterion='RIUE', threshold=self.threshold, adaptive_damping=self.adaptive,
+                              damping_max=self.dmax, damping_min=self.dmin, damping_factors=self.factors)
+            mu = self.chempots[i]
+            r0 = torch.exp(self.beta * (mu - self.V.reshape(self.grid.npoints))) / self.lam3 if rho0 is None \
+                else as_device(rho0)[i]
+            Ni, ri, ci = solver.solve(mu, r0)
+            rho[i].copy_(ri.reshape(self.grid.npoints))
+            N[i] = Ni
+            iters[i] = solver.curr_step
+            status[i] = 1 if ci else 2
+        return dict(N=N, rho=rho, converged=status == 1, iters=iters, status=status, omega=st['omega'],
+                    resolved=redo)
+
+    def close(self):
+        if self._graph is not None and self._plan is not None:
+            self._plan.graph_destroy(self._graph)
+            self._graph = None

@@ -60,6 +60,15 @@ struct cmm_plan {
   const int* active;
   int tbx, tby, pz, pphi;
   size_t ws_bytes;
+  // batched-solve bookkeeping (allocated on first use)
+  int* b_active;
+  int* b_iters;
+  int* b_status;
+  int* b_final;
+  double* b_omega_prev;
+  double* b_omega;
+  double* b_mu;
+  BatchDev bd;
 };
 
 namespace {
@@ -207,7 +216,9 @@ int cmm_plan_destroy(cmm_plan* p) {
     if (p->ktab[a]) (void)hipFree(p->ktab[a]);
     if (p->ltab[a]) (void)hipFree(p->ltab[a]);
   }
-  void* bufs[] = {p->rz, p->F, p->partials, p->counters, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef, p->alpha_out};
+  void* bufs[] = {p->rz, p->F, p->partials, p->counters, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef,
+                  p->alpha_out, p->b_active, p->b_iters, p->b_status, p->b_final, p->b_omega_prev, p->b_omega,
+                  p->b_mu};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete p;
@@ -652,5 +663,100 @@ int cmm_lj_points(const double* points, long long M, const double* rvecs, const 
   dim3 grid((unsigned)((M + LJ_NT - 1) / LJ_NT));
   hipLaunchKernelGGL(k_lj_points, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, natom, d, points, M, V);
   LAUNCHCHK("k_lj_points");
+  return CMM_OK;
+}
+
+int cmm_batch_reset(cmm_plan* p, const double* mu, double beta, double lam3, double dr, double tol, int nsteps) {
+  if (!p || !mu) return fail(CMM_E_ARG, "null argument");
+  const int nb = p->nb;
+  if (!p->b_active) {
+    hipError_t e = hipSuccess;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_active, nb * sizeof(int)) : e;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_iters, nb * sizeof(int)) : e;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_status, nb * sizeof(int)) : e;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_final, nb * sizeof(int)) : e;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_omega_prev, nb * sizeof(double)) : e;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_omega, nb * sizeof(double)) : e;
+    e = (e == hipSuccess) ? hipMalloc(&p->b_mu, nb * sizeof(double)) : e;
+    if (e != hipSuccess) return fail(CMM_E_HIP, std::string("batch allocation: ") + hipGetErrorString(e));
+  }
+  std::vector<int> ones(nb, 1), zeros(nb, 0);
+  HIPCHK(hipMemcpyAsync(p->b_active, ones.data(), nb * sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemcpyAsync(p->b_iters, zeros.data(), nb * sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemcpyAsync(p->b_status, zeros.data(), nb * sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemcpyAsync(p->b_final, zeros.data(), nb * sizeof(int), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemcpyAsync(p->b_mu, mu, nb * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  BatchDev& b = p->bd;
+  b.active = p->b_active;
+  b.iters = p->b_iters;
+  b.status = p->b_status;
+  b.final_slot = p->b_final;
+  b.omega_prev = p->b_omega_prev;
+  b.omega = p->b_omega;
+  b.mu = p->b_mu;
+  b.beta = beta;
+  b.kT = 1.0 / beta;
+  b.lam3 = lam3;
+  b.dr = dr;
+  b.tol = tol;
+  b.nsteps = nsteps;
+  b.nb = nb;
+  p->active = p->b_active;
+  return CMM_OK;
+}
+
+int cmm_batch_check(cmm_plan* p, int new_slot, int init) {
+  if (!p || !p->b_active) return fail(CMM_E_ARG, "batch not initialised");
+  hipLaunchKernelGGL(k_batch_check, dim3((p->nb + 63) / 64), dim3(64), 0, p->stream, p->bd, red_of(p), new_slot, init);
+  LAUNCHCHK("k_batch_check");
+  return CMM_OK;
+}
+
+int cmm_batch_state(cmm_plan* p, int* active, int* iters, int* status, int* final_slot, double* omega) {
+  if (!p || !p->b_active) return fail(CMM_E_ARG, "batch not initialised");
+  const int nb = p->nb;
+  if (active) HIPCHK(hipMemcpyAsync(active, p->b_active, nb * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  if (iters) HIPCHK(hipMemcpyAsync(iters, p->b_iters, nb * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, p->b_status, nb * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  if (final_slot) HIPCHK(hipMemcpyAsync(final_slot, p->b_final, nb * sizeof(int), hipMemcpyDeviceToHost, p->stream));
+  if (omega) HIPCHK(hipMemcpyAsync(omega, p->b_omega, nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
+int cmm_batch_release(cmm_plan* p) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (p->active == p->b_active) p->active = nullptr;
+  return CMM_OK;
+}
+
+int cmm_graph_begin(cmm_plan* p) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (p->stream == 0) return fail(CMM_E_ARG, "graph capture needs a non-default stream");
+  HIPCHK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  return CMM_OK;
+}
+
+int cmm_graph_end(cmm_plan* p, void** exec_out) {
+  if (!p || !exec_out) return fail(CMM_E_ARG, "null argument");
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamEndCapture(p->stream, &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail(CMM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  *exec_out = (void*)ex;
+  return CMM_OK;
+}
+
+int cmm_graph_launch(cmm_plan* p, void* exec) {
+  if (!p || !exec) return fail(CMM_E_ARG, "null argument");
+  HIPCHK(hipGraphLaunch((hipGraphExec_t)exec, p->stream));
+  return CMM_OK;
+}
+
+int cmm_graph_destroy(void* exec) {
+  if (exec) HIPCHK(hipGraphExecDestroy((hipGraphExec_t)exec));
   return CMM_OK;
 }

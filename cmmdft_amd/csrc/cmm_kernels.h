@@ -91,4 +91,20 @@ struct AndDev {
   double* alpha_out;
 };
 
+// Batched solve bookkeeping (one entry per instance).
+enum { ST_RUNNING = 0, ST_CONVERGED = 1, ST_MAXSTEPS = 2, ST_FALLBACK = 3, ST_NONFINITE = 4 };
+
+struct BatchDev {
+  int* active;
+  int* iters;
+  int* status;
+  int* final_slot;
+  double* omega_prev;
+  double* omega;
+  const double* mu;
+  double beta, kT, lam3, dr, tol;
+  int nsteps;
+  int nb;
+};
+
 }  // namespace cmm

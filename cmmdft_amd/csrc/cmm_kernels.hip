@@ -805,4 +805,49 @@ __global__ void __launch_bounds__(64) k_anderson_coef(AndDev ad, const int* __re
   }
 }
 
+// Per-instance convergence and safeguard test of one batched Anderson
+// iteration (Solver._check_convergence with RIUE, solver.py:167-234, and the
+// grand-potential test of Anderson.update_rho, solver.py:567-574).  Reads the
+// scalars of the density just produced (N', sum|drho|, non-finite count,
+// ideal/external sums) and of its functional chain (sum Phi, sum f_ex).
+// init = 1 only records Omega of the seeded densities.
+__global__ void k_batch_check(BatchDev b, RedDev red, int new_slot, int init) {
+  const int inst = blockIdx.x * blockDim.x + threadIdx.x;
+  if (inst >= b.nb || !b.active[inst]) return;
+  const double* s = red.scal + inst * NSCAL;
+  const double N = s[S_PN];
+  const double om = s[S_PN + 4] * b.dr * b.kT - b.mu[inst] * N * b.dr + s[S_PHI] * b.dr * b.kT + s[S_PHI + 1] * b.dr +
+                    s[S_PN + 5] * b.dr;
+  b.omega[inst] = om;
+  if (init) {
+    b.omega_prev[inst] = om;
+    b.final_slot[inst] = new_slot;
+    return;
+  }
+  const int it = b.iters[inst] + 1;
+  b.iters[inst] = it;
+  const bool nonfinite = s[S_PN + 3] > 0.0 || !isfinite(s[S_PHI]) || !isfinite(s[S_PHI + 1]) || !isfinite(N);
+  if (nonfinite) {
+    b.status[inst] = ST_NONFINITE;
+    b.active[inst] = 0;
+    return;
+  }
+  if (om > 0.8 * b.omega_prev[inst]) {     // the reference would take a hybrid Picard step here
+    b.status[inst] = ST_FALLBACK;
+    b.active[inst] = 0;
+    return;
+  }
+  b.omega_prev[inst] = om;
+  b.final_slot[inst] = new_slot;
+  const double iue = s[S_PN + 1];
+  const bool conv = (N > 0.0) ? (iue / N < b.tol) : (iue == 0.0);
+  if (conv) {
+    b.status[inst] = ST_CONVERGED;
+    b.active[inst] = 0;
+  } else if (it >= b.nsteps) {
+    b.status[inst] = ST_MAXSTEPS;
+    b.active[inst] = 0;
+  }
+}
+
 }  // namespace cmm

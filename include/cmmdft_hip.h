@@ -140,6 +140,24 @@ int cmm_anderson_state(cmm_plan* plan, double* damping, double* alpha);
 int cmm_trial(cmm_plan* plan, const double* rho, const double* rhist, int slot, double* rho_out,
               const double* vext, int vext_batched, double lam3, int clip);
 
+/* Batched solves (no reference counterpart: the reference loops over state
+ * points serially, examples/methane_ZIF8.ipynb).  cmm_batch_reset uploads the
+ * per-instance chemical potentials, marks every instance active and makes the
+ * batch's device mask the plan's active mask; cmm_batch_check tests the
+ * density just produced and derived (RIUE < tol, the grand-potential
+ * safeguard of Anderson.update_rho, solver.py:567-574, non-finite values,
+ * nsteps) and deactivates finished instances; status codes: 0 running,
+ * 1 converged, 2 step limit, 3 needs the hybrid fallback, 4 non-finite. */
+int cmm_batch_reset(cmm_plan* plan, const double* mu, double beta, double lam3, double dr, double tol, int nsteps);
+int cmm_batch_check(cmm_plan* plan, int new_slot, int init);
+int cmm_batch_state(cmm_plan* plan, int* active, int* iters, int* status, int* final_slot, double* omega);
+int cmm_batch_release(cmm_plan* plan);
+/* capture the plan stream's launches into a HIP graph (kernels only) */
+int cmm_graph_begin(cmm_plan* plan);
+int cmm_graph_end(cmm_plan* plan, void** exec_out);
+int cmm_graph_launch(cmm_plan* plan, void* exec);
+int cmm_graph_destroy(void* exec);
+
 /* copy scalars [nbatch][CMM_NSCAL] to host (waits for the stream) */
 int cmm_get_scalars(cmm_plan* plan, double* host_out);
 int cmm_synchronize(cmm_plan* plan);
