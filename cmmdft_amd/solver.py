@@ -265,9 +265,11 @@ class Picard(Solver):
         return s
 
     def _start(self, chempot, rho):
-        torch = torch_mod()
         eng = _Engine(self.grid, self.fener)
         self._initiate_solving(chempot)
+        # this solve's fugacity must be on the device before the first
+        # Boltzmann update (the plan is shared between solves)
+        self._set_inst(eng, 0.0)
         self.was_numpy = isinstance(rho, np.ndarray)
         r0 = as_device(rho).reshape(-1)
         return eng, r0

@@ -50,7 +50,7 @@ def rho_rel_err(rho, ref):
     """max |drho|/rho over rho > 1e-10 A^-3, and the count of points where
     exactly one side is below that floor."""
     floor = 1e-10 / U.angstrom ** 3
-    m = ref > floor
+    m = (ref > floor) & (rho > floor)
     rel = np.max(np.abs(rho[m] - ref[m]) / ref[m])
-    mismatch = int(np.sum((rho > floor) != m))
+    mismatch = int(np.sum((rho > floor) != (ref > floor)))
     return rel, mismatch

@@ -57,8 +57,10 @@ def test_batch_matches_oracle(setup):
     i = 3
     og, parts, beta, lam, V = oracle_system(host, SHAPE, 'WBII')
     rho0 = O.initial_density(og, mus[i], beta, lam, V=V)
-    s = O.Picard(og, parts, beta, lam, nsteps=6000, alpha_mix=0.05, method='static', criterion='RIUE',
-                 threshold=1e-13)
+    # same solver family (Anderson clips at 1e-12 bohr^-3; static Picard
+    # zeroes below 1e-10 A^-3), so the density floors must coincide too
+    s = O.Anderson(og, parts, beta, lam, nsteps=1000, method='anderson', m=5, damping=0.1, criterion='RIUE',
+                   threshold=1e-13)
     Nref, rho_ref, conv = s.solve(mus[i], rho0)
     assert conv
     b = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-13, nsteps=600)
