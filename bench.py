@@ -5,11 +5,14 @@ Workload: the 128^3 host (cubic a = 25.6 A, 276 LJ atoms of 4 types, seeded),
 united-atom methane at 300 K, White-Bear II FMT + mean-field WDA + LJ external
 potential, Anderson mixing (m = 5, damping 0.1, RIUE < 1e-8), over the
 isotherm of 32 pressures log-spaced 1e-2..1e2 bar (mu from the MBWR EOS).
-A "step" is one converged isotherm point (one full Euler-Lagrange solve from
-the ideal-gas initial density); the external potential is computed before the
-timed region and reported separately.
+A "step" is one full isotherm: all 32 state points solved to convergence from
+their ideal-gas initial densities.  ``value`` is the batched rate (every
+state point iterated together, HIP-graph replay); the plain rate (the drop-in
+solver, one state point after the other) is reported beside it.  The external
+potential is computed before the timed region and reported separately.
 
-Prints one JSON line (rank 0).  Default: N = 1, 8 timed points, 1 warm-up.
+Prints one JSON line (rank 0).  Default: N = 1, 2 timed isotherms per mode,
+1 warm-up.
 """
 from __future__ import annotations
 
@@ -120,7 +123,9 @@ def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=8)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--points', type=int, default=32)
+    ap.add_argument('--mode', choices=['both', 'batched', 'plain'], default='both')
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--grid', type=int, default=NSIDE)
     ap.add_argument('--cpu-sample', type=int, default=4, help='oracle iterations timed (0 skips the CPU leg)')
@@ -142,29 +147,64 @@ def main():
     vext_ms = (time.perf_counter() - tv) * 1e3
     solver = Anderson(prog, nsteps=500, method='anderson', m=5, damping=0.1, criterion='RIUE', threshold=1e-8)
     prog.set_solver(solver)
+    from cmmdft_amd.batch import BatchedAnderson
+    mus = mus[:args.points]
+    npt = len(mus)
 
-    def solve_point(i):
-        mu = mus[i % len(mus)]
-        rho0 = prog._set_initial_density(chempot=mu, temperature=W.TEMPERATURE)
-        N, rho, conv = solver.solve(mu, rho0)
-        return solver.curr_step, conv, N
+    def plain_isotherm():
+        its, cv = [], []
+        for mu in mus:
+            rho0 = prog._set_initial_density(chempot=mu, temperature=W.TEMPERATURE)
+            N, rho, conv = solver.solve(mu, rho0)
+            its.append(solver.curr_step)
+            cv.append(conv)
+        return its, cv
 
-    for i in range(args.warmup):
-        solve_point(i)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    iters, convs, loads = [], [], []
-    for k in range(args.steps):
-        it, cv, N = solve_point(k)
-        iters.append(it)
-        convs.append(cv)
-        loads.append(N)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    points_per_s = args.steps / dt
+    batch = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-8, nsteps=500)
+
+    def batched_isotherm():
+        res = batch.solve()
+        return list(res['iters']), list(res['converged']), res
+
+    plain_pps = batched_pps = None
+    iters, convs = [], []
+    if args.mode in ('both', 'plain'):
+        for _ in range(args.warmup):
+            plain_isotherm()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            its, cv = plain_isotherm()
+            iters += its
+            convs += cv
+        torch.cuda.synchronize()
+        dt_plain = time.perf_counter() - t0
+        plain_pps = npt * args.steps / dt_plain
+    biters, bconvs, resolved = [], [], 0
+    if args.mode in ('both', 'batched'):
+        for _ in range(args.warmup):
+            batched_isotherm()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            its, cv, res = batched_isotherm()
+            biters += its
+            bconvs += cv
+            resolved += len(res['resolved'])
+        torch.cuda.synchronize()
+        dt_batch = time.perf_counter() - t0
+        batched_pps = npt * args.steps / dt_batch
+    value = batched_pps if batched_pps is not None else plain_pps
+    dt = (dt_batch if batched_pps is not None else dt_plain)
+    if not iters:
+        iters, convs = biters, bconvs
+    points_per_s = value
     iters_per_point = float(np.mean(iters))
 
-    # dominant pass: per-launch time x launches per iteration (2 y passes of 6 fields)
+    # dominant pass: per-launch time x launches per iteration (2 y passes of 6 fields),
+    # timed on the single-instance plan (solve one point first if only the batch ran)
+    if plain_pps is None:
+        solver.solve(mus[0], prog._set_initial_density(chempot=mus[0], temperature=W.TEMPERATURE))
     plan = prog.grid.plan()
     tp = time_passes(plan)
     per_iter_weight = {0: 1, 1: 1, 2: 1, 3: 2}
@@ -173,30 +213,35 @@ def main():
     achieved = pb[dom] / (tp[dom] * 1e-3) / 1e9
     npts = int(np.prod(shape))
     b_iter = npts * (16 * 14 + 24) + npts * 8 * (3 * 5 + 2)
-    iter_gbps = b_iter * sum(iters) / dt / 1e9
+    iter_gbps = b_iter * int(sum(int(v) for v in (biters if biters else iters))) / dt / 1e9
 
     cpu = None
     if args.cpu_sample > 0:
         V = prog.fener.part_dict['ExtPot'].potential
-        cpu = cpu_baseline(V, shape, mus[0], iters_per_point, args.cpu_sample)
+        cpu = cpu_baseline(V, shape, mus[len(mus) // 2], iters_per_point, args.cpu_sample)
 
     line = {
         'metric': METRIC, 'value': points_per_s, 'unit': 'points/s', 'n_gpus': 1, 'steps': args.steps,
         'warmup': args.warmup, 'ms_per_step': dt / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f64', 'data': 'synthetic',
-        'config': {'workload': 'plain GPU drop-in: case-2 host (a=25.6 A, 276 LJ atoms) at %dx%dx%d, WBII FMT + '
-                               'MFA-WDA + LJ ext. potential, Anderson m=5 damping 0.1 RIUE<1e-8, isotherm of 32 '
-                               'pressures 1e-2..1e2 bar, one point per step' % shape,
-                   'grid': list(shape), 'solver': 'anderson', 'mode': 'plain'},
+        'config': {'workload': 'isotherm at %dx%dx%d: case-2 host (a=25.6 A, 276 LJ atoms), WBII FMT + MFA-WDA + '
+                               'LJ ext. potential, Anderson m=5 damping 0.1 RIUE<1e-8, %d pressures 1e-2..1e2 bar '
+                               'solved per step' % (shape + (npt,)),
+                   'grid': list(shape), 'solver': 'anderson', 'points_per_step': npt,
+                   'mode': 'batched' if batched_pps is not None else 'plain'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': PEAK_HBM_GBPS, 'unit': 'GB/s',
                      'frac': achieved / PEAK_HBM_GBPS, 'traffic': None, 'kernel': PASS_NAMES[dom],
                      'avg_launch_ms': tp[dom], 'algorithmic_bytes_per_launch': pb[dom]},
         'cpu_baseline': cpu,
-        'plain_points_per_s': points_per_s,
+        'plain_points_per_s': plain_pps,
+        'batched_points_per_s': batched_pps,
+        'batched_vs_plain': (batched_pps / plain_pps) if (batched_pps and plain_pps) else None,
+        'batched_resolved_points': resolved,
+        'batch_chunk': batch.chunk,
         'iterations_per_point': iters_per_point,
-        'ms_per_iteration': dt * 1e3 / sum(iters),
+        'max_iterations_per_point': int(max(iters)),
         'iteration_algorithmic_GBps': iter_gbps,
-        'all_converged': bool(all(convs)),
+        'all_converged': bool(all(convs) and all(bconvs)),
         'vext_ms': vext_ms,
         'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
     }

@@ -13,6 +13,13 @@ m (m + 1), and that many iterations are captured once into a HIP graph and
 replayed until every instance has finished.  The host only reads the status
 block between graph launches.
 
+Instances are grouped in chunks (``chunk`` per group, each group its own
+plan and history rings); within an iteration every group runs its whole
+pipeline before the next group starts, so the intermediate spectra one pass
+writes and the next reads stay in the 256 MB Infinity Cache for grids where a
+single instance's working set fits (128^3), while small grids use large
+groups to fill the GPU.
+
 Per instance the arithmetic is that of the plain ``Anderson(method='anderson')``
 solver (same kernels, same tiles, same reduction order), so a state point that
 converges without the grand-potential safeguard firing gives the same result
@@ -38,7 +45,7 @@ STATUS = {0: 'running', 1: 'converged', 2: 'step limit', 3: 'fallback', 4: 'non-
 class BatchedAnderson:
     def __init__(self, program, chempots, m=5, damping=0.1, threshold=1e-8, nsteps=500, use_graph=True,
                  adaptive_damping=True, damping_max=0.8, damping_min=0.01, damping_factors=(1.5, 0.5),
-                 resolve_fallbacks=True):
+                 resolve_fallbacks=True, chunk=None):
         assert 1 <= m <= L.MMAX
         self.program = program
         self.grid = program.grid
@@ -53,7 +60,12 @@ class BatchedAnderson:
         self.dmax, self.dmin = damping_max, damping_min
         self.factors = damping_factors
         self.resolve_fallbacks = resolve_fallbacks
-        self._plan = None
+        if chunk is None:
+            # about 256 MB of spectra (7 half spectra per instance) per group
+            per_inst = 7 * 16 * self.grid.npoints[0] * self.grid.npoints[1] * (self.grid.npoints[2] // 2 + 1)
+            chunk = max(1, int(256e6 // per_inst))
+        self.chunk = int(chunk)
+        self._groups = None
         self._graph = None
 
     # ------------------------------------------------------------------
@@ -75,92 +87,110 @@ class BatchedAnderson:
         self.beta = fener.beta
         self.lam3 = fener.wavelength ** 3
         self.V = V.contiguous() if V is not None else None
-        if self._plan is None or self._plan.nbatch != B:
+        sizes = [min(self.chunk, B - i) for i in range(0, B, self.chunk)]
+        if self._groups is None or [g['n'] for g in self._groups] != sizes:
             from ._lib import Plan
+            self.close()
             self.stream = torch.cuda.Stream()
-            self._plan = Plan(grid.npoints, grid.cell.rvecs, nbatch=B)
-            self._plan.bind_stream(self.stream)
             n = grid.npts
-            self.rho_ring = torch.empty((self.m + 1, B, n), dtype=torch.float64, device='cuda')
-            self.res_ring = torch.zeros((self.m, B, n), dtype=torch.float64, device='cuda')
-            if self._graph is not None:
-                self._plan.graph_destroy(self._graph)
-                self._graph = None
-        self._plan.set_functional(build_functional(hs=hs, wda=wda, kT=1.0 / self.beta))
-        fug = np.exp(self.beta * self.chempots) / self.beta / self.lam3
-        self._plan.set_instances(fug, np.zeros(B))
-        return self._plan
+            self._groups = []
+            first = 0
+            for nb in sizes:
+                plan = Plan(grid.npoints, grid.cell.rvecs, nbatch=nb)
+                plan.bind_stream(self.stream)
+                self._groups.append(dict(
+                    n=nb, first=first, plan=plan,
+                    rho_ring=torch.empty((self.m + 1, nb, n), dtype=torch.float64, device='cuda'),
+                    res_ring=torch.zeros((self.m, nb, n), dtype=torch.float64, device='cuda')))
+                first += nb
+        func = build_functional(hs=hs, wda=wda, kT=1.0 / self.beta)
+        for g in self._groups:
+            mu = self.chempots[g['first']:g['first'] + g['n']]
+            g['mu'] = mu
+            g['plan'].set_functional(func)
+            g['plan'].set_instances(np.exp(self.beta * mu) / self.beta / self.lam3, np.zeros(g['n']))
 
     def _iteration(self, s):
-        """Enqueue batched Anderson step s (1-based)."""
-        m, p = self.m, self._plan
+        """Enqueue batched Anderson step s (1-based) for every group."""
+        m = self.m
         cur, rslot, new = (s - 1) % (m + 1), (s - 1) % m, s % (m + 1)
         mk = min(s, m)
         ts = range(s - mk, s)
         order_r = [t % m for t in ts]
         order_rho = [t % (m + 1) for t in ts]
         slot_prev = (s - 2) % m if s >= 2 else -1
-        p.finish_anderson(self.rho_ring[cur], self.V, False, self.res_ring, m, rslot, None, self.beta, self.lam3)
-        p.anderson_coef(m, order_r, rslot, slot_prev, int(self.adaptive), self.factors[0], self.factors[1],
-                        self.dmin, self.dmax)
-        p.anderson_mix(self.rho_ring, order_rho, self.res_ring, order_r, self.rho_ring[cur], self.rho_ring[new],
-                       self.V, False, self.lam3)
-        p.derive()
-        p.batch_check(new)
+        for g in self._groups:
+            p, rr, rs = g['plan'], g['rho_ring'], g['res_ring']
+            p.finish_anderson(rr[cur], self.V, False, rs, m, rslot, None, self.beta, self.lam3)
+            p.anderson_coef(m, order_r, rslot, slot_prev, int(self.adaptive), self.factors[0], self.factors[1],
+                            self.dmin, self.dmax)
+            p.anderson_mix(rr, order_rho, rs, order_r, rr[cur], rr[new], self.V, False, self.lam3)
+            p.derive()
+            p.batch_check(new)
+
+    def _state(self):
+        sts = [g['plan'].batch_state() for g in self._groups]
+        return {k: np.concatenate([st[k] for st in sts]) for k in sts[0]}
 
     def solve(self, rho0=None):
         """Returns dict(N, rho (B, nx, ny, nz) device tensor, converged, iters,
-        status, omega)."""
+        status, omega, resolved)."""
         torch = torch_mod()
-        p = self._setup()
+        self._setup()
         B, m = len(self.chempots), self.m
         self.stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(self.stream):
-            if rho0 is None:
-                # ideal-gas initial densities (program.py:296-305), one per state point
-                Vd = self.V if self.V is not None else torch.zeros(self.grid.npts, dtype=torch.float64, device='cuda')
-                mu = torch.from_numpy(self.chempots).to('cuda')[:, None]
-                self.rho_ring[0].copy_(torch.exp(self.beta * (mu - Vd[None, :])) / self.lam3)
-            else:
-                self.rho_ring[0].copy_(as_device(rho0).reshape(B, -1))
-            self.res_ring.zero_()
-            p.anderson_init(np.full(B, self.damping), np.ones(B))
-            p.batch_reset(self.chempots, self.beta, self.lam3, self.grid.dr, self.threshold, self.nsteps)
-            p.seed(self.rho_ring[0], self.V, False, self.lam3)
-            p.derive()
-            p.batch_check(0, init=True)
+            Vd = self.V if self.V is not None else torch.zeros(self.grid.npts, dtype=torch.float64, device='cuda')
+            for g in self._groups:
+                p, rr = g['plan'], g['rho_ring']
+                if rho0 is None:
+                    # ideal-gas initial densities (program.py:296-305), one per state point
+                    mu = torch.from_numpy(g['mu']).to('cuda')[:, None]
+                    rr[0].copy_(torch.exp(self.beta * (mu - Vd[None, :])) / self.lam3)
+                else:
+                    rr[0].copy_(as_device(rho0).reshape(B, -1)[g['first']:g['first'] + g['n']])
+                g['res_ring'].zero_()
+                p.anderson_init(np.full(g['n'], self.damping), np.ones(g['n']))
+                p.batch_reset(g['mu'], self.beta, self.lam3, self.grid.dr, self.threshold, self.nsteps)
+                p.seed(rr[0], self.V, False, self.lam3)
+                p.derive()
+                p.batch_check(0, init=True)
             s = 0
             # warm-up: the first m steps have growing history lengths
             while s < m and s < self.nsteps:
                 s += 1
                 self._iteration(s)
-            st = p.batch_state()
+            st = self._state()
             period = m * (m + 1)
             if self.use_graph and st['active'].any() and s < self.nsteps:
+                p0 = self._groups[0]['plan']
                 if self._graph is None:
-                    p.graph_begin()
+                    p0.graph_begin()
                     try:
                         for k in range(period):
                             self._iteration(m + 1 + k)
                     finally:
-                        self._graph = p.graph_end()
+                        self._graph = p0.graph_end()
                 while st['active'].any() and s < self.nsteps:
-                    p.graph_launch(self._graph)
+                    p0.graph_launch(self._graph)
                     s += period
-                    st = p.batch_state()
+                    st = self._state()
             else:
                 while st['active'].any() and s < self.nsteps:
                     s += 1
                     self._iteration(s)
                     if s % 8 == 0:
-                        st = p.batch_state()
-                st = p.batch_state()
-            scal = p.scalars()
+                        st = self._state()
+                st = self._state()
+            scal = np.concatenate([g['plan'].scalars() for g in self._groups])
             rho = torch.empty((B,) + tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
-            for i in range(B):
-                rho[i].copy_(self.rho_ring[st['final_slot'][i], i].reshape(self.grid.npoints))
+            for g in self._groups:
+                for j in range(g['n']):
+                    i = g['first'] + j
+                    rho[i].copy_(g['rho_ring'][st['final_slot'][i], j].reshape(self.grid.npoints))
         self.stream.synchronize()
-        p.batch_release()
+        for g in self._groups:
+            g['plan'].batch_release()
         N = scal[:, L.S_PN] * self.grid.dr
         status = st['status'].copy()
         iters = st['iters'].copy()
@@ -170,7 +200,7 @@ class BatchedAnderson:
                               criterion='RIUE', threshold=self.threshold, adaptive_damping=self.adaptive,
                               damping_max=self.dmax, damping_min=self.dmin, damping_factors=self.factors)
             mu = self.chempots[i]
-            r0 = torch.exp(self.beta * (mu - self.V.reshape(self.grid.npoints))) / self.lam3 if rho0 is None \
+            r0 = torch.exp(self.beta * (mu - Vd.reshape(self.grid.npoints))) / self.lam3 if rho0 is None \
                 else as_device(rho0)[i]
             Ni, ri, ci = solver.solve(mu, r0)
             rho[i].copy_(ri.reshape(self.grid.npoints))
@@ -181,6 +211,6 @@ class BatchedAnderson:
                     resolved=redo)
 
     def close(self):
-        if self._graph is not None and self._plan is not None:
-            self._plan.graph_destroy(self._graph)
-            self._graph = None
+        if self._graph is not None and self._groups:
+            self._groups[0]['plan'].graph_destroy(self._graph)
+        self._graph = None

@@ -83,7 +83,12 @@ bool radix_plan(int n, Fft1D& f) {
   while (m % 4 == 0) { f.rad[f.nst++] = 4; m /= 4; }
   while (m % 2 == 0) { f.rad[f.nst++] = 2; m /= 2; }
   if (m != 1 || f.nst > CMM_MAX_STAGES) return false;
-  if (f.nst == 0) { f.rad[0] = 2; f.nst = 0; }
+  int Ns = 1;
+  for (int s = 0; s < f.nst; ++s) {
+    f.fns[s] = make_fastdiv((unsigned)Ns);
+    f.fnq[s] = make_fastdiv((unsigned)(n / f.rad[s]));
+    Ns *= f.rad[s];
+  }
   return true;
 }
 
@@ -102,11 +107,13 @@ void default_tiles(cmm_plan* p) {
   p->tbx = std::max(1, tbx);
   int tby = std::min(32, std::min(budget / p->ny, 4096 / p->ny));
   p->tby = std::max(1, tby);
-  // simple z pass: (P/2)*nz <= budget and P*nz*8 <= 64 KB
-  int pz = std::min(32, std::min(2 * budget / p->nz, 8192 / p->nz));
+  // simple z pass: (P/2)*nz <= budget and nz*((P/2)|1)*16 <= 64 KB
+  int pz = std::min(32, 2 * budget / p->nz);
+  while (pz > 2 && (size_t)p->nz * ((pz / 2) | 1) * 16 > 65536) pz -= 2;
   p->pz = largest_even_le(pz);
-  // 6-field z pass: 6*(P/2)*nz <= budget, 6*P*nz*8 <= 64 KB
-  int pphi = std::min(16, std::min(budget / (3 * p->nz), 1365 / p->nz));
+  // 6-field z pass: 3*P*nz <= budget, nz*((3P)|1)*16 <= 64 KB
+  int pphi = std::min(16, budget / (3 * p->nz));
+  while (pphi > 2 && (size_t)p->nz * ((3 * pphi) | 1) * 16 > 65536) pphi -= 2;
   p->pphi = largest_even_le(pphi);
 }
 
@@ -114,7 +121,19 @@ RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->counters,
 
 long long zblocks(cmm_plan* p, int P) { return ((long long)p->nx * p->ny + P - 1) / P; }
 
-size_t zlds(cmm_plan* p, int P, int nf) { return (size_t)nf * (P / 2) * p->nz * sizeof(double2); }
+ZTile ztile(cmm_plan* p, int P, int nf) {
+  ZTile t;
+  t.P = P;
+  t.B = P / 2;
+  t.S = nf * t.B;
+  t.ES = t.S | 1;
+  t.fn = make_fastdiv((unsigned)p->nz);
+  t.fnzh = make_fastdiv((unsigned)p->nzh);
+  t.fS = make_fastdiv((unsigned)t.S);
+  return t;
+}
+
+size_t zlds(cmm_plan* p, int P, int nf) { return (size_t)p->nz * ((nf * (P / 2)) | 1) * sizeof(double2); }
 
 }  // namespace
 
@@ -127,7 +146,8 @@ const char* cmm_last_error(void) { return g_err.c_str(); }
 int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm_plan** out) {
   if (!out || !rvecs) return fail(CMM_E_ARG, "null argument");
   if (nx < 2 || ny < 2 || nz < 2 || nbatch < 1 || nbatch > 4096) return fail(CMM_E_ARG, "bad grid or batch size");
-  if (nx > 512 || ny > 512 || nz > 512) return fail(CMM_E_ARG, "grid axis longer than 512 is not supported");
+  if (nx > CMM_MAX_N || ny > CMM_MAX_N || nz > CMM_MAX_N)
+    return fail(CMM_E_ARG, "grid axis longer than 256 is not supported");
   cmm_plan* p = new cmm_plan();
   std::memset(p, 0, sizeof(*p));
   p->nx = nx; p->ny = ny; p->nz = nz; p->nzh = nz / 2 + 1; p->nb = nbatch;
@@ -243,11 +263,11 @@ int cmm_plan_set_tiles(cmm_plan* p, int tbx, int tby, int pz, int pphi) {
     p->tby = tby;
   }
   if (pz > 0) {
-    if ((pz & 1) || (pz / 2) * p->nz > budget || pz * p->nz * 8 > 65536) return fail(CMM_E_ARG, "p_z invalid");
+    if ((pz & 1) || (pz / 2) * p->nz > budget || zlds(p, pz, 1) > 65536) return fail(CMM_E_ARG, "p_z invalid");
     p->pz = pz;
   }
   if (pphi > 0) {
-    if ((pphi & 1) || 3 * pphi * p->nz > budget || 6 * pphi * p->nz * 8 > 65536) return fail(CMM_E_ARG, "p_phi invalid");
+    if ((pphi & 1) || 3 * pphi * p->nz > budget || zlds(p, pphi, 6) > 65536) return fail(CMM_E_ARG, "p_phi invalid");
     p->pphi = pphi;
   }
   return CMM_OK;
@@ -313,8 +333,11 @@ static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
   dim3 grid(p->nx * ntk, nf, p->nb);
   size_t lds = (size_t)p->ny * p->tby * sizeof(double2);
   long long is = (nf == 1 && F == p->rz) ? p->nhalf : 6 * p->nhalf;
-  if (sgn < 0) hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, p->active);
-  else hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, p->active);
+  const FastDiv fT = make_fastdiv((unsigned)p->tby);
+  if (sgn < 0)
+    hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, p->active);
+  else
+    hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, p->active);
   LAUNCHCHK("k_ypass");
   return CMM_OK;
 }
@@ -323,8 +346,11 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
   const long long nb = (long long)p->ny * p->nzh;
   dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
   size_t lds = (size_t)p->nx * p->tbx * sizeof(double2);
-  if (sgn < 0) hipLaunchKernelGGL(k_xpass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tbx, p->active);
-  else hipLaunchKernelGGL(k_xpass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tbx, p->active);
+  const FastDiv fT = make_fastdiv((unsigned)p->tbx);
+  if (sgn < 0)
+    hipLaunchKernelGGL(k_xpass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tbx, fT, p->active);
+  else
+    hipLaunchKernelGGL(k_xpass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tbx, fT, p->active);
   LAUNCHCHK("k_xpass");
   return CMM_OK;
 }
@@ -333,15 +359,15 @@ int cmm_fft_r2c(cmm_plan* p, const double* x, double* X) {
   if (!p || !x || !X) return fail(CMM_E_ARG, "null argument");
   double2* Xc = reinterpret_cast<double2*>(X);
   dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
-  hipLaunchKernelGGL(k_zr2c, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, x, p->npts, Xc, p->nhalf, p->g, p->pz,
-                     p->active);
+  hipLaunchKernelGGL(k_zr2c, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, x, p->npts, Xc, p->nhalf, p->g,
+                     ztile(p, p->pz, 1), p->active);
   LAUNCHCHK("k_zr2c");
   // y and x passes on the caller's buffer (instance stride nhalf)
   {
     const int ntk = (p->nzh + p->tby - 1) / p->tby;
     dim3 gy(p->nx * ntk, 1, p->nb);
     hipLaunchKernelGGL(k_ypass<-1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, Xc, p->nhalf,
-                       p->nhalf, p->g, p->tby, p->active);
+                       p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
     LAUNCHCHK("k_ypass");
   }
   return launch_xpass(p, Xc, p->nhalf, -1);
@@ -357,12 +383,12 @@ int cmm_fft_c2r(cmm_plan* p, const double* X, double* x) {
     const int ntk = (p->nzh + p->tby - 1) / p->tby;
     dim3 gy(p->nx * ntk, 1, p->nb);
     hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->rz, p->nhalf,
-                       p->nhalf, p->g, p->tby, p->active);
+                       p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
     LAUNCHCHK("k_ypass");
   }
   dim3 grid((unsigned)zblocks(p, p->pz), 1, p->nb);
   hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->rz, p->nhalf, 0LL, x, p->npts, 0LL,
-                     p->g, p->pz, p->active);
+                     p->g, ztile(p, p->pz, 1), p->active);
   LAUNCHCHK("k_zc2r");
   return CMM_OK;
 }
@@ -384,8 +410,8 @@ static ElDev el_base(cmm_plan* p, const double* rho, const double* vext, int vb,
 template <int MODE>
 static int launch_produce(cmm_plan* p, const ElDev& el, const char* name) {
   dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
-  hipLaunchKernelGGL(k_zproduce<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, el, p->pz, p->active,
-                     red_of(p));
+  hipLaunchKernelGGL(k_zproduce<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, el, ztile(p, p->pz, 1),
+                     p->active, red_of(p));
   LAUNCHCHK(name);
   return CMM_OK;
 }
@@ -404,7 +430,7 @@ static int forward_chain(cmm_plan* p) {
   dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
   size_t lds = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
   hipLaunchKernelGGL(k_xweights, grid, dim3(NT), lds, p->stream, p->rz, p->nhalf, p->F, 6 * p->nhalf, p->nhalf, p->g,
-                     p->fp, p->tbx, p->active);
+                     p->fp, p->tbx, make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
   LAUNCHCHK("k_xweights");
   return launch_ypass(p, p->F, 6, 1);
 }
@@ -412,7 +438,7 @@ static int forward_chain(cmm_plan* p) {
 static int launch_zphi(cmm_plan* p, int store) {
   dim3 grid((unsigned)zblocks(p, p->pphi), p->nb);
   hipLaunchKernelGGL(k_zphi, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g,
-                     p->fp, p->pphi, store, p->active, red_of(p));
+                     p->fp, ztile(p, p->pphi, 6), store, p->active, red_of(p));
   LAUNCHCHK("k_zphi");
   return CMM_OK;
 }
@@ -429,13 +455,13 @@ int cmm_derive(cmm_plan* p) {
   dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
   size_t lds = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
   hipLaunchKernelGGL(k_xback, grid, dim3(NT), lds, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
-                     p->active);
+                     make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
   LAUNCHCHK("k_xback");
   // inverse y of the sum (field 0 of F)
   const int ntk = (p->nzh + p->tby - 1) / p->tby;
   dim3 gy(p->nx * ntk, 1, p->nb);
   hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->F,
-                     6 * p->nhalf, p->nhalf, p->g, p->tby, p->active);
+                     6 * p->nhalf, p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
   LAUNCHCHK("k_ypass");
   return CMM_OK;
 }
@@ -457,12 +483,13 @@ int cmm_launch_pass(cmm_plan* p, int which) {
     case 0: return launch_zphi(p, 1);
     case 1:
       hipLaunchKernelGGL(k_xweights, gx, dim3(NT), ldsx, p->stream, p->rz, p->nhalf, p->F, 6 * p->nhalf, p->nhalf,
-                         p->g, p->fp, p->tbx, p->active);
+                         p->g, p->fp, p->tbx, make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh),
+                         p->active);
       LAUNCHCHK("k_xweights");
       return CMM_OK;
     case 2:
       hipLaunchKernelGGL(k_xback, gx, dim3(NT), ldsx, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
-                         p->active);
+                         make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
       LAUNCHCHK("k_xback");
       return CMM_OK;
     case 3: return launch_ypass(p, p->F, 6, -1);
@@ -477,7 +504,7 @@ int cmm_weighted_densities(cmm_plan* p, double* out) {
   if (rc) return rc;
   dim3 grid((unsigned)zblocks(p, p->pz), 6, p->nb);
   hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->nhalf, out,
-                     6 * p->npts, p->npts, p->g, p->pz, p->active);
+                     6 * p->npts, p->npts, p->g, ztile(p, p->pz, 1), p->active);
   LAUNCHCHK("k_zc2r");
   return CMM_OK;
 }
@@ -485,8 +512,8 @@ int cmm_weighted_densities(cmm_plan* p, double* out) {
 template <int MODE>
 static int launch_final(cmm_plan* p, const ElDev& el, const char* name) {
   dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
-  hipLaunchKernelGGL(k_zfinal<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, el, p->pz,
-                     p->active, red_of(p));
+  hipLaunchKernelGGL(k_zfinal<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, el,
+                     ztile(p, p->pz, 1), p->active, red_of(p));
   LAUNCHCHK(name);
   return CMM_OK;
 }

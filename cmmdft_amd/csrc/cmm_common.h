@@ -6,15 +6,38 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define CMM_MAX_STAGES 10
+#define CMM_MAX_STAGES 8
+#define CMM_MAX_N 256   // longest transform axis (twiddles are staged in LDS)
 #define CMM_MAX_INST 256
 
-// One 1-D transform length: radix schedule and twiddle table
-// tw[j] = exp(-2*pi*i*j/n), j in [0, n).
+// Unsigned division by a runtime constant d as a multiply-high and a shift
+// (round-up method, Granlund & Montgomery 1994): q = (umulhi(x, m) + x) >> l
+// with l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1.  Exact for
+// x < 2^31 (all indices here are far below that).
+struct FastDiv {
+  unsigned d, m, l;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  unsigned l = 0;
+  while ((1u << l) < d) ++l;
+  f.l = l;
+  f.m = (unsigned)(((((unsigned long long)1) << 32) * ((((unsigned long long)1) << l) - d)) / d + 1);
+  return f;
+}
+
+__device__ __forceinline__ unsigned fdiv(unsigned x, const FastDiv& f) { return (__umulhi(x, f.m) + x) >> f.l; }
+
+// One 1-D transform length: radix schedule, per-stage divisors and the
+// twiddle table tw[j] = exp(-2*pi*i*j/n), j in [0, n).
 struct Fft1D {
   int n;
   int nst;
   int rad[CMM_MAX_STAGES];
+  FastDiv fns[CMM_MAX_STAGES];   // Ns of the stage (product of the previous radices)
+  FastDiv fnq[CMM_MAX_STAGES];   // n / radix of the stage
   const double2* tw;
 };
 
@@ -131,7 +154,8 @@ __device__ __forceinline__ void dftr(double2* v) {
 // in registers: all lanes load, barrier, all lanes store, barrier.
 // ---------------------------------------------------------------------------
 template <int R, int SGN, bool EM, int NB>
-__device__ __forceinline__ void stockham_stage(double2* buf, int batch, int bs, int es, const Fft1D& p, int Ns) {
+__device__ __forceinline__ void stockham_stage(double2* buf, int batch, const FastDiv& fb, int bs, int es, const Fft1D& p,
+                                               const double2* tws, int st, int Ns) {
   const int n = p.n;
   const int nq = n / R;                 // butterflies per sequence
   const int total = batch * nq;
@@ -142,26 +166,28 @@ __device__ __forceinline__ void stockham_stage(double2* buf, int batch, int bs, 
   int dst[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
-    int b = tid + i * nt;
+    const int b = tid + i * nt;
     dst[i] = -1;
     if (b < total) {
       int s, j;
-      if (EM) { s = b % batch; j = b / batch; }
-      else { j = b % nq; s = b / nq; }
-      const int k = j % Ns;
+      if (EM) { j = (int)fdiv((unsigned)b, fb); s = b - j * batch; }
+      else { s = (int)fdiv((unsigned)b, p.fnq[st]); j = b - s * nq; }
+      const int jn = (int)fdiv((unsigned)j, p.fns[st]);
+      const int k = j - jn * Ns;
       double2* base = buf + s * bs;
 #pragma unroll
       for (int t = 0; t < R; ++t) v[i][t] = base[(j + t * nq) * es];
       if (Ns > 1) {
+        // t k twstride < R Ns (n / (Ns R)) = n: no wrap-around
 #pragma unroll
         for (int t = 1; t < R; ++t) {
-          double2 w = p.tw[(t * k * twstride) % n];
+          double2 w = tws[t * k * twstride];
           if (SGN > 0) w.y = -w.y;
           v[i][t] = cmul(v[i][t], w);
         }
       }
       dftr<R, SGN>(v[i]);
-      dst[i] = s * bs + ((j / Ns) * Ns * R + k) * es;
+      dst[i] = s * bs + (jn * Ns * R + k) * es;
     }
   }
   __syncthreads();
@@ -175,24 +201,33 @@ __device__ __forceinline__ void stockham_stage(double2* buf, int batch, int bs, 
   __syncthreads();
 }
 
-// Full transform of `batch` sequences of length p.n, in place in LDS.
-// VPT bounds the complex values one thread holds per stage: the caller must
-// guarantee batch * n <= VPT * blockDim.x, and that a barrier separates the
-// data's producers from this call.  Ends with a barrier.
+// Full transform of `batch` sequences of length p.n, in place in LDS;
+// element e of sequence s at buf[s*bs + e*es]; tws: the twiddle table in LDS.  EM (element-major): the
+// sequences are the fast index (bs = 1, es >= batch), so consecutive lanes
+// touch consecutive addresses in every stage.  VPT bounds the complex values
+// one thread holds per stage: the caller must guarantee
+// batch * n <= VPT * blockDim.x, and that a barrier separates the data's
+// producers from this call.  Ends with a barrier.
 template <int SGN, bool EM, int VPT>
-__device__ void lds_fft(double2* buf, int batch, int bs, int es, const Fft1D& p) {
+__device__ void lds_fft(double2* buf, int batch, const FastDiv& fb, int bs, int es, const Fft1D& p,
+                        const double2* tws) {
   int Ns = 1;
   for (int s = 0; s < p.nst; ++s) {
     const int r = p.rad[s];
     switch (r) {
-      case 2: stockham_stage<2, SGN, EM, (VPT + 1) / 2>(buf, batch, bs, es, p, Ns); break;
-      case 3: stockham_stage<3, SGN, EM, (VPT + 2) / 3>(buf, batch, bs, es, p, Ns); break;
-      case 4: stockham_stage<4, SGN, EM, (VPT + 3) / 4>(buf, batch, bs, es, p, Ns); break;
-      case 5: stockham_stage<5, SGN, EM, (VPT + 4) / 5>(buf, batch, bs, es, p, Ns); break;
-      default: stockham_stage<8, SGN, EM, (VPT + 7) / 8>(buf, batch, bs, es, p, Ns); break;
+      case 2: stockham_stage<2, SGN, EM, (VPT + 1) / 2>(buf, batch, fb, bs, es, p, tws, s, Ns); break;
+      case 3: stockham_stage<3, SGN, EM, (VPT + 2) / 3>(buf, batch, fb, bs, es, p, tws, s, Ns); break;
+      case 4: stockham_stage<4, SGN, EM, (VPT + 3) / 4>(buf, batch, fb, bs, es, p, tws, s, Ns); break;
+      case 5: stockham_stage<5, SGN, EM, (VPT + 4) / 5>(buf, batch, fb, bs, es, p, tws, s, Ns); break;
+      default: stockham_stage<8, SGN, EM, (VPT + 7) / 8>(buf, batch, fb, bs, es, p, tws, s, Ns); break;
     }
     Ns *= r;
   }
+}
+
+// stage an axis' twiddle table into LDS (caller adds the barrier)
+__device__ __forceinline__ void load_twiddles(double2* tws, const Fft1D& p) {
+  for (int i = threadIdx.x; i < p.n; i += blockDim.x) tws[i] = p.tw[i];
 }
 
 // ---------------------------------------------------------------------------

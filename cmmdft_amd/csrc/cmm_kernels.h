@@ -20,6 +20,15 @@ constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (fi
 constexpr int S_GRAM = S_NCUR + 4;
 constexpr int S_PN = 16;    // N', sum|d|, max|d|, nonfinite, ideal', ext'  (density producers)
 
+// A z-pass tile: P pencils (B = P/2 complex sequences per field), S = nf*B
+// sequences in all, element row stride ES (odd, >= S) in LDS.
+struct ZTile {
+  int P, B, S, ES;
+  FastDiv fn;     // divide by nz
+  FastDiv fnzh;   // divide by nz/2 + 1
+  FastDiv fS;     // divide by S
+};
+
 struct GridDev {
   int nx, ny, nz, nzh;
   double inv_npts;

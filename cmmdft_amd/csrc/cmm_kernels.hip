@@ -17,35 +17,46 @@
 namespace cmm {
 
 constexpr int NT = 256;   // threads per block of the FFT passes
-constexpr int VPT = 16;   // complex values per thread per Stockham stage
+constexpr int VPT = 8;    // complex values per thread per Stockham stage
 constexpr int EPTX = 8;   // x-pass tile elements per thread (nx * TB <= EPTX * NT)
 
 // ---------------------------------------------------------------------------
-// z-pass helpers (pencils along the contiguous axis, two real pencils per
-// complex sequence).  LDS holds (P/2) sequences of length n: seq q = pencils
-// p0+2q (real part) and p0+2q+1 (imag part).
+// z-pass tiles.  A block owns P consecutive z pencils (two real pencils per
+// complex sequence: pair q = pencils p0+2q (real part) and p0+2q+1 (imag
+// part)) of nf fields.  LDS layout is element-major with an odd row stride:
+// element e of sequence s = f*B + q sits at lds[e*ES + s], B = P/2, ES >= nf*B
+// and odd, so both the transposing global<->LDS copies and every Stockham
+// stage are free of LDS bank conflicts.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void z_load_real(double2* lds, const double* src, long long p0, int P, long long npen, int n) {
-  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
-    const int pl = i / n, e = i - pl * n;
+__device__ __forceinline__ double& z_val(double2* lds, const ZTile& t, int f, int pl, int e) {
+  return reinterpret_cast<double*>(lds)[2 * (e * t.ES + f * t.B + (pl >> 1)) + (pl & 1)];
+}
+
+__device__ __forceinline__ void z_load_real(double2* lds, const double* src, long long p0, const ZTile& t, long long npen,
+                                            int n, int f) {
+  const int tot = t.P * n;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
     const long long p = p0 + pl;
-    const double v = (p < npen) ? src[p * n + e] : 0.0;
-    reinterpret_cast<double*>(&lds[(pl >> 1) * n + e])[pl & 1] = v;
+    z_val(lds, t, f, pl, e) = (p < npen) ? src[p * n + e] : 0.0;
   }
 }
 
 // forward-transformed pairs -> two half spectra
-__device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, long long p0, int P, long long npen, int n,
-                                             int nzh) {
-  for (int i = threadIdx.x; i < (P >> 1) * nzh; i += blockDim.x) {
-    const int q = i / nzh, k = i - q * nzh;
-    const double2 zk = lds[q * n + k];
-    const double2 zm = lds[q * n + (k == 0 ? 0 : n - k)];
+__device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, long long p0, const ZTile& t, long long npen,
+                                             int n, int nzh, int f) {
+  const int tot = t.B * nzh;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
+    const double2 zk = lds[k * t.ES + f * t.B + q];
+    const double2 zm = lds[(k == 0 ? 0 : n - k) * t.ES + f * t.B + q];
     const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-    const double2 B = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    const double2 Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
     const long long pa = p0 + 2 * q, pb = pa + 1;
     if (pa < npen) dst[pa * nzh + k] = A;
-    if (pb < npen) dst[pb * nzh + k] = B;
+    if (pb < npen) dst[pb * nzh + k] = Bv;
   }
 }
 
@@ -53,65 +64,119 @@ __device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, l
 // The k = 0 and k = n/2 coefficients are projected onto their real parts,
 // i.e. the Hermitian part is taken, which is what the real part of a full
 // complex inverse transform does.
-__device__ __forceinline__ void z_load_half(double2* lds, const double2* src, long long p0, int P, long long npen, int n,
-                                            int nzh) {
-  for (int i = threadIdx.x; i < (P >> 1) * nzh; i += blockDim.x) {
-    const int q = i / nzh, k = i - q * nzh;
+__device__ __forceinline__ void z_load_half(double2* lds, const double2* src, long long p0, const ZTile& t, long long npen,
+                                            int n, int nzh, int f) {
+  const int tot = t.B * nzh;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
     const long long pa = p0 + 2 * q, pb = pa + 1;
     double2 A = (pa < npen) ? src[pa * nzh + k] : make_double2(0.0, 0.0);
     double2 B = (pb < npen) ? src[pb * nzh + k] : make_double2(0.0, 0.0);
     const bool selfconj = (k == 0) || (2 * k == n);
     if (selfconj) { A.y = 0.0; B.y = 0.0; }
-    lds[q * n + k] = make_double2(A.x - B.y, A.y + B.x);
-    if (!selfconj) lds[q * n + n - k] = make_double2(A.x + B.y, B.x - A.y);
+    lds[k * t.ES + f * t.B + q] = make_double2(A.x - B.y, A.y + B.x);
+    if (!selfconj) lds[(n - k) * t.ES + f * t.B + q] = make_double2(A.x + B.y, B.x - A.y);
   }
 }
 
-__device__ __forceinline__ double& z_val(double2* lds, int pl, int e, int n) {
-  return reinterpret_cast<double*>(&lds[(pl >> 1) * n + e])[pl & 1];
+// nf half-spectrum fields (field stride fs) into the tile in one loop, so
+// every thread has several independent loads in flight
+__device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* src, long long fs, int nf, long long p0,
+                                                  const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn) {
+  const int per = t.B * nzh;
+  const int tot = nf * per;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int f = (int)fdiv((unsigned)i, fBn);
+    const int r = i - f * per;
+    const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
+    const long long pa = p0 + 2 * q, pb = pa + 1;
+    const double2* sf = src + f * fs;
+    double2 A = (pa < npen) ? sf[pa * nzh + k] : make_double2(0.0, 0.0);
+    double2 B = (pb < npen) ? sf[pb * nzh + k] : make_double2(0.0, 0.0);
+    const bool selfconj = (k == 0) || (2 * k == n);
+    if (selfconj) { A.y = 0.0; B.y = 0.0; }
+    lds[k * t.ES + f * t.B + q] = make_double2(A.x - B.y, A.y + B.x);
+    if (!selfconj) lds[(n - k) * t.ES + f * t.B + q] = make_double2(A.x + B.y, B.x - A.y);
+  }
+}
+
+__device__ __forceinline__ void z_store_half_multi(const double2* lds, double2* dst, long long fs, int nf, long long p0,
+                                                   const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn) {
+  const int per = t.B * nzh;
+  const int tot = nf * per;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int f = (int)fdiv((unsigned)i, fBn);
+    const int r = i - f * per;
+    const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
+    const double2 zk = lds[k * t.ES + f * t.B + q];
+    const double2 zm = lds[(k == 0 ? 0 : n - k) * t.ES + f * t.B + q];
+    const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+    const double2 Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    const long long pa = p0 + 2 * q, pb = pa + 1;
+    double2* df = dst + f * fs;
+    if (pa < npen) df[pa * nzh + k] = A;
+    if (pb < npen) df[pb * nzh + k] = Bv;
+  }
+}
+
+// all nf*B sequences of a z tile, in place
+template <int SGN>
+__device__ __forceinline__ void z_fft(double2* lds, const ZTile& t, const Fft1D& fz, const double2* tws) {
+  lds_fft<SGN, true, VPT>(lds, t.S, t.fS, 1, t.ES, fz, tws);
 }
 
 // ---------------------------------------------------------------------------
 // Plain transform passes (3-D FFT entry points and building blocks)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ x, long long xs, double2* __restrict__ X, long long Xs,
-                                            GridDev g, int P, const int* __restrict__ active) {
+                                            GridDev g, ZTile t, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fz);
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const long long npen = (long long)g.nx * g.ny;
-  const long long p0 = (long long)blockIdx.x * P;
-  z_load_real(sm, x + inst * xs, p0, P, npen, g.nz);
+  const long long p0 = (long long)blockIdx.x * t.P;
+  z_load_real(sm, x + inst * xs, p0, t, npen, g.nz, 0);
   __syncthreads();
-  lds_fft<-1, false, VPT>(sm, P >> 1, g.nz, 1, g.fz);
-  z_store_half(sm, X + inst * Xs, p0, P, npen, g.nz, g.nzh);
+  z_fft<-1>(sm, t, g.fz, s_tw);
+  z_store_half(sm, X + inst * Xs, p0, t, npen, g.nz, g.nzh, 0);
 }
 
-// nf fields, field stride fs (in complex / real elements)
+// field blockIdx.y of a batch of nf-field spectra -> real field
 __global__ void __launch_bounds__(NT) k_zc2r(const double2* __restrict__ X, long long Xs, long long Xfs, double* __restrict__ x,
-                                            long long xs, long long xfs, GridDev g, int P, const int* __restrict__ active) {
+                                            long long xs, long long xfs, GridDev g, ZTile t, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fz);
   const int inst = blockIdx.z;
   const int f = blockIdx.y;
   if (active && !active[inst]) return;
   const long long npen = (long long)g.nx * g.ny;
-  const long long p0 = (long long)blockIdx.x * P;
-  z_load_half(sm, X + inst * Xs + f * Xfs, p0, P, npen, g.nz, g.nzh);
+  const long long p0 = (long long)blockIdx.x * t.P;
+  z_load_half(sm, X + inst * Xs + f * Xfs, p0, t, npen, g.nz, g.nzh, 0);
   __syncthreads();
-  lds_fft<1, false, VPT>(sm, P >> 1, g.nz, 1, g.fz);
+  z_fft<1>(sm, t, g.fz, s_tw);
   double* out = x + inst * xs + f * xfs;
-  for (int i = threadIdx.x; i < P * g.nz; i += blockDim.x) {
-    const int pl = i / g.nz, e = i - pl * g.nz;
+  const int tot = t.P * g.nz;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * g.nz;
     const long long p = p0 + pl;
-    if (p < npen) out[p * g.nz + e] = z_val(sm, pl, e, g.nz);
+    if (p < npen) out[p * g.nz + e] = z_val(sm, t, 0, pl, e);
   }
 }
 
 // y pass: tile = one x plane, TB consecutive kz columns, all y.
 template <int SGN>
 __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
-                                             const int* __restrict__ active) {
+                                             FastDiv fTB, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fy);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const int ntk = (g.nzh + TB - 1) / TB;
@@ -119,14 +184,17 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   const int k0 = tk * TB;
   const int tbv = min(TB, g.nzh - k0);
   double2* base = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh + k0;
-  for (int i = threadIdx.x; i < g.ny * TB; i += blockDim.x) {
-    const int y = i / TB, b = i - y * TB;
+  const int tot = g.ny * TB;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
     sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  lds_fft<SGN, true, VPT>(sm, TB, 1, TB, g.fy);
-  for (int i = threadIdx.x; i < g.ny * TB; i += blockDim.x) {
-    const int y = i / TB, b = i - y * TB;
+  lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fy, s_tw);
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
     if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
   }
 }
@@ -134,22 +202,27 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
 // x pass: tile = TB consecutive (ky, kz) columns of the flattened plane, all x.
 template <int SGN>
 __global__ void __launch_bounds__(NT) k_xpass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
-                                             const int* __restrict__ active) {
+                                             FastDiv fTB, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = (long long)blockIdx.x * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   double2* base = F + inst * is + blockIdx.y * fs + b0;
-  for (int i = threadIdx.x; i < g.nx * TB; i += blockDim.x) {
-    const int x = i / TB, b = i - x * TB;
+  const int tot = g.nx * TB;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     sm[i] = (b < tbv) ? base[(long long)x * nb + b] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  lds_fft<SGN, true, VPT>(sm, TB, 1, TB, g.fx);
-  for (int i = threadIdx.x; i < g.nx * TB; i += blockDim.x) {
-    const int x = i / TB, b = i - x * TB;
+  lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fx, s_tw);
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     if (b < tbv) base[(long long)x * nb + b] = sm[i];
   }
 }
@@ -218,9 +291,11 @@ __device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix
 // copies -> inverse x FFT each -> F[0..5].  The 1/N of the inverse 3-D
 // transform is folded into the weights.
 __global__ void __launch_bounds__(NT) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
-                                                long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
-                                                const int* __restrict__ active) {
+                                                long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB, FastDiv fTB,
+                                                FastDiv fnzh, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const int tile = g.nx * TB;
@@ -230,22 +305,23 @@ __global__ void __launch_bounds__(NT) k_xweights(const double2* __restrict__ rz,
   const long long b0 = (long long)blockIdx.x * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = rz + inst * rzs + b0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-    const int x = i / TB, b = i - x * TB;
+    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     A[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  lds_fft<-1, true, VPT>(A, TB, 1, TB, g.fx);
+  lds_fft<-1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
   KW wr[EPTX];
   int wy[EPTX], wz[EPTX];
 #pragma unroll
   for (int e = 0; e < EPTX; ++e) {
     const int i = threadIdx.x + e * NT;
     if (i < tile) {
-      const int x = i / TB, b = i - x * TB;
-      const long long bb = min(b0 + b, nb - 1);
-      wy[e] = (int)(bb / g.nzh);
-      wz[e] = (int)(bb - (long long)wy[e] * g.nzh);
+      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+      const unsigned bb = (unsigned)min(b0 + b, nb - 1);
+      wy[e] = (int)fdiv(bb, fnzh);
+      wz[e] = (int)bb - wy[e] * g.nzh;
       wr[e] = kweights(g, fp, x, wy[e], wz[e]);
     }
   }
@@ -253,13 +329,14 @@ __global__ void __launch_bounds__(NT) k_xweights(const double2* __restrict__ rz,
 #pragma unroll
     for (int e = 0; e < EPTX; ++e) {
       const int i = threadIdx.x + e * NT;
-      if (i < tile) B[i] = cscale(apply_w(g, wr[e], i / TB, wy[e], wz[e], f, A[i], -1), g.inv_npts);
+      if (i < tile) B[i] = cscale(apply_w(g, wr[e], (int)fdiv((unsigned)i, fTB), wy[e], wz[e], f, A[i], -1), g.inv_npts);
     }
     __syncthreads();
-    lds_fft<1, true, VPT>(B, TB, 1, TB, g.fx);
+    lds_fft<1, true, VPT>(B, TB, fTB, 1, TB, g.fx, s_tw);
     double2* dst = F + inst * Fs + f * Ffs + b0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-      const int x = i / TB, b = i - x * TB;
+      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
       if (b < tbv) dst[(long long)x * nb + b] = B[i];
     }
     __syncthreads();
@@ -270,8 +347,10 @@ __global__ void __launch_bounds__(NT) k_xweights(const double2* __restrict__ rz,
 // weighted sum -> inverse x FFT -> F[0] (in place: every field's tile is read
 // before the sum is written).
 __global__ void __launch_bounds__(NT) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
-                                             int TB, const int* __restrict__ active) {
+                                             int TB, FastDiv fTB, FastDiv fnzh, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const int tile = g.nx * TB;
@@ -286,33 +365,35 @@ __global__ void __launch_bounds__(NT) k_xback(double2* __restrict__ F, long long
   for (int e = 0; e < EPTX; ++e) {
     const int i = threadIdx.x + e * NT;
     if (i < tile) {
-      const int x = i / TB, b = i - x * TB;
-      const long long bb = min(b0 + b, nb - 1);
-      wy[e] = (int)(bb / g.nzh);
-      wz[e] = (int)(bb - (long long)wy[e] * g.nzh);
+      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+      const unsigned bb = (unsigned)min(b0 + b, nb - 1);
+      wy[e] = (int)fdiv(bb, fnzh);
+      wz[e] = (int)bb - wy[e] * g.nzh;
       wr[e] = kweights(g, fp, x, wy[e], wz[e]);
       A[i] = make_double2(0.0, 0.0);
     }
   }
   for (int f = 0; f < 6; ++f) {
     const double2* src = F + inst * Fs + f * Ffs + b0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-      const int x = i / TB, b = i - x * TB;
+      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
       B[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    lds_fft<-1, true, VPT>(B, TB, 1, TB, g.fx);
+    lds_fft<-1, true, VPT>(B, TB, fTB, 1, TB, g.fx, s_tw);
 #pragma unroll
     for (int e = 0; e < EPTX; ++e) {
       const int i = threadIdx.x + e * NT;
-      if (i < tile) A[i] = cadd(A[i], cscale(apply_w(g, wr[e], i / TB, wy[e], wz[e], f, B[i], 1), g.inv_npts));
+      if (i < tile) A[i] = cadd(A[i], cscale(apply_w(g, wr[e], (int)fdiv((unsigned)i, fTB), wy[e], wz[e], f, B[i], 1), g.inv_npts));
     }
     __syncthreads();
   }
-  lds_fft<1, true, VPT>(A, TB, 1, TB, g.fx);
+  lds_fft<1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
   double2* dst = F + inst * Fs + b0;
+#pragma unroll 4
   for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-    const int x = i / TB, b = i - x * TB;
+    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     if (b < tbv) dst[(long long)x * nb + b] = A[i];
   }
 }
@@ -467,28 +548,31 @@ __device__ __forceinline__ void wda_point(const FuncDev& fp, double rb, double& 
 // -> C2R -> pointwise Phi / WDA derivatives -> R2C -> F[0..5] in place.
 // Reduces sum Phi, sum f_ex(rho_bar), max n3 (unclipped) per instance.
 __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
-                                            int P, int store, const int* __restrict__ active, RedDev red) {
+                                            ZTile t, int store, const int* __restrict__ active, RedDev red) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fz);
   __shared__ double sh[64];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;
   const long long npen = (long long)g.nx * g.ny;
-  const long long p0 = (long long)blockIdx.x * P;
-  const int seg = (P >> 1) * n;
-  for (int f = 0; f < 6; ++f) z_load_half(sm + f * seg, F + inst * Fs + f * Ffs, p0, P, npen, n, g.nzh);
+  const long long p0 = (long long)blockIdx.x * t.P;
+  const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
+  z_load_half_multi(sm, F + inst * Fs, Ffs, 6, p0, t, npen, n, g.nzh, fBn);
   __syncthreads();
-  lds_fft<1, false, VPT>(sm, 6 * (P >> 1), n, 1, g.fz);
+  z_fft<1>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
-  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
-    const int pl = i / n, e = i - pl * n;
+  const int tot = t.P * n;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
     if (p0 + pl >= npen) continue;
-    double& r0 = z_val(sm, pl, e, n);
-    double& r1 = z_val(sm + seg, pl, e, n);
-    double& r2 = z_val(sm + 2 * seg, pl, e, n);
-    double& r3 = z_val(sm + 3 * seg, pl, e, n);
-    double& r4 = z_val(sm + 4 * seg, pl, e, n);
-    double& r5 = z_val(sm + 5 * seg, pl, e, n);
+    double& r0 = z_val(sm, t, 0, pl, e);
+    double& r1 = z_val(sm, t, 1, pl, e);
+    double& r2 = z_val(sm, t, 2, pl, e);
+    double& r3 = z_val(sm, t, 3, pl, e);
+    double& r4 = z_val(sm, t, 4, pl, e);
+    double& r5 = z_val(sm, t, 5, pl, e);
     double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
     if (fp.use_hs) {
       const PhiOut o = fmt_point(fp, r0, r1, r2, r3, r4);
@@ -515,8 +599,8 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   }
   if (store) {
     __syncthreads();
-    lds_fft<-1, false, VPT>(sm, 6 * (P >> 1), n, 1, g.fz);
-    for (int f = 0; f < 6; ++f) z_store_half(sm + f * seg, F + inst * Fs + f * Ffs, p0, P, npen, n, g.nzh);
+    z_fft<-1>(sm, t, g.fz, s_tw);
+    z_store_half_multi(sm, F + inst * Fs, Ffs, 6, p0, t, npen, n, g.nzh, fBn);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
@@ -535,17 +619,19 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, long long Fs, GridDev g, ElDev el,
-                                              int P, const int* __restrict__ active, RedDev red) {
+                                              ZTile t, const int* __restrict__ active, RedDev red) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fz);
   __shared__ double sh[64];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;
   const long long npen = (long long)g.nx * g.ny;
-  const long long p0 = (long long)blockIdx.x * P;
-  z_load_half(sm, F + inst * Fs, p0, P, npen, n, g.nzh);
+  const long long p0 = (long long)blockIdx.x * t.P;
+  z_load_half(sm, F + inst * Fs, p0, t, npen, n, g.nzh, 0);
   __syncthreads();
-  lds_fft<1, false, VPT>(sm, P >> 1, n, 1, g.fz);
+  z_fft<1>(sm, t, g.fz, s_tw);
   const double* rho = el.rho + inst * el.fstride;
   const double* V = el.vext ? el.vext + (el.vext_batched ? inst * el.fstride : 0) : nullptr;
   const InstDev ip = el.inst[inst];
@@ -555,12 +641,13 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
   for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
   double* rnew_hist = nullptr;
   if (MODE == MODE_ANDRES) rnew_hist = el.rhist + (long long)el.slot_new * el.hstride + inst * el.fstride;
-  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
-    const int pl = i / n, e = i - pl * n;
+  const int tot = t.P * n;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
     const long long p = p0 + pl;
     if (p >= npen) continue;
     const long long idx = p * n + e;
-    double& cv = z_val(sm, pl, e, n);
+    double& cv = z_val(sm, t, 0, pl, e);
     const double vv = V ? V[idx] : 0.0;
     const double C1 = cv + vv;
     const double r = rho[idx];
@@ -600,8 +687,8 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
   }
   if (MODE == MODE_PICARD) {
     __syncthreads();
-    lds_fft<-1, false, VPT>(sm, P >> 1, n, 1, g.fz);
-    z_store_half(sm, el.rz + inst * el.rzstride, p0, P, npen, n, g.nzh);
+    z_fft<-1>(sm, t, g.fz, s_tw);
+    z_store_half(sm, el.rz + inst * el.rzstride, p0, t, npen, n, g.nzh, 0);
   }
   int ops[NQ];
 #pragma unroll
@@ -619,22 +706,25 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
 // Reduces N', sum|rho'-rho_cur|, max|rho'-rho_cur|, non-finite count, and the
 // ideal-gas / external sums of rho'.
 template <int MODE>
-__global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, int P, const int* __restrict__ active,
+__global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, const int* __restrict__ active,
                                                 RedDev red) {
   extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[CMM_MAX_N];
+  load_twiddles(s_tw, g.fz);
   __shared__ double sh[64];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;
   const long long npen = (long long)g.nx * g.ny;
-  const long long p0 = (long long)blockIdx.x * P;
+  const long long p0 = (long long)blockIdx.x * t.P;
   const double* V = el.vext ? el.vext + (el.vext_batched ? inst * el.fstride : 0) : nullptr;
   const double* rcur = el.rho + inst * el.fstride;
   double* rout = el.rho_out + inst * el.fstride;
   const double* coef = el.coef + inst * 2 * CMM_MMAX;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int i = threadIdx.x; i < P * n; i += blockDim.x) {
-    const int pl = i / n, e = i - pl * n;
+  const int tot = t.P * n;
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
     const long long p = p0 + pl;
     double rn = 0.0;
     if (p < npen) {
@@ -666,11 +756,11 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, int P, con
       acc[4] += rc * (log(el.lam3 * rc) - 1.0);
       acc[5] += rn * vv;
     }
-    z_val(sm, pl, e, n) = rn;
+    z_val(sm, t, 0, pl, e) = rn;
   }
   __syncthreads();
-  lds_fft<-1, false, VPT>(sm, P >> 1, n, 1, g.fz);
-  z_store_half(sm, el.rz + inst * el.rzstride, p0, P, npen, n, g.nzh);
+  z_fft<-1>(sm, t, g.fz, s_tw);
+  z_store_half(sm, el.rz + inst * el.rzstride, p0, t, npen, n, g.nzh, 0);
   const int ops[7] = {0, 0, 1, 0, 0, 0, 0};
   block_reduce<7>(acc, ops, sh);
   grid_reduce_finish<7>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,

@@ -25,13 +25,13 @@ def setup():
     return host, prog, _mus(6)
 
 
-@pytest.mark.parametrize('use_graph', [True, False])
-def test_batch_equals_plain_per_point(setup, use_graph):
+@pytest.mark.parametrize('use_graph,chunk', [(True, None), (False, None), (True, 4), (True, 1)])
+def test_batch_equals_plain_per_point(setup, use_graph, chunk):
     import torch
     from cmmdft_amd.batch import BatchedAnderson
     from cmmdft_amd.solver import Anderson
     host, prog, mus = setup
-    b = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400, use_graph=use_graph)
+    b = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400, use_graph=use_graph, chunk=chunk)
     res = b.solve()
     assert res['converged'].all(), res['status']
     nplain = 0
