@@ -247,8 +247,13 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
+// LDS scratch of a block reduction: 16 wave slots (up to 1024 threads) for
+// each of at most 16 quantities.
+#define CMM_RED_SH 256
+
 template <int NQ>
-__device__ void block_reduce(double (&v)[NQ], const int (&ops)[NQ], double* sh /* >= 16*NQ */) {
+__device__ void block_reduce(double (&v)[NQ], const int (&ops)[NQ], double* sh /* CMM_RED_SH doubles */) {
+  static_assert(NQ <= CMM_RED_SH / 16, "too many quantities for one block reduction");
   const int lane = threadIdx.x & 63;
   const int wid = threadIdx.x >> 6;
   const int nw = (blockDim.x + 63) >> 6;

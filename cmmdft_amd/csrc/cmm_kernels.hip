@@ -552,7 +552,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fz);
-  __shared__ double sh[64];
+  __shared__ double sh[CMM_RED_SH];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;
@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fz);
-  __shared__ double sh[64];
+  __shared__ double sh[CMM_RED_SH];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;
@@ -711,7 +711,7 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fz);
-  __shared__ double sh[64];
+  __shared__ double sh[CMM_RED_SH];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;

@@ -70,3 +70,25 @@ def test_batch_matches_oracle(setup):
     assert rel <= 1e-8 and mism == 0
     assert abs(res['N'][i] - Nref) <= 1e-8 * Nref
     b.close()
+
+
+def test_batch_equals_plain_larger_grid():
+    """Multi-instance groups at 64^3 (hundreds of blocks per reduction): every
+    instance must follow the plain solver's trajectory exactly."""
+    import torch
+    from cmmdft_amd.batch import BatchedAnderson
+    from cmmdft_amd.solver import Anderson
+    host = W.host_case2()
+    shape = (64, 64, 64)
+    prog = product_program(host, shape, 'WBII')
+    mus = _mus(32)[[4, 12, 20, 28]]
+    b = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-9, nsteps=300, chunk=4)
+    res = b.solve()
+    for i, mu in enumerate(mus):
+        if i in res['resolved']:
+            continue
+        s = Anderson(prog, nsteps=300, method='anderson', m=5, damping=0.1, criterion='RIUE', threshold=1e-9)
+        N, rho, conv = s.solve(mu, prog._set_initial_density(chempot=mu, temperature=T))
+        assert conv and s.curr_step == res['iters'][i], (i, s.curr_step, res['iters'][i])
+        assert torch.equal(rho.reshape(-1), res['rho'][i].reshape(-1))
+    b.close()
