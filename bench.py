@@ -120,6 +120,39 @@ def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample):
             'seconds_per_iteration': per_iter}
 
 
+def passes_only(shape, reps=5):
+    """Functional chain of one realistic density (ideal-gas start at 10 bar),
+    each pass launched ``reps`` times, then a plain 256 MB device-to-device
+    copy (the reference for scaling FETCH_SIZE / WRITE_SIZE counters)."""
+    import torch
+    from cmmdft_amd import workloads as W
+    from cmmdft_amd.eos import ModifiedBenedictWebbRubinEOS
+    host, prog = build_program(shape)
+    prog.fener.add_external_potential(cutoff=W.CUTOFF)
+    from cmmdft_amd import units as U
+    mu = ModifiedBenedictWebbRubinEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON).calculate_mu(W.TEMPERATURE,
+                                                                                                  10 * U.bar)
+    from cmmdft_amd.solver import _Engine
+    eng = _Engine(prog.grid, prog.fener)
+    rho = prog._set_initial_density(chempot=mu, temperature=W.TEMPERATURE).reshape(-1).contiguous()
+    eng.plan.seed(rho, eng.V, False, eng.lam3)
+    eng.plan.derive()
+    torch.cuda.synchronize()
+    for which in PASS_NAMES:
+        for _ in range(reps):
+            eng.plan.launch_pass(which)
+    torch.cuda.synchronize()
+    a = torch.empty(256 * 2 ** 20 // 8, dtype=torch.float64, device='cuda').fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t = time_passes(eng.plan)
+    print(json.dumps({'passes_only': True, 'grid': list(shape), 'pass_ms': {PASS_NAMES[k]: v for k, v in t.items()},
+                      'pass_bytes': {PASS_NAMES[k]: v for k, v in pass_bytes(*shape).items()},
+                      'copy_bytes': a.numel() * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -129,6 +162,9 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--grid', type=int, default=NSIDE)
     ap.add_argument('--cpu-sample', type=int, default=4, help='oracle iterations timed (0 skips the CPU leg)')
+    ap.add_argument('--passes-only', action='store_true',
+                    help='short run for hardware counters: the functional passes at the bench grid, then a 256 MB '
+                         'device copy as the byte-count reference')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported'
@@ -138,6 +174,8 @@ def main():
     from cmmdft_amd import units as U, workloads as W
     from cmmdft_amd.solver import Anderson
     shape = (args.grid,) * 3
+    if args.passes_only:
+        return passes_only(shape)
     mus = chempots(32)
     host, prog = build_program(shape)
     torch.cuda.synchronize()

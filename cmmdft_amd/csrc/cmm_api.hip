@@ -329,8 +329,7 @@ int cmm_set_instances(cmm_plan* p, const double* fug, const double* alpha) {
 // launch helpers
 // --------------------------------------------------------------------------
 static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
-  const int ntk = (p->nzh + p->tby - 1) / p->tby;
-  dim3 grid(p->nx * ntk, nf, p->nb);
+  dim3 grid(p->nx, nf, p->nb);
   size_t lds = (size_t)p->ny * p->tby * sizeof(double2);
   long long is = (nf == 1 && F == p->rz) ? p->nhalf : 6 * p->nhalf;
   const FastDiv fT = make_fastdiv((unsigned)p->tby);
@@ -364,8 +363,7 @@ int cmm_fft_r2c(cmm_plan* p, const double* x, double* X) {
   LAUNCHCHK("k_zr2c");
   // y and x passes on the caller's buffer (instance stride nhalf)
   {
-    const int ntk = (p->nzh + p->tby - 1) / p->tby;
-    dim3 gy(p->nx * ntk, 1, p->nb);
+    dim3 gy(p->nx, 1, p->nb);
     hipLaunchKernelGGL(k_ypass<-1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, Xc, p->nhalf,
                        p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
     LAUNCHCHK("k_ypass");
@@ -380,8 +378,7 @@ int cmm_fft_c2r(cmm_plan* p, const double* X, double* x) {
   int rc = launch_xpass(p, p->rz, p->nhalf, 1);
   if (rc) return rc;
   {
-    const int ntk = (p->nzh + p->tby - 1) / p->tby;
-    dim3 gy(p->nx * ntk, 1, p->nb);
+    dim3 gy(p->nx, 1, p->nb);
     hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->rz, p->nhalf,
                        p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
     LAUNCHCHK("k_ypass");
@@ -458,8 +455,7 @@ int cmm_derive(cmm_plan* p) {
                      make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
   LAUNCHCHK("k_xback");
   // inverse y of the sum (field 0 of F)
-  const int ntk = (p->nzh + p->tby - 1) / p->tby;
-  dim3 gy(p->nx * ntk, 1, p->nb);
+  dim3 gy(p->nx, 1, p->nb);
   hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->F,
                      6 * p->nhalf, p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
   LAUNCHCHK("k_ypass");

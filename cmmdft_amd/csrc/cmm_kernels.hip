@@ -170,7 +170,10 @@ __global__ void __launch_bounds__(NT) k_zc2r(const double2* __restrict__ X, long
   }
 }
 
-// y pass: tile = one x plane, TB consecutive kz columns, all y.
+// y pass: a block owns one x plane of one field and walks its kz columns in
+// tiles of TB (all y each).  Keeping a plane's tiles on one block makes the
+// cache lines two neighbouring tiles share (rows of nz/2+1 complex are not
+// line aligned) hit in that CU's caches instead of being fetched twice.
 template <int SGN>
 __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
                                              FastDiv fTB, const int* __restrict__ active) {
@@ -179,23 +182,25 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   load_twiddles(s_tw, g.fy);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
-  const int ntk = (g.nzh + TB - 1) / TB;
-  const int x = blockIdx.x / ntk, tk = blockIdx.x - x * ntk;
-  const int k0 = tk * TB;
-  const int tbv = min(TB, g.nzh - k0);
-  double2* base = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh + k0;
+  const int x = blockIdx.x;
+  double2* plane = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh;
   const int tot = g.ny * TB;
+  for (int k0 = 0; k0 < g.nzh; k0 += TB) {
+    const int tbv = min(TB, g.nzh - k0);
+    double2* base = plane + k0;
 #pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
-    sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fy, s_tw);
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+      const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
+      sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fy, s_tw);
 #pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
-    if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+      const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
+      if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
+    }
+    __syncthreads();
   }
 }
 
@@ -409,36 +414,42 @@ struct PhiOut {
 
 __device__ __forceinline__ void phi23(int base, double n3, double ln3, double& p2, double& p3, double& d2,
                                       double& d3) {
+  // Same forms as the reference (functionals.py:726-767); divisions by the
+  // common factors q = 1-2n3+n3^2, c = 1-3n3+3n3^2-n3^3 and n3^2 are taken
+  // as multiplications by their reciprocals.
   const double n3_2 = n3 * n3, n3_3 = n3_2 * n3;
   const double q = 1.0 - 2.0 * n3 + n3_2;
   const double c = 1.0 - 3.0 * n3 + 3.0 * n3_2 - n3_3;
-  const double pi = M_PI;
+  const double iq = 1.0 / q, ic = 1.0 / c;
+  const double i24 = 1.0 / (24.0 * M_PI), i36 = 1.0 / (36.0 * M_PI);
   if (base == 0) {                       // Rosenfeld
     p2 = 1.0 / (1.0 - n3);
-    d2 = 1.0 / q;
-    p3 = 1.0 / (24.0 * pi * q);
-    d3 = 1.0 / (12.0 * pi * c);
+    d2 = iq;
+    p3 = i24 * iq;
+    d3 = (1.0 / (12.0 * M_PI)) * ic;
   } else if (base == 1) {                // modified FMT
     p2 = 1.0 / (1.0 - n3);
-    d2 = 1.0 / q;
+    d2 = iq;
     if (n3 <= 1e-8) {
-      p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) / (24.0 * pi * q);
-      d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) / (36.0 * pi * c);
+      p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) * i24 * iq;
+      d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) * i36 * ic;
     } else {
-      p3 = (n3 + q * ln3) / (36.0 * pi * n3_2 * q);
-      d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) / (36.0 * pi * n3_3 * c);
+      const double in2 = 1.0 / n3_2;
+      p3 = (n3 + q * ln3) * i36 * in2 * iq;
+      d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) * i36 * (in2 / n3) * ic;
     }
   } else {                               // White-Bear II
     if (n3 <= 1e-8) {
       p2 = (1.0 + n3_2 / 9.0) / (1.0 - n3);
-      d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) / q;
-      p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) / (24.0 * pi * q);
-      d3 = (7.0 / 3.0 - n3 / 2.0 + n3_2 / 10.0) / (36.0 * pi * c);
+      d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) * iq;
+      p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) * i24 * iq;
+      d3 = (7.0 / 3.0 - n3 / 2.0 + n3_2 / 10.0) * i36 * ic;
     } else {
+      const double in2 = 1.0 / n3_2;
       p2 = (5.0 * n3 - n3_2 + 2.0 * (1.0 - n3) * ln3) / (3.0 * (n3 - n3_2));
-      d2 = -2.0 * (n3 - 3.0 * n3_2 + q * ln3) / (3.0 * n3_2 * q);
-      p3 = -2.0 * (n3 - 3.0 * n3_2 + n3_3 + ln3 * q) / ((3.0 * n3_2) * 24.0 * pi * q);
-      d3 = (2.0 * n3 - 5.0 * n3_2 + 6.0 * n3_3 - n3_2 * n3_2 + 2.0 * c * ln3) / (36.0 * pi * n3_3 * c);
+      d2 = -2.0 * (n3 - 3.0 * n3_2 + q * ln3) * (1.0 / 3.0) * in2 * iq;
+      p3 = -2.0 * (n3 - 3.0 * n3_2 + n3_3 + ln3 * q) * (1.0 / 3.0) * in2 * i24 * iq;
+      d3 = (2.0 * n3 - 5.0 * n3_2 + 6.0 * n3_3 - n3_2 * n3_2 + 2.0 * c * ln3) * i36 * (in2 / n3) * ic;
     }
   }
 }
@@ -564,8 +575,11 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   z_fft<1>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
   const int tot = t.P * n;
+  // pencil index fastest: consecutive lanes walk along an LDS row (no bank
+  // conflicts); this phase touches no global memory
+  const FastDiv fP = make_fastdiv((unsigned)t.P);
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
+    const int e = (int)fdiv((unsigned)i, fP), pl = i - e * t.P;
     if (p0 + pl >= npen) continue;
     double& r0 = z_val(sm, t, 0, pl, e);
     double& r1 = z_val(sm, t, 1, pl, e);
