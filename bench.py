@@ -65,6 +65,20 @@ def pass_bytes(nx, ny, nz):
     return {0: 12 * nh, 1: 7 * nh, 2: 7 * nh, 3: 12 * nh}
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of ``kernel`` at n^3 from the committed
+    FETCH_SIZE/WRITE_SIZE summary (profiles/functional_passes_128cube.md),
+    or None when no measurement for that grid is recorded."""
+    fn = os.path.join(ROOT, 'profiles', 'functional_passes_128cube.md')
+    if not os.path.exists(fn):
+        return None
+    for line in open(fn):
+        w = line.split()
+        if len(w) == 4 and w[0] == 'traffic_bytes_per_launch' and w[1] == kernel and int(w[2]) == n:
+            return int(w[3])
+    return None
+
+
 def time_passes(plan, reps=20):
     """Average duration of each pass, HIP events on the plan's stream."""
     import torch
@@ -268,7 +282,10 @@ def main():
                    'grid': list(shape), 'solver': 'anderson', 'points_per_step': npt,
                    'mode': 'batched' if batched_pps is not None else 'plain'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': PEAK_HBM_GBPS, 'unit': 'GB/s',
-                     'frac': achieved / PEAK_HBM_GBPS, 'traffic': None, 'kernel': PASS_NAMES[dom],
+                     'frac': achieved / PEAK_HBM_GBPS,
+                     'traffic': (measured_traffic(PASS_NAMES[dom], shape[0]) if shape[0] == shape[1] == shape[2]
+                                 else None),
+                     'kernel': PASS_NAMES[dom],
                      'avg_launch_ms': tp[dom], 'algorithmic_bytes_per_launch': pb[dom]},
         'cpu_baseline': cpu,
         'plain_points_per_s': plain_pps,
