@@ -24,6 +24,7 @@ S_NCUR, S_FID, S_EXT = 3, 4, 5
 S_PIC_NNEW, S_PIC_IUE, S_PIC_DMAX, S_PIC_NONFIN = 6, 7, 8, 9
 S_AND_NONFIN, S_GRAM = 6, 7
 S_PN, S_PIUE, S_PDMAX, S_PNONFIN, S_PFID, S_PEXT = 16, 17, 18, 19, 20, 21
+S_ADJ_PQ, S_ADJ_RR, S_ADJ_LAM = 24, 25, 27
 
 
 class CmmFunctional(C.Structure):
@@ -77,6 +78,16 @@ EXPORTS = {
     'cmm_synchronize': ([C.c_void_p], C.c_int),
     'cmm_lj_grid': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                      C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_hessian_apply': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_adjoint_seed': ([C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                          C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_adjoint_final': ([C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_adjoint_update': ([C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_adjoint_lambda': ([C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_lj_grad_workspace': ([C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
+    'cmm_lj_grid_grad': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                          C.c_void_p, C.c_void_p], C.c_int),
     'cmm_lj_points': ([C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                        C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
 }
@@ -241,6 +252,23 @@ class Plan:
         check(self.lib.cmm_trial(self.h, _ptr(rho), _ptr(rhist), int(slot), _ptr(rho_out), _ptr(vext),
                                  int(vext_batched), float(lam3), int(clip)))
 
+    # ---- linear response ---------------------------------------------------
+    def hessian_apply(self, nfld):
+        check(self.lib.cmm_hessian_apply(self.h, _ptr(nfld)))
+
+    def adjoint_seed(self, rho, beta, dr, init, parity, y, r, p):
+        check(self.lib.cmm_adjoint_seed(self.h, _ptr(rho), float(beta), float(dr), int(init), int(parity), _ptr(y),
+                                        _ptr(r), _ptr(p)))
+
+    def adjoint_final(self, rho, beta, p, q):
+        check(self.lib.cmm_adjoint_final(self.h, _ptr(rho), float(beta), _ptr(p), _ptr(q)))
+
+    def adjoint_update(self, parity, y, r, p, q):
+        check(self.lib.cmm_adjoint_update(self.h, int(parity), _ptr(y), _ptr(r), _ptr(p), _ptr(q)))
+
+    def adjoint_lambda(self, rho, beta, y, lam):
+        check(self.lib.cmm_adjoint_lambda(self.h, _ptr(rho), float(beta), _ptr(y), _ptr(lam)))
+
     # ---- batched solves and graphs -----------------------------------------
     def batch_reset(self, mu, beta, lam3, dr, tol, nsteps):
         mu = np.ascontiguousarray(np.asarray(mu, dtype=np.float64))
@@ -303,3 +331,21 @@ def lj_points(points_dev, M, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff
     check(lib.cmm_lj_points(_ptr(points_dev), int(M), rv.ctypes.data_as(C.c_void_p), _ptr(atoms_dev), int(natom),
                             float(sigma_f), float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(out),
                             C.c_void_p(s.cuda_stream)))
+
+
+def lj_grid_grad(npoints, rvecs, atoms_dev, types_dev, natom, ntypes, sigma_f, epsilon_f, cutoff, images, limit, w,
+                 stream=None):
+    """Device tensor (ntypes, 2): sum_i w_i dV_i/dsigma_t, sum_i w_i dV_i/depsilon_t."""
+    torch = _torch()
+    lib = load()
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    nw = C.c_longlong()
+    check(lib.cmm_lj_grad_workspace(int(npoints[0]), int(npoints[1]), int(npoints[2]), int(ntypes), C.byref(nw)))
+    work = torch.empty(int(nw.value), dtype=torch.float64, device=w.device)
+    grad = torch.empty((int(ntypes), 2), dtype=torch.float64, device=w.device)
+    check(lib.cmm_lj_grid_grad(int(npoints[0]), int(npoints[1]), int(npoints[2]), rv.ctypes.data_as(C.c_void_p),
+                               _ptr(atoms_dev), _ptr(types_dev), int(natom), int(ntypes), float(sigma_f),
+                               float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(w), _ptr(work),
+                               _ptr(grad), C.c_void_p(s.cuda_stream)))
+    return grad

@@ -3,6 +3,7 @@
 // included here so the launch templates are instantiated next to their use.
 #include "cmm_kernels.hip"
 #include "cmm_lj.hip"
+#include "cmm_adjoint.hip"
 #include "../../include/cmmdft_hip.h"
 
 #include <algorithm>
@@ -440,13 +441,10 @@ static int launch_zphi(cmm_plan* p, int store) {
   return CMM_OK;
 }
 
-int cmm_derive(cmm_plan* p) {
-  if (!p) return fail(CMM_E_ARG, "null plan");
-  int rc = forward_chain(p);
-  if (rc) return rc;
-  rc = launch_zphi(p, 1);
-  if (rc) return rc;
-  rc = launch_ypass(p, p->F, 6, -1);
+// back-convolution of the 6 derivative fields in F: inverse y, the x pass
+// with the weighted sum, inverse y of the sum (left in field 0 of F)
+static int back_chain(cmm_plan* p) {
+  int rc = launch_ypass(p, p->F, 6, -1);
   if (rc) return rc;
   const long long nb = (long long)p->ny * p->nzh;
   dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
@@ -454,12 +452,31 @@ int cmm_derive(cmm_plan* p) {
   hipLaunchKernelGGL(k_xback, grid, dim3(NT), lds, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
                      make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
   LAUNCHCHK("k_xback");
-  // inverse y of the sum (field 0 of F)
   dim3 gy(p->nx, 1, p->nb);
   hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->F,
                      6 * p->nhalf, p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
   LAUNCHCHK("k_ypass");
   return CMM_OK;
+}
+
+int cmm_derive(cmm_plan* p) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  int rc = forward_chain(p);
+  if (rc) return rc;
+  rc = launch_zphi(p, 1);
+  if (rc) return rc;
+  return back_chain(p);
+}
+
+int cmm_hessian_apply(cmm_plan* p, const double* nfld) {
+  if (!p || !nfld) return fail(CMM_E_ARG, "null argument");
+  int rc = forward_chain(p);
+  if (rc) return rc;
+  dim3 grid((unsigned)zblocks(p, p->pphi), p->nb);
+  hipLaunchKernelGGL(k_zhess, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, nfld,
+                     p->npts, p->g, p->fp, ztile(p, p->pphi, 6), p->active);
+  LAUNCHCHK("k_zhess");
+  return back_chain(p);
 }
 
 int cmm_value(cmm_plan* p) {
@@ -637,6 +654,77 @@ int cmm_synchronize(cmm_plan* p) {
   return CMM_OK;
 }
 
+static AdjDev adj_base(cmm_plan* p, const double* rho, double beta) {
+  AdjDev a;
+  std::memset(&a, 0, sizeof(a));
+  a.rho = rho;
+  a.fstride = p->npts;
+  a.beta = beta;
+  return a;
+}
+
+int cmm_adjoint_seed(cmm_plan* p, const double* rho, double beta, double dr, int init, int parity, double* y,
+                     double* r, double* pvec) {
+  if (!p || !rho || !r || !pvec || (init && !y)) return fail(CMM_E_ARG, "null argument");
+  if (!(beta > 0.0)) return fail(CMM_E_ARG, "beta must be positive");
+  AdjDev a = adj_base(p, rho, beta);
+  a.dr = dr;
+  a.y = y;
+  a.r = r;
+  a.p = pvec;
+  a.parity = parity & 1;
+  dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
+  if (init)
+    hipLaunchKernelGGL(k_adj_seed<ADJ_INIT>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, a,
+                       ztile(p, p->pz, 1), p->rz, p->nhalf, p->active, red_of(p));
+  else
+    hipLaunchKernelGGL(k_adj_seed<ADJ_STEP>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, a,
+                       ztile(p, p->pz, 1), p->rz, p->nhalf, p->active, red_of(p));
+  LAUNCHCHK("k_adj_seed");
+  return CMM_OK;
+}
+
+int cmm_adjoint_final(cmm_plan* p, const double* rho, double beta, const double* pvec, double* q) {
+  if (!p || !rho || !pvec || !q) return fail(CMM_E_ARG, "null argument");
+  AdjDev a = adj_base(p, rho, beta);
+  a.p = const_cast<double*>(pvec);
+  a.q = q;
+  dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
+  hipLaunchKernelGGL(k_adj_final, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, a,
+                     ztile(p, p->pz, 1), p->active, red_of(p));
+  LAUNCHCHK("k_adj_final");
+  return CMM_OK;
+}
+
+static dim3 flat_grid(cmm_plan* p) {
+  long long nbk = (p->npts + NT * 8 - 1) / (NT * 8);
+  nbk = std::min(nbk, p->pstride);
+  return dim3((unsigned)std::max(1LL, nbk), p->nb);
+}
+
+int cmm_adjoint_update(cmm_plan* p, int parity, double* y, double* r, const double* pvec, const double* q) {
+  if (!p || !y || !r || !pvec || !q) return fail(CMM_E_ARG, "null argument");
+  AdjDev a = adj_base(p, nullptr, 1.0);
+  a.y = y;
+  a.r = r;
+  a.p = const_cast<double*>(pvec);
+  a.q = const_cast<double*>(q);
+  a.parity = parity & 1;
+  hipLaunchKernelGGL(k_adj_update, flat_grid(p), dim3(NT), 0, p->stream, a, p->active, red_of(p));
+  LAUNCHCHK("k_adj_update");
+  return CMM_OK;
+}
+
+int cmm_adjoint_lambda(cmm_plan* p, const double* rho, double beta, const double* y, double* lam) {
+  if (!p || !rho || !y || !lam) return fail(CMM_E_ARG, "null argument");
+  AdjDev a = adj_base(p, rho, beta);
+  a.y = const_cast<double*>(y);
+  a.lam = lam;
+  hipLaunchKernelGGL(k_adj_lambda, flat_grid(p), dim3(NT), 0, p->stream, a, p->active, red_of(p));
+  LAUNCHCHK("k_adj_lambda");
+  return CMM_OK;
+}
+
 static int lj_setup(int nx, int ny, int nz, const double* rvecs, double sigma_f, double epsilon_f, double cutoff,
                     int images, double limit, LJDev& d) {
   if (!rvecs) return fail(CMM_E_ARG, "null argument");
@@ -686,6 +774,34 @@ int cmm_lj_points(const double* points, long long M, const double* rvecs, const 
   dim3 grid((unsigned)((M + LJ_NT - 1) / LJ_NT));
   hipLaunchKernelGGL(k_lj_points, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, natom, d, points, M, V);
   LAUNCHCHK("k_lj_points");
+  return CMM_OK;
+}
+
+static dim3 lj_grid_dims(int nx, int ny, int nz) {
+  return dim3((nz + LJ_BZ - 1) / LJ_BZ, (ny + LJ_BY - 1) / LJ_BY, (nx + LJ_BX - 1) / LJ_BX);
+}
+
+int cmm_lj_grad_workspace(int nx, int ny, int nz, int ntypes, long long* ndoubles) {
+  if (!ndoubles || nx < 1 || ny < 1 || nz < 1 || ntypes < 1) return fail(CMM_E_ARG, "bad arguments");
+  const dim3 g = lj_grid_dims(nx, ny, nz);
+  *ndoubles = 2LL * ntypes * g.x * g.y * g.z;
+  return CMM_OK;
+}
+
+int cmm_lj_grid_grad(int nx, int ny, int nz, const double* rvecs, const double* atoms, const int* types, int natom,
+                     int ntypes, double sigma_f, double epsilon_f, double cutoff, int images, double limit,
+                     const double* w, double* work, double* grad, void* stream) {
+  if (!w || !work || !grad || (natom > 0 && (!atoms || !types)) || natom < 0 || ntypes < 1)
+    return fail(CMM_E_ARG, "null argument");
+  LJDev d;
+  int rc = lj_setup(nx, ny, nz, rvecs, sigma_f, epsilon_f, cutoff, images, limit, d);
+  if (rc) return rc;
+  const dim3 grid = lj_grid_dims(nx, ny, nz);
+  const long long nblocks = (long long)grid.x * grid.y * grid.z;
+  hipLaunchKernelGGL(k_lj_bwd, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, types, natom, ntypes, d, w, work);
+  LAUNCHCHK("k_lj_bwd");
+  hipLaunchKernelGGL(k_lj_bwd_reduce, dim3(2 * ntypes), dim3(LJ_NT), 0, (hipStream_t)stream, work, nblocks, grad);
+  LAUNCHCHK("k_lj_bwd_reduce");
   return CMM_OK;
 }
 

@@ -408,19 +408,70 @@ __global__ void __launch_bounds__(NT) k_xback(double2* __restrict__ F, long long
 // (Rosenfeld 'FMT', 'MFMT', White-Bear II 'WBII', optional 'a' variant), and
 // the WDA excess free energy per volume of a sum of EOS terms.
 // ---------------------------------------------------------------------------
+// Forward-mode dual number (value, directional derivative).  The pointwise
+// functions below are templates over the scalar type: with double they are the
+// functional itself, with Dual they also give the derivative of every output
+// along a direction of the weighted densities, i.e. a row of the Jacobian of
+// the derivative fields (the Hessian of Phi) applied to that direction.
+// Branches and clips select on the value, so the derivative is that of the
+// branch the value takes, and a clipped input has zero derivative.
+using ::exp;
+using ::log;
+struct Dual {
+  double v, d;
+  __device__ __forceinline__ Dual() : v(0.0), d(0.0) {}
+  __device__ __forceinline__ Dual(double x) : v(x), d(0.0) {}
+  __device__ __forceinline__ Dual(double x, double dx) : v(x), d(dx) {}
+};
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return Dual(a.v + b, a.d); }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return Dual(a.v - b, a.d); }
+__device__ __forceinline__ Dual operator-(double a, Dual b) { return Dual(a - b.v, -b.d); }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual(q, (a.d - q * b.d) / b.v);
+}
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return Dual(a.v / b, a.d / b); }
+__device__ __forceinline__ Dual operator/(double a, Dual b) {
+  const double q = a / b.v;
+  return Dual(q, -q * b.d / b.v);
+}
+__device__ __forceinline__ Dual& operator+=(Dual& a, Dual b) { a = a + b; return a; }
+__device__ __forceinline__ Dual& operator*=(Dual& a, Dual b) { a = a * b; return a; }
+__device__ __forceinline__ Dual log(Dual a) { return Dual(::log(a.v), a.d / a.v); }
+__device__ __forceinline__ Dual exp(Dual a) {
+  const double e = ::exp(a.v);
+  return Dual(e, e * a.d);
+}
+__device__ __forceinline__ double val(double x) { return x; }
+__device__ __forceinline__ double val(Dual x) { return x.v; }
+// min(max(x, lo), hi) with fmin/fmax semantics (NaN -> lo)
+__device__ __forceinline__ double clip(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+__device__ __forceinline__ Dual clip(Dual x, double lo, double hi) {
+  if (x.v >= lo && x.v <= hi) return x;
+  return Dual(x.v > hi ? hi : lo, 0.0);
+}
+
+template <typename T>
 struct PhiOut {
-  double phi, g0, g1, gv[3];
+  T phi, g0, g1, gv[3];
 };
 
-__device__ __forceinline__ void phi23(int base, double n3, double ln3, double& p2, double& p3, double& d2,
-                                      double& d3) {
+template <typename T>
+__device__ __forceinline__ void phi23(int base, T n3, T ln3, T& p2, T& p3, T& d2, T& d3) {
   // Same forms as the reference (functionals.py:726-767); divisions by the
   // common factors q = 1-2n3+n3^2, c = 1-3n3+3n3^2-n3^3 and n3^2 are taken
   // as multiplications by their reciprocals.
-  const double n3_2 = n3 * n3, n3_3 = n3_2 * n3;
-  const double q = 1.0 - 2.0 * n3 + n3_2;
-  const double c = 1.0 - 3.0 * n3 + 3.0 * n3_2 - n3_3;
-  const double iq = 1.0 / q, ic = 1.0 / c;
+  const T n3_2 = n3 * n3, n3_3 = n3_2 * n3;
+  const T q = 1.0 - 2.0 * n3 + n3_2;
+  const T c = 1.0 - 3.0 * n3 + 3.0 * n3_2 - n3_3;
+  const T iq = 1.0 / q, ic = 1.0 / c;
   const double i24 = 1.0 / (24.0 * M_PI), i36 = 1.0 / (36.0 * M_PI);
   if (base == 0) {                       // Rosenfeld
     p2 = 1.0 / (1.0 - n3);
@@ -430,22 +481,22 @@ __device__ __forceinline__ void phi23(int base, double n3, double ln3, double& p
   } else if (base == 1) {                // modified FMT
     p2 = 1.0 / (1.0 - n3);
     d2 = iq;
-    if (n3 <= 1e-8) {
+    if (val(n3) <= 1e-8) {
       p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) * i24 * iq;
       d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) * i36 * ic;
     } else {
-      const double in2 = 1.0 / n3_2;
+      const T in2 = 1.0 / n3_2;
       p3 = (n3 + q * ln3) * i36 * in2 * iq;
       d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) * i36 * (in2 / n3) * ic;
     }
   } else {                               // White-Bear II
-    if (n3 <= 1e-8) {
+    if (val(n3) <= 1e-8) {
       p2 = (1.0 + n3_2 / 9.0) / (1.0 - n3);
       d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) * iq;
       p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) * i24 * iq;
       d3 = (7.0 / 3.0 - n3 / 2.0 + n3_2 / 10.0) * i36 * ic;
     } else {
-      const double in2 = 1.0 / n3_2;
+      const T in2 = 1.0 / n3_2;
       p2 = (5.0 * n3 - n3_2 + 2.0 * (1.0 - n3) * ln3) / (3.0 * (n3 - n3_2));
       d2 = -2.0 * (n3 - 3.0 * n3_2 + q * ln3) * (1.0 / 3.0) * in2 * iq;
       p3 = -2.0 * (n3 - 3.0 * n3_2 + n3_3 + ln3 * q) * (1.0 / 3.0) * in2 * i24 * iq;
@@ -459,23 +510,23 @@ __device__ __forceinline__ void phi23(int base, double n3, double ln3, double& p
 //   g0 = kT (dPhi/dn0 / 4piR^2 + dPhi/dn1 / 4piR + dPhi/dn2)    with w2
 //   g1 = kT dPhi/dn3                                           with w3
 //   gv = kT (dPhi/dnv1 / 4piR + dPhi/dnv2)                      with -wv2
-__device__ __forceinline__ PhiOut fmt_point(const FuncDev& fp, double n2, double n3raw, double vx, double vy, double vz) {
-  PhiOut o;
+template <typename T>
+__device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw, T vx, T vy, T vz) {
+  PhiOut<T> o;
   const double R = fp.R;
   const double i4pR2 = fp.inv4piR2, i4pR = fp.inv4piR;
-  const double n0 = n2 * i4pR2, n1 = n2 * i4pR;
-  const double ux = vx * i4pR, uy = vy * i4pR, uz = vz * i4pR;   // n_v1
-  const double n3 = fmin(fmax(n3raw, 1e-30), 0.99);
-  const double ln3 = log(1.0 - n3);
-  double p2, p3, d2, d3;
-  phi23(fp.base, n3, ln3, p2, p3, d2, d3);
-  const double v2 = vx * vx + vy * vy + vz * vz;
-  const double dot12 = ux * vx + uy * vy + uz * vz;
-  double q3, dq3_n2, fv;
+  const T n0 = n2 * i4pR2, n1 = n2 * i4pR;
+  const T ux = vx * i4pR, uy = vy * i4pR, uz = vz * i4pR;   // n_v1
+  const T n3 = clip(n3raw, 1e-30, 0.99);
+  const T ln3 = log(1.0 - n3);
+  T p2, p3, d2, d3;
+  phi23<T>(fp.base, n3, ln3, p2, p3, d2, d3);
+  const T v2 = vx * vx + vy * vy + vz * vz;
+  const T dot12 = ux * vx + uy * vy + uz * vz;
+  T q3, dq3_n2, fv;
   if (fp.anti) {
-    double xi = v2 / (n2 * n2 + 1e-16);
-    xi = fmin(fmax(xi, 0.0), 1.0);
-    const double om = 1.0 - xi;
+    const T xi = clip(v2 / (n2 * n2 + 1e-16), 0.0, 1.0);
+    const T om = 1.0 - xi;
     q3 = n2 * n2 * n2 * om * om * om;
     dq3_n2 = 3.0 * n2 * n2 * (1.0 + xi) * om * om;
     fv = -6.0 * n2 * om * om * p3;
@@ -485,10 +536,10 @@ __device__ __forceinline__ PhiOut fmt_point(const FuncDev& fp, double n2, double
     fv = -6.0 * n2 * p3;
   }
   o.phi = n0 * (-ln3) + (n1 * n2 - dot12) * p2 + q3 * p3;
-  const double dn0 = -ln3;
-  const double dn1 = n2 * p2;
-  const double dn2 = n1 * p2 + dq3_n2 * p3;
-  const double dn3 = n0 / (1.0 - n3) + (n1 * n2 - dot12) * d2 + q3 * d3;
+  const T dn0 = -ln3;
+  const T dn1 = n2 * p2;
+  const T dn2 = n1 * p2 + dq3_n2 * p3;
+  const T dn3 = n0 / (1.0 - n3) + (n1 * n2 - dot12) * d2 + q3 * d3;
   const double kT = fp.kT;
   o.g0 = kT * (dn0 * i4pR2 + dn1 * i4pR + dn2);
   o.g1 = kT * dn3;
@@ -501,55 +552,57 @@ __device__ __forceinline__ PhiOut fmt_point(const FuncDev& fp, double n2, double
 }
 
 // EOS term: excess free energy per volume f(rho) and df/drho.
-__device__ __forceinline__ void eos_term(int kind, const double* p, double rho, double& f, double& df) {
+template <typename T>
+__device__ __forceinline__ void eos_term(int kind, const double* p, T rho, T& f, T& df) {
   if (kind == 0) {            // mean field: a_ex = a rho
     const double a = p[0];
     f = rho * (a * rho);
     df = rho * a + a * rho;
   } else if (kind == 1) {     // Carnahan-Starling, p = {eta (=4 pi R^3/3), kT}
     const double eta = p[0], kT = p[1];
-    const double x = eta * rho;
-    const double om = 1.0 - x;
-    const double ap = kT * (4.0 * x - 3.0 * x * x) / (om * om);
-    const double dap = 2.0 * kT * eta * (2.0 - x) / (om * om * om);
+    const T x = eta * rho;
+    const T om = 1.0 - x;
+    const T ap = kT * (4.0 * x - 3.0 * x * x) / (om * om);
+    const T dap = 2.0 * kT * eta * (2.0 - x) / (om * om * om);
     f = rho * ap;
     df = rho * dap + ap;
   } else {                    // MBWR (Johnson et al. 1993): p = {a1..a8, b1..b6, sigma^3, eps}
     const double s3 = p[14], eps = p[15], gam = 3.0;
-    const double rr = rho * s3;
-    const double F = exp(-gam * rr * rr);
+    const T rr = rho * s3;
+    const T F = exp(-gam * rr * rr);
     const double ig = 1.0 / (2.0 * gam);
-    double A = 0.0, dA = 0.0, pw = 1.0;
+    T A = 0.0, dA = 0.0, pw = 1.0;
     for (int i = 0; i < 8; ++i) {      // sum a_i/(i+1) rr^(i+1) ; d: a_i rr^i s3
       dA += p[i] * pw * s3;
       pw *= rr;
       A += p[i] / (i + 1) * pw;
     }
-    double G = ig * (1.0 - F);
+    T G = ig * (1.0 - F);
     A += p[8] * G;
-    double rr2 = rr * rr, pw2 = rr2;
+    T rr2 = rr * rr, pw2 = rr2;
     for (int t = 1; t < 6; ++t) {      // G_{t+1} = -ig (F rr^(2t) - 2t G_t)
       G = -ig * (F * pw2 - 2.0 * t * G);
       A += p[8 + t] * G;
       pw2 *= rr2;
     }
-    double po = rr;                   // rr^(2t+1)
+    T po = rr;                   // rr^(2t+1)
     for (int t = 0; t < 6; ++t) {
       dA += p[8 + t] * s3 * po * F;
       po *= rr2;
     }
-    const double ap = A * eps, dap = dA * eps;
+    const T ap = A * eps, dap = dA * eps;
     f = rho * ap;
     df = rho * dap + ap;
   }
 }
 
-__device__ __forceinline__ void wda_point(const FuncDev& fp, double rb, double& f, double& df) {
+template <typename T>
+__device__ __forceinline__ void wda_point(const FuncDev& fp, T rb, T& f, T& df) {
   f = 0.0;
   df = 0.0;
   for (int t = 0; t < fp.neos; ++t) {
-    double ft, dft;
-    eos_term(fp.eos_kind[t], fp.eos_par[t], rb, ft, dft);
+    T ft, dft;
+    eos_term<T>(fp.eos_kind[t], fp.eos_par[t], rb, ft, dft);
     f += fp.eos_fac[t] * ft;
     df += fp.eos_fac[t] * dft;
   }
@@ -589,7 +642,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
     double& r5 = z_val(sm, t, 5, pl, e);
     double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
     if (fp.use_hs) {
-      const PhiOut o = fmt_point(fp, r0, r1, r2, r3, r4);
+      const PhiOut<double> o = fmt_point<double>(fp, r0, r1, r2, r3, r4);
       acc[0] += o.phi;
       acc[2] = fmax(acc[2], r1);
       g0 = o.g0;
@@ -600,7 +653,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
     }
     if (fp.use_wda) {
       double fw, dfw;
-      wda_point(fp, r5, fw, dfw);
+      wda_point<double>(fp, r5, fw, dfw);
       acc[1] += fw;
       g5 = dfw;
     }

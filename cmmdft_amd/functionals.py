@@ -243,6 +243,7 @@ class ExternalPotential(Functional):
         self.limit_potential = limit_potential
         self.degree = degree
         self._dev = None
+        self.generated = None     # {'images': bool} when generated from the host
 
     def copy(self, grid=None):
         e = type(self)(grid if grid is not None else self.grid.copy(), self.system, self.epot_dr,
@@ -257,6 +258,7 @@ class ExternalPotential(Functional):
             raise NotImplementedError('only spherical LJ guests are supported')
         V = get_external_potential(self.grid, self.host.ff, self.guest.sigma, self.guest.epsilon, self.host.mol,
                                    cutoff=self.cutoff, images=images, as_tensor=True)
+        self.generated = {'images': bool(images)}
         self._dev = V.reshape(self.grid.npoints)
         self.potential = self._dev.cpu().numpy()
 
@@ -284,6 +286,7 @@ class ExternalPotential(Functional):
             self.potential = potential
         else:
             raise ValueError('Grid shape %s does not match potential shape %s' % (self.grid.npoints, potential.shape))
+        self.generated = None
         self._dev = None
 
     def dump_potential(self, fn):

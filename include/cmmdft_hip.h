@@ -38,6 +38,15 @@
  *                                                      cmmdft/solver.py:103-119
  *  cmm_lj_grid/_points   get_external_potential / lennard_jones
  *                                                      cmmdft/extpot_calculator.py:156-222
+ *  cmm_lj_grid_grad      lennard_jones(derivative=True) (the reference's
+ *                        radial derivatives; here with respect to the
+ *                        host-type sigma/epsilon)     cmmdft/extpot_calculator.py:156-183
+ *  cmm_hessian_apply, cmm_adjoint_*
+ *                        no reference counterpart: the reference has no
+ *                        derivatives of the converged density; these give
+ *                        dN/dmu and dN/d(host LJ parameters) by the implicit
+ *                        function theorem at the fixed point of
+ *                        Solver.get_new_rho                cmmdft/solver.py:128-129
  */
 #ifndef CMMDFT_HIP_H
 #define CMMDFT_HIP_H
@@ -158,6 +167,26 @@ int cmm_graph_end(cmm_plan* plan, void** exec_out);
 int cmm_graph_launch(cmm_plan* plan, void* exec);
 int cmm_graph_destroy(void* exec);
 
+/* Linear response of a converged density (adjoint solve).
+ * cmm_hessian_apply: M v for the half spectrum of v in the plan workspace
+ * (left there by cmm_seed(v)), M = d c_ex / d rho at the density whose
+ * weighted densities nfld[nbatch][6][npts] were written by
+ * cmm_weighted_densities; cmm_finish_c1 with vext = NULL then writes M v.
+ * The adjoint solve J lambda = dr (J = diag(kT/rho) + M) runs as conjugate
+ * gradients on (I + S M S) y = S dr, S = sqrt(beta rho), lambda = S y, on
+ * caller vectors y, r, p, q [nbatch][npts]:
+ *   cmm_adjoint_seed(init=1, parity 0); then per iteration k (parity k & 1):
+ *   cmm_hessian_apply -> cmm_adjoint_final -> cmm_adjoint_update(parity) ->
+ *   cmm_adjoint_seed(init=0, parity of k+1).
+ * Scalars per instance: [24] p.q, [25 + parity] r.r, [27] sum lambda. */
+int cmm_hessian_apply(cmm_plan* plan, const double* nfld);
+int cmm_adjoint_seed(cmm_plan* plan, const double* rho, double beta, double dr, int init, int parity, double* y,
+                     double* r, double* p);
+int cmm_adjoint_final(cmm_plan* plan, const double* rho, double beta, const double* p, double* q);
+int cmm_adjoint_update(cmm_plan* plan, int parity, double* y, double* r, const double* p, const double* q);
+/* lambda = S y; sum lambda = dN/dmu into scalar [27] */
+int cmm_adjoint_lambda(cmm_plan* plan, const double* rho, double beta, const double* y, double* lambda);
+
 /* copy scalars [nbatch][CMM_NSCAL] to host (waits for the stream) */
 int cmm_get_scalars(cmm_plan* plan, double* host_out);
 int cmm_synchronize(cmm_plan* plan);
@@ -174,6 +203,16 @@ int cmm_lj_grid(int nx, int ny, int nz, const double* rvecs, const double* atoms
 /* the same sum at M explicit points (points: M x 3, device), V[M] */
 int cmm_lj_points(const double* points, long long M, const double* rvecs, const double* atoms, int natom,
                   double sigma_f, double epsilon_f, double cutoff, int images, double limit, double* V, void* stream);
+
+/* Backward of cmm_lj_grid: grad[2t] = sum_i w_i dV_i/dsigma_t,
+ * grad[2t+1] = sum_i w_i dV_i/depsilon_t for host atom types t < ntypes
+ * (types: device int[natom]); clamped points contribute nothing.  work is a
+ * device buffer of cmm_lj_grad_workspace() doubles; grad is a device
+ * double[2 ntypes]. */
+int cmm_lj_grad_workspace(int nx, int ny, int nz, int ntypes, long long* ndoubles);
+int cmm_lj_grid_grad(int nx, int ny, int nz, const double* rvecs, const double* atoms, const int* types, int natom,
+                     int ntypes, double sigma_f, double epsilon_f, double cutoff, int images, double limit,
+                     const double* w, double* work, double* grad, void* stream);
 
 #ifdef __cplusplus
 }
