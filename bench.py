@@ -134,6 +134,56 @@ def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample):
             'seconds_per_iteration': per_iter}
 
 
+def adjoint_leg(shape, reps=3):
+    """Forward solve vs adjoint gradients at one state point: the case-4 host
+    (case-1 cell a = 24 A, 64 atoms in 4 LJ types) at 10 bar on the bench
+    grid.  Forward: Anderson (m = 5, damping 0.1, RIUE < 1e-8) from the
+    ideal-gas density.  Adjoint: the linear-response CG solve (relative
+    residual 1e-10) plus dN/dmu and the 8 host-parameter gradients.  Medians
+    over ``reps`` after one warm-up."""
+    import torch
+    from cmmdft_amd import units as U, workloads as W
+    from cmmdft_amd.program import Program
+    from cmmdft_amd.system import NanoporousHost, SphericalLJGuest
+    from cmmdft_amd.eos import MFAEOS, ModifiedBenedictWebbRubinEOS
+    from cmmdft_amd.solver import Anderson
+    from cmmdft_amd.adjoint import AdjointSolver
+    host = W.host_case4()
+    prog = Program(write_files=False)
+    prog.set_system(NanoporousHost('case4', host, host.ff),
+                    SphericalLJGuest('CH4_UA', W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON))
+    prog.set_grid(npoints=list(shape))
+    prog.init_free_energy(W.TEMPERATURE)
+    prog.fener.add_hard_sphere('WBII')
+    prog.fener.add_wdav(MFAEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON))
+    prog.fener.add_external_potential(cutoff=W.CUTOFF)
+    mu = ModifiedBenedictWebbRubinEOS(W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON).calculate_mu(W.TEMPERATURE,
+                                                                                                  10 * U.bar)
+    solver = Anderson(prog, nsteps=500, method='anderson', m=5, damping=0.1, criterion='RIUE', threshold=1e-8)
+    adj = AdjointSolver(prog, tol=1e-10)
+    tf, ta = [], []
+    for k in range(reps + 1):
+        rho0 = prog._set_initial_density(chempot=mu, temperature=W.TEMPERATURE)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        N, rho, conv = solver.solve(mu, rho0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        g = adj.gradients(rho)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if k > 0:
+            tf.append(t1 - t0)
+            ta.append(t2 - t1)
+    fwd, bwd = float(np.median(tf)), float(np.median(ta))
+    return {'workload': 'case-4 host (a=24 A, 64 atoms, 4 LJ types) at %dx%dx%d, 10 bar, WBII + MFA-WDA' % shape,
+            'forward_ms': fwd * 1e3, 'adjoint_ms': bwd * 1e3, 'adjoint_vs_forward': bwd / fwd,
+            'forward_iterations': int(solver.curr_step), 'cg_iterations': int(adj.iters),
+            'cg_relative_residual': float(adj.residual), 'converged': bool(conv), 'N': float(N),
+            'dN_dmu': g['dN_dmu'], 'dN_dsigma': [float(v) for v in g['dN_dsigma']],
+            'dN_depsilon': [float(v) for v in g['dN_depsilon']]}
+
+
 def passes_only(shape, reps=5):
     """Functional chain of one realistic density (ideal-gas start at 10 bar),
     each pass launched ``reps`` times, then a plain 256 MB device-to-device
@@ -176,6 +226,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--grid', type=int, default=NSIDE)
     ap.add_argument('--cpu-sample', type=int, default=4, help='oracle iterations timed (0 skips the CPU leg)')
+    ap.add_argument('--no-adjoint', action='store_true', help='skip the forward-vs-adjoint timing')
     ap.add_argument('--passes-only', action='store_true',
                     help='short run for hardware counters: the functional passes at the bench grid, then a 256 MB '
                          'device copy as the byte-count reference')
@@ -267,6 +318,8 @@ def main():
     b_iter = npts * (16 * 14 + 24) + npts * 8 * (3 * 5 + 2)
     iter_gbps = b_iter * int(sum(int(v) for v in (biters if biters else iters))) / dt / 1e9
 
+    adjoint = None if args.no_adjoint else adjoint_leg(shape)
+
     cpu = None
     if args.cpu_sample > 0:
         V = prog.fener.part_dict['ExtPot'].potential
@@ -299,6 +352,7 @@ def main():
         'all_converged': bool(all(convs) and all(bconvs)),
         'vext_ms': vext_ms,
         'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
+        'adjoint': adjoint,
     }
     print(json.dumps(line))
 
