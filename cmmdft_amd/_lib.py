@@ -43,6 +43,8 @@ EXPORTS = {
     'cmm_plan_set_stream': ([C.c_void_p, C.c_void_p], C.c_int),
     'cmm_plan_set_tiles': ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     'cmm_plan_get_info': ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
+    'cmm_plan_set_radix': ([C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    'cmm_plan_get_radix': ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p], C.c_int),
     'cmm_plan_set_active': ([C.c_void_p, C.c_void_p], C.c_int),
     'cmm_set_functional': ([C.c_void_p, C.POINTER(CmmFunctional)], C.c_int),
     'cmm_set_instances': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
@@ -156,6 +158,9 @@ class Plan:
         self.device = torch.device('cuda', torch.cuda.current_device())
         self._scal = np.zeros(self.nbatch * NSCAL)
         self.bind_stream()
+        # tuned FFT radix sequences and tiles of this grid shape, if any
+        from . import autotune
+        self.tuned = autotune.apply(self)
 
     def bind_stream(self, stream=None):
         torch = _torch()
@@ -180,6 +185,16 @@ class Plan:
 
     def set_tiles(self, tb_x=0, tb_y=0, p_z=0, p_phi=0):
         check(self.lib.cmm_plan_set_tiles(self.h, tb_x, tb_y, p_z, p_phi))
+
+    def set_radix(self, axis, radices):
+        r = _ints([int(v) for v in radices])
+        check(self.lib.cmm_plan_set_radix(self.h, int(axis), len(radices), r))
+
+    def radix(self, axis):
+        n = C.c_int()
+        r = (C.c_int * 8)()
+        check(self.lib.cmm_plan_get_radix(self.h, int(axis), C.byref(n), r))
+        return [int(r[i]) for i in range(n.value)]
 
     def set_active(self, mask):
         self._active = mask

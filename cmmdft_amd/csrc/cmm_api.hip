@@ -74,22 +74,28 @@ struct cmm_plan {
 
 namespace {
 
+// stage divisors of a radix sequence (already in f.rad[0..nst))
+void radix_finish(Fft1D& f) {
+  int Ns = 1;
+  for (int s = 0; s < f.nst; ++s) {
+    f.fns[s] = make_fastdiv((unsigned)Ns);
+    f.fnq[s] = make_fastdiv((unsigned)(f.n / f.rad[s]));
+    Ns *= f.rad[s];
+  }
+}
+
+// default radix sequence: 5s, 3s, then 8s, 4s, 2s
 bool radix_plan(int n, Fft1D& f) {
   f.n = n;
   f.nst = 0;
   int m = n;
-  while (m % 5 == 0) { f.rad[f.nst++] = 5; m /= 5; }
-  while (m % 3 == 0) { f.rad[f.nst++] = 3; m /= 3; }
-  while (m % 8 == 0) { f.rad[f.nst++] = 8; m /= 8; }
-  while (m % 4 == 0) { f.rad[f.nst++] = 4; m /= 4; }
-  while (m % 2 == 0) { f.rad[f.nst++] = 2; m /= 2; }
-  if (m != 1 || f.nst > CMM_MAX_STAGES) return false;
-  int Ns = 1;
-  for (int s = 0; s < f.nst; ++s) {
-    f.fns[s] = make_fastdiv((unsigned)Ns);
-    f.fnq[s] = make_fastdiv((unsigned)(n / f.rad[s]));
-    Ns *= f.rad[s];
-  }
+  while (m % 5 == 0 && f.nst < CMM_MAX_STAGES) { f.rad[f.nst++] = 5; m /= 5; }
+  while (m % 3 == 0 && f.nst < CMM_MAX_STAGES) { f.rad[f.nst++] = 3; m /= 3; }
+  while (m % 8 == 0 && f.nst < CMM_MAX_STAGES) { f.rad[f.nst++] = 8; m /= 8; }
+  while (m % 4 == 0 && f.nst < CMM_MAX_STAGES) { f.rad[f.nst++] = 4; m /= 4; }
+  while (m % 2 == 0 && f.nst < CMM_MAX_STAGES) { f.rad[f.nst++] = 2; m /= 2; }
+  if (m != 1) return false;
+  radix_finish(f);
   return true;
 }
 
@@ -271,6 +277,33 @@ int cmm_plan_set_tiles(cmm_plan* p, int tbx, int tby, int pz, int pphi) {
     if ((pphi & 1) || 3 * pphi * p->nz > budget || zlds(p, pphi, 6) > 65536) return fail(CMM_E_ARG, "p_phi invalid");
     p->pphi = pphi;
   }
+  return CMM_OK;
+}
+
+int cmm_plan_set_radix(cmm_plan* p, int axis, int nstages, const int* radices) {
+  if (!p || !radices) return fail(CMM_E_ARG, "null argument");
+  if (axis < 0 || axis > 2) return fail(CMM_E_ARG, "axis must be 0, 1 or 2");
+  if (nstages < 1 || nstages > CMM_MAX_STAGES) return fail(CMM_E_ARG, "bad number of stages");
+  Fft1D* f = axis == 0 ? &p->g.fx : (axis == 1 ? &p->g.fy : &p->g.fz);
+  long long prod = 1;
+  for (int s = 0; s < nstages; ++s) {
+    const int r = radices[s];
+    if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return fail(CMM_E_ARG, "radix must be 2, 3, 4, 5 or 8");
+    prod *= r;
+  }
+  if (prod != f->n) return fail(CMM_E_ARG, "radices do not multiply to the axis length");
+  f->nst = nstages;
+  for (int s = 0; s < nstages; ++s) f->rad[s] = radices[s];
+  radix_finish(*f);
+  return CMM_OK;
+}
+
+int cmm_plan_get_radix(cmm_plan* p, int axis, int* nstages, int* radices) {
+  if (!p || !nstages || !radices) return fail(CMM_E_ARG, "null argument");
+  if (axis < 0 || axis > 2) return fail(CMM_E_ARG, "axis must be 0, 1 or 2");
+  const Fft1D* f = axis == 0 ? &p->g.fx : (axis == 1 ? &p->g.fy : &p->g.fz);
+  *nstages = f->nst;
+  for (int s = 0; s < f->nst; ++s) radices[s] = f->rad[s];
   return CMM_OK;
 }
 

@@ -63,6 +63,7 @@ extern "C" {
 
 #define CMM_NSCAL 32      /* doubles of reduction results per instance */
 #define CMM_MMAX_HIST 8   /* longest Anderson history */
+#define CMM_MAX_RADIX_STAGES 8
 
 typedef struct cmm_plan cmm_plan;
 
@@ -95,6 +96,12 @@ int cmm_plan_set_stream(cmm_plan* plan, void* stream);
 /* tile shapes: x-pass columns, y-pass columns, z-pass pencils (simple),
  * z-pass pencils (6-field functional pass); 0 keeps the current value */
 int cmm_plan_set_tiles(cmm_plan* plan, int tb_x, int tb_y, int p_z, int p_phi);
+/* FFT radix sequence of one axis (0 x, 1 y, 2 z): radices from {2,3,4,5,8}
+ * whose product is the axis length, at most CMM_MAX_RADIX_STAGES stages,
+ * applied in order (the autotuner's search space together with the tiles;
+ * the default is 5s, 3s, 8s, 4s, 2s) */
+int cmm_plan_set_radix(cmm_plan* plan, int axis, int nstages, const int* radices);
+int cmm_plan_get_radix(cmm_plan* plan, int axis, int* nstages, int* radices);
 /* info[0..9] = nx, ny, nz, nbatch, tb_x, tb_y, p_z, p_phi, workspace bytes, nzh */
 int cmm_plan_get_info(cmm_plan* plan, long long* info, int n);
 /* device int[nbatch] mask of instances to process (NULL = all) */

@@ -131,3 +131,16 @@ def test_product_eos_matches_oracle_state_points():
     kinds = [k for k, f, par in terms.device_terms()]
     facs = [f for k, f, par in terms.device_terms()]
     assert kinds == [E.EOS_MBWR, E.EOS_CS, E.EOS_MFA] and facs == [1, -1, -1]
+
+
+def test_radix_candidates_multiply_to_length():
+    from cmmdft_amd.autotune import radix_candidates
+    for n in (8, 24, 64, 72, 96, 100, 128, 160, 250, 256):
+        c = radix_candidates(n)
+        assert c, n
+        assert len(set(c)) == len(c)
+        lo = min(len(s) for s in c)
+        for s in c:
+            assert int(np.prod(s)) == n and set(s) <= {2, 3, 4, 5, 8} and len(s) <= lo + 1
+    # 128 = 8*8*2 (3 orders), 8*4*4 (3), 8*4*2*2 (12), 4*4*4*2 (4)
+    assert len(radix_candidates(128)) == 22
