@@ -152,8 +152,7 @@ __global__ void __launch_bounds__(NT) k_adj_seed(GridDev g, AdjDev a, ZTile t, d
   if (MODE == ADJ_INIT) {
     const int ops[1] = {0};
     block_reduce<1>(acc, ops, sh);
-    grid_reduce_finish<1>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                          red.counters + inst * RED_SITES + 2, red.scal + inst * NSCAL + S_ADJ_RR, sh);
+    grid_reduce_store<1>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
   }
 }
 
@@ -188,8 +187,7 @@ __global__ void __launch_bounds__(NT) k_adj_final(const double2* __restrict__ F,
   }
   const int ops[1] = {0};
   block_reduce<1>(acc, ops, sh);
-  grid_reduce_finish<1>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                        red.counters + inst * RED_SITES + 1, red.scal + inst * NSCAL + S_ADJ_PQ, sh);
+  grid_reduce_store<1>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 // y += alpha p, r -= alpha q with alpha = rr_cur / p.q; reduces the new r.r
@@ -213,9 +211,7 @@ __global__ void __launch_bounds__(NT) k_adj_update(AdjDev a, const int* __restri
   }
   const int ops[1] = {0};
   block_reduce<1>(acc, ops, sh);
-  grid_reduce_finish<1>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                        red.counters + inst * RED_SITES + 3, red.scal + inst * NSCAL + S_ADJ_RR + (a.parity ^ 1),
-                        sh);
+  grid_reduce_store<1>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 // lambda = S y; reduces sum lambda (= dN/dmu).
@@ -234,8 +230,7 @@ __global__ void __launch_bounds__(NT) k_adj_lambda(AdjDev a, const int* __restri
   }
   const int ops[1] = {0};
   block_reduce<1>(acc, ops, sh);
-  grid_reduce_finish<1>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                        red.counters + inst * RED_SITES + 3, red.scal + inst * NSCAL + S_ADJ_LAM, sh);
+  grid_reduce_store<1>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 }  // namespace cmm

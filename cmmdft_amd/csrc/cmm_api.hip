@@ -50,7 +50,6 @@ struct cmm_plan {
   long long npts;
   double* partials;
   long long pstride;
-  unsigned* counters;
   double* scal;
   InstDev* inst;
   double* gram;
@@ -124,7 +123,16 @@ void default_tiles(cmm_plan* p) {
   p->pphi = largest_even_le(pphi);
 }
 
-RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->counters, p->scal}; }
+RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->scal}; }
+
+// totals of the partials the previous kernel stored (see k_reduce_partials)
+int launch_reduce(cmm_plan* p, long long nblocks, int nq, unsigned maxmask, int slot) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3(p->nb), dim3(NT), 0, p->stream, red_of(p), (int)nblocks, nq, maxmask,
+                     slot, p->active);
+  hipError_t e_ = hipGetLastError();
+  if (e_ != hipSuccess) return fail(CMM_E_HIP, std::string("k_reduce_partials: ") + hipGetErrorString(e_));
+  return CMM_OK;
+}
 
 long long zblocks(cmm_plan* p, int P) { return ((long long)p->nx * p->ny + P - 1) / P; }
 
@@ -209,7 +217,6 @@ int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm
   e = (e == hipSuccess) ? hipMalloc(&p->rz, half_bytes) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->F, 6 * half_bytes) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->partials, (size_t)p->pstride * nbatch * sizeof(double)) : e;
-  e = (e == hipSuccess) ? hipMalloc(&p->counters, (size_t)RED_SITES * nbatch * sizeof(unsigned)) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->scal, (size_t)NSCAL * nbatch * sizeof(double)) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->inst, (size_t)nbatch * sizeof(InstDev)) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->gram, (size_t)nbatch * CMM_MMAX * CMM_MMAX * sizeof(double)) : e;
@@ -221,7 +228,6 @@ int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm
     cmm_plan_destroy(p);
     return fail(CMM_E_HIP, std::string("workspace allocation: ") + hipGetErrorString(e));
   }
-  (void)hipMemset(p->counters, 0, (size_t)RED_SITES * nbatch * sizeof(unsigned));
   (void)hipMemset(p->scal, 0, (size_t)NSCAL * nbatch * sizeof(double));
   (void)hipMemset(p->gram, 0, (size_t)nbatch * CMM_MMAX * CMM_MMAX * sizeof(double));
   (void)hipMemset(p->coef, 0, (size_t)nbatch * 2 * CMM_MMAX * sizeof(double));
@@ -243,7 +249,7 @@ int cmm_plan_destroy(cmm_plan* p) {
     if (p->ktab[a]) (void)hipFree(p->ktab[a]);
     if (p->ltab[a]) (void)hipFree(p->ltab[a]);
   }
-  void* bufs[] = {p->rz, p->F, p->partials, p->counters, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef,
+  void* bufs[] = {p->rz, p->F, p->partials, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef,
                   p->alpha_out, p->b_active, p->b_iters, p->b_status, p->b_final, p->b_omega_prev, p->b_omega,
                   p->b_mu};
   for (void* b : bufs)
@@ -444,7 +450,7 @@ static int launch_produce(cmm_plan* p, const ElDev& el, const char* name) {
   hipLaunchKernelGGL(k_zproduce<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, el, ztile(p, p->pz, 1),
                      p->active, red_of(p));
   LAUNCHCHK(name);
-  return CMM_OK;
+  return launch_reduce(p, grid.x, 7, 1u << 2, S_PN);
 }
 
 int cmm_seed(cmm_plan* p, const double* rho, const double* vext, int vb, double lam3) {
@@ -471,7 +477,7 @@ static int launch_zphi(cmm_plan* p, int store) {
   hipLaunchKernelGGL(k_zphi, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g,
                      p->fp, ztile(p, p->pphi, 6), store, p->active, red_of(p));
   LAUNCHCHK("k_zphi");
-  return CMM_OK;
+  return launch_reduce(p, grid.x, 3, 1u << 2, S_PHI);
 }
 
 // back-convolution of the 6 derivative fields in F: inverse y, the x pass
@@ -561,7 +567,8 @@ static int launch_final(cmm_plan* p, const ElDev& el, const char* name) {
   hipLaunchKernelGGL(k_zfinal<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, el,
                      ztile(p, p->pz, 1), p->active, red_of(p));
   LAUNCHCHK(name);
-  return CMM_OK;
+  constexpr int NQ = (MODE == MODE_ANDRES) ? (4 + CMM_MMAX) : 8;
+  return launch_reduce(p, grid.x, NQ, MODE == MODE_PICARD ? (1u << 5) : 0u, S_NCUR);
 }
 
 int cmm_finish_c1(cmm_plan* p, const double* rho, const double* vext, int vb, double* c1, double lam3) {
@@ -714,7 +721,7 @@ int cmm_adjoint_seed(cmm_plan* p, const double* rho, double beta, double dr, int
     hipLaunchKernelGGL(k_adj_seed<ADJ_STEP>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->g, a,
                        ztile(p, p->pz, 1), p->rz, p->nhalf, p->active, red_of(p));
   LAUNCHCHK("k_adj_seed");
-  return CMM_OK;
+  return init ? launch_reduce(p, grid.x, 1, 0u, S_ADJ_RR) : CMM_OK;
 }
 
 int cmm_adjoint_final(cmm_plan* p, const double* rho, double beta, const double* pvec, double* q) {
@@ -726,7 +733,7 @@ int cmm_adjoint_final(cmm_plan* p, const double* rho, double beta, const double*
   hipLaunchKernelGGL(k_adj_final, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, a,
                      ztile(p, p->pz, 1), p->active, red_of(p));
   LAUNCHCHK("k_adj_final");
-  return CMM_OK;
+  return launch_reduce(p, grid.x, 1, 0u, S_ADJ_PQ);
 }
 
 static dim3 flat_grid(cmm_plan* p) {
@@ -743,9 +750,10 @@ int cmm_adjoint_update(cmm_plan* p, int parity, double* y, double* r, const doub
   a.p = const_cast<double*>(pvec);
   a.q = const_cast<double*>(q);
   a.parity = parity & 1;
-  hipLaunchKernelGGL(k_adj_update, flat_grid(p), dim3(NT), 0, p->stream, a, p->active, red_of(p));
+  const dim3 grid = flat_grid(p);
+  hipLaunchKernelGGL(k_adj_update, grid, dim3(NT), 0, p->stream, a, p->active, red_of(p));
   LAUNCHCHK("k_adj_update");
-  return CMM_OK;
+  return launch_reduce(p, grid.x, 1, 0u, S_ADJ_RR + ((parity & 1) ^ 1));
 }
 
 int cmm_adjoint_lambda(cmm_plan* p, const double* rho, double beta, const double* y, double* lam) {
@@ -753,9 +761,10 @@ int cmm_adjoint_lambda(cmm_plan* p, const double* rho, double beta, const double
   AdjDev a = adj_base(p, rho, beta);
   a.y = const_cast<double*>(y);
   a.lam = lam;
-  hipLaunchKernelGGL(k_adj_lambda, flat_grid(p), dim3(NT), 0, p->stream, a, p->active, red_of(p));
+  const dim3 grid = flat_grid(p);
+  hipLaunchKernelGGL(k_adj_lambda, grid, dim3(NT), 0, p->stream, a, p->active, red_of(p));
   LAUNCHCHK("k_adj_lambda");
-  return CMM_OK;
+  return launch_reduce(p, grid.x, 1, 0u, S_ADJ_LAM);
 }
 
 static int lj_setup(int nx, int ny, int nz, const double* rvecs, double sigma_f, double epsilon_f, double cutoff,

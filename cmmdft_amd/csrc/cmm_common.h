@@ -274,39 +274,16 @@ __device__ void block_reduce(double (&v)[NQ], const int (&ops)[NQ], double* sh /
   __syncthreads();
 }
 
-// Called by every block with its block-reduced values valid in thread 0.
-// partials: [NQ][nblocks]; counter: one unsigned per reduction instance.
+// Called by every block with its block-reduced values valid in thread 0:
+// stores them as partials[q*nblocks + bid].  The totals are formed by a
+// separate small kernel launched after the producer (k_reduce_partials): the
+// kernel boundary makes every block's partials visible on every XCD, where a
+// last-block-done scheme inside the producer would need a device-scope
+// release fence -- an L2 write-back and invalidate -- in every block.
 template <int NQ>
-__device__ void grid_reduce_finish(const double (&v)[NQ], const int (&ops)[NQ], double* partials, int nblocks,
-                                   int bid, unsigned* counter, double* out, double* sh) {
-  __shared__ int is_last;
+__device__ __forceinline__ void grid_reduce_store(const double (&v)[NQ], double* partials, int nblocks, int bid) {
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) partials[q * nblocks + bid] = v[q];
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    is_last = (prev == (unsigned)(nblocks - 1));
   }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  // fixed-order two-level sum: lanes stride over partials, then tree
-  for (int q = 0; q < NQ; ++q) {
-    double a = ops[q] ? -INFINITY : 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
-      double x = __hip_atomic_load(&partials[q * nblocks + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      a = ops[q] ? fmax(a, x) : a + x;
-    }
-    double r = ops[q] ? wave_max(a) : wave_sum(a);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-    if (lane == 0) sh[wid] = r;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = sh[0];
-      for (int w = 1; w < nw; ++w) t = ops[q] ? fmax(t, sh[w]) : t + sh[w];
-      out[q] = t;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *counter = 0u;
 }

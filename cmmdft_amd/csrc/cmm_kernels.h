@@ -12,7 +12,6 @@ enum { MODE_MIX = 0, MODE_TRIAL = 1, MODE_SEED = 2 };
 
 // Per-instance reduction results (doubles, on the device), NSCAL per instance.
 constexpr int NSCAL = 32;
-constexpr int RED_SITES = 4;
 constexpr int S_PHI = 0;    // sum Phi, sum f_ex(rho_bar), max n3            (functional pass)
 constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (final pass)
                             //   picard: N', sum|d|, max|d|, nonfinite
@@ -54,9 +53,8 @@ struct InstDev {
 };
 
 struct RedDev {
-  double* partials;
+  double* partials;   // per instance (stride pstride): [quantity][block]
   long long pstride;
-  unsigned* counters;
   double* scal;
 };
 

@@ -671,8 +671,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
-  grid_reduce_finish<3>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                        red.counters + inst * RED_SITES + 0, red.scal + inst * NSCAL + S_PHI, sh);
+  grid_reduce_store<3>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -762,8 +761,7 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
   for (int q = 0; q < NQ; ++q) ops[q] = 0;
   if (MODE == MODE_PICARD) ops[5] = 1;
   block_reduce<NQ>(acc, ops, sh);
-  grid_reduce_finish<NQ>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                         red.counters + inst * RED_SITES + 1, red.scal + inst * NSCAL + S_NCUR, sh);
+  grid_reduce_store<NQ>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 // Density producer passes (fused with the z R2C of the produced density):
@@ -830,8 +828,38 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
   z_store_half(sm, el.rz + inst * el.rzstride, p0, t, npen, n, g.nzh, 0);
   const int ops[7] = {0, 0, 1, 0, 0, 0, 0};
   block_reduce<7>(acc, ops, sh);
-  grid_reduce_finish<7>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x,
-                        red.counters + inst * RED_SITES + 2, red.scal + inst * NSCAL + S_PN, sh);
+  grid_reduce_store<7>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
+}
+
+// Per-instance totals of the block partials a producer kernel stored
+// (grid_reduce_store): out[q] = sum (or max, bit q of maxmask) over the
+// nblocks partials of quantity q, in a fixed order (lane-strided, then the
+// wavefront tree, then the waves in order).  One block per instance.
+__global__ void __launch_bounds__(NT) k_reduce_partials(RedDev red, int nblocks, int nq, unsigned maxmask, int slot,
+                                                       const int* __restrict__ active) {
+  __shared__ double sh[NT / 64];
+  const int inst = blockIdx.x;
+  if (active && !active[inst]) return;
+  const double* part = red.partials + (long long)inst * red.pstride;
+  double* out = red.scal + (long long)inst * NSCAL + slot;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  for (int q = 0; q < nq; ++q) {
+    const bool mx = (maxmask >> q) & 1u;
+    double a = mx ? -INFINITY : 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+      const double x = part[(long long)q * nblocks + i];
+      a = mx ? fmax(a, x) : a + x;
+    }
+    const double r = mx ? wave_max(a) : wave_sum(a);
+    if (lane == 0) sh[wid] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = sh[0];
+      for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[w]) : t + sh[w];
+      out[q] = t;
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
