@@ -108,9 +108,8 @@ int largest_even_le(int v) { return std::max(2, v - (v & 1)); }
 
 void default_tiles(cmm_plan* p) {
   const int budget = VPT * NT;               // complex values a block stages
-  // x pass keeps two tiles (64 KB LDS budget): nx*TB*2*16 <= 65536
-  int tbx = std::min(32, std::min(budget / p->nx, 2048 / p->nx));
-  p->tbx = std::max(1, tbx);
+  // x passes: one tile of nx * tb_x columns per NTX threads (all 6 fields in LDS)
+  p->tbx = std::max(1, std::min(32, NTX / p->nx));
   int tby = std::min(32, std::min(budget / p->ny, 4096 / p->ny));
   p->tby = std::max(1, tby);
   // simple z pass: (P/2)*nz <= budget and nz*((P/2)|1)*16 <= 64 KB
@@ -268,7 +267,7 @@ int cmm_plan_set_tiles(cmm_plan* p, int tbx, int tby, int pz, int pphi) {
   if (!p) return fail(CMM_E_ARG, "null plan");
   const int budget = VPT * NT;
   if (tbx > 0) {
-    if (tbx * p->nx > budget || tbx * p->nx * 32 > 65536) return fail(CMM_E_ARG, "tb_x too large");
+    if (tbx * p->nx > NTX) return fail(CMM_E_ARG, "tb_x too large (nx * tb_x must not exceed 512)");
     p->tbx = tbx;
   }
   if (tby > 0) {
@@ -394,6 +393,34 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
   return CMM_OK;
 }
 
+// x passes of the functional chain (see k_xweights / k_xback)
+static int launch_xweights(cmm_plan* p) {
+  const long long nb = (long long)p->ny * p->nzh;
+  const long long ntiles = (nb + p->tbx - 1) / p->tbx;
+  const int per = (int)((ntiles + 7) / 8);
+  const int T = p->nx * p->tbx;
+  dim3 grid((unsigned)(8 * per), 1, p->nb);
+  hipLaunchKernelGGL(k_xweights, grid, dim3(NTX), 7 * (size_t)T * sizeof(double2), p->stream, p->rz, p->nhalf, p->F,
+                     6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx, make_fastdiv((unsigned)p->tbx),
+                     make_fastdiv((unsigned)(6 * p->tbx)), make_fastdiv((unsigned)p->nzh), per, ntiles, p->active);
+  LAUNCHCHK("k_xweights");
+  return CMM_OK;
+}
+
+static int launch_xback(cmm_plan* p) {
+  const long long nb = (long long)p->ny * p->nzh;
+  const long long ntiles = (nb + p->tbx - 1) / p->tbx;
+  const int per = (int)((ntiles + 7) / 8);
+  const int T = p->nx * p->tbx;
+  dim3 grid((unsigned)(8 * per), 1, p->nb);
+  hipLaunchKernelGGL(k_xback, grid, dim3(NTX), 7 * (size_t)T * sizeof(double2), p->stream, p->F, 6 * p->nhalf,
+                     p->nhalf, p->g, p->fp, p->tbx, make_fastdiv((unsigned)p->tbx),
+                     make_fastdiv((unsigned)(6 * p->tbx)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh),
+                     per, ntiles, p->active);
+  LAUNCHCHK("k_xback");
+  return CMM_OK;
+}
+
 int cmm_fft_r2c(cmm_plan* p, const double* x, double* X) {
   if (!p || !x || !X) return fail(CMM_E_ARG, "null argument");
   double2* Xc = reinterpret_cast<double2*>(X);
@@ -463,12 +490,8 @@ static int forward_chain(cmm_plan* p) {
   if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
   int rc = launch_ypass(p, p->rz, 1, -1);
   if (rc) return rc;
-  const long long nb = (long long)p->ny * p->nzh;
-  dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
-  size_t lds = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
-  hipLaunchKernelGGL(k_xweights, grid, dim3(NT), lds, p->stream, p->rz, p->nhalf, p->F, 6 * p->nhalf, p->nhalf, p->g,
-                     p->fp, p->tbx, make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
-  LAUNCHCHK("k_xweights");
+  rc = launch_xweights(p);
+  if (rc) return rc;
   return launch_ypass(p, p->F, 6, 1);
 }
 
@@ -485,12 +508,8 @@ static int launch_zphi(cmm_plan* p, int store) {
 static int back_chain(cmm_plan* p) {
   int rc = launch_ypass(p, p->F, 6, -1);
   if (rc) return rc;
-  const long long nb = (long long)p->ny * p->nzh;
-  dim3 grid((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
-  size_t lds = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
-  hipLaunchKernelGGL(k_xback, grid, dim3(NT), lds, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
-                     make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
-  LAUNCHCHK("k_xback");
+  rc = launch_xback(p);
+  if (rc) return rc;
   dim3 gy(p->nx, 1, p->nb);
   hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->F,
                      6 * p->nhalf, p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
@@ -528,22 +547,10 @@ int cmm_value(cmm_plan* p) {
 int cmm_launch_pass(cmm_plan* p, int which) {
   if (!p) return fail(CMM_E_ARG, "null plan");
   if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
-  const long long nb = (long long)p->ny * p->nzh;
-  dim3 gx((unsigned)((nb + p->tbx - 1) / p->tbx), 1, p->nb);
-  size_t ldsx = 2 * (size_t)p->nx * p->tbx * sizeof(double2);
   switch (which) {
     case 0: return launch_zphi(p, 1);
-    case 1:
-      hipLaunchKernelGGL(k_xweights, gx, dim3(NT), ldsx, p->stream, p->rz, p->nhalf, p->F, 6 * p->nhalf, p->nhalf,
-                         p->g, p->fp, p->tbx, make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh),
-                         p->active);
-      LAUNCHCHK("k_xweights");
-      return CMM_OK;
-    case 2:
-      hipLaunchKernelGGL(k_xback, gx, dim3(NT), ldsx, p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx,
-                         make_fastdiv((unsigned)p->tbx), make_fastdiv((unsigned)p->nzh), p->active);
-      LAUNCHCHK("k_xback");
-      return CMM_OK;
+    case 1: return launch_xweights(p);
+    case 2: return launch_xback(p);
     case 3: return launch_ypass(p, p->F, 6, -1);
     case 4: return launch_ypass(p, p->rz, 1, -1);
     default: return fail(CMM_E_ARG, "unknown pass");

@@ -18,7 +18,6 @@ namespace cmm {
 
 constexpr int NT = 256;   // threads per block of the FFT passes
 constexpr int VPT = 8;    // complex values per thread per Stockham stage
-constexpr int EPTX = 8;   // x-pass tile elements per thread (nx * TB <= EPTX * NT)
 
 // ---------------------------------------------------------------------------
 // z-pass tiles.  A block owns P consecutive z pencils (two real pencils per
@@ -292,112 +291,116 @@ __device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix
   }
 }
 
-// x pass of the forward chain: rho (z,y transformed) -> x FFT -> 6 weighted
-// copies -> inverse x FFT each -> F[0..5].  The 1/N of the inverse 3-D
-// transform is folded into the weights.
-__global__ void __launch_bounds__(NT) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
-                                                long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB, FastDiv fTB,
-                                                FastDiv fnzh, const int* __restrict__ active) {
+// ---------------------------------------------------------------------------
+// x passes of the functional chain.  A block (NTX threads) owns a tile of TB
+// consecutive (ky, kz) columns of the flattened (y, z) plane, all nx, with
+// nx * TB <= NTX, and holds all six fields of the tile in LDS at once
+// (element-major, [x][6 TB]), so that its global loads are all in flight
+// together and one Stockham pass transforms the 6 TB sequences.
+// Tiles are mapped XCD-aware: the hardware deals consecutive workgroups to
+// the 8 XCDs in turn, so workgroup w takes tile (w % 8) * per + w / 8 (the
+// grid is padded to 8 * per): runs of consecutive tiles, which share the
+// cache lines their column ranges straddle, stay on one XCD's L2.
+// ---------------------------------------------------------------------------
+constexpr int NTX = 512;
+
+__device__ __forceinline__ long long xtile(int per) {
+  return (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// forward chain: rho (z,y transformed) -> x FFT -> 6 weighted copies ->
+// inverse x FFT of all six -> F[0..5].  The 1/N of the inverse 3-D transform
+// is folded into the weights.
+__global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
+                                                 long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
+                                                 FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
+                                                 const int* __restrict__ active) {
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
-  const int tile = g.nx * TB;
-  double2* A = sm;
-  double2* B = sm + tile;
+  const long long tile = xtile(per);
+  if (tile >= ntiles) return;
+  const int T = g.nx * TB;
+  const int S6 = 6 * TB;
+  double2* A = sm;        // [nx][TB]
+  double2* B = sm + T;    // [nx][6 TB]
   const long long nb = (long long)g.ny * g.nzh;
-  const long long b0 = (long long)blockIdx.x * TB;
+  const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = rz + inst * rzs + b0;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+  for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     A[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
   }
   __syncthreads();
   lds_fft<-1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
-  KW wr[EPTX];
-  int wy[EPTX], wz[EPTX];
+  for (int i = threadIdx.x; i < T; i += blockDim.x) {
+    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+    const unsigned bb = (unsigned)min(b0 + b, nb - 1);
+    const int wy = (int)fdiv(bb, fnzh);
+    const int wz = (int)bb - wy * g.nzh;
+    const KW w = kweights(g, fp, x, wy, wz);
+    const double2 a = A[i];
 #pragma unroll
-  for (int e = 0; e < EPTX; ++e) {
-    const int i = threadIdx.x + e * NT;
-    if (i < tile) {
-      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-      const unsigned bb = (unsigned)min(b0 + b, nb - 1);
-      wy[e] = (int)fdiv(bb, fnzh);
-      wz[e] = (int)bb - wy[e] * g.nzh;
-      wr[e] = kweights(g, fp, x, wy[e], wz[e]);
-    }
+    for (int f = 0; f < 6; ++f) B[x * S6 + f * TB + b] = cscale(apply_w(g, w, x, wy, wz, f, a, -1), g.inv_npts);
   }
-  for (int f = 0; f < 6; ++f) {
+  __syncthreads();
+  lds_fft<1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
 #pragma unroll
-    for (int e = 0; e < EPTX; ++e) {
-      const int i = threadIdx.x + e * NT;
-      if (i < tile) B[i] = cscale(apply_w(g, wr[e], (int)fdiv((unsigned)i, fTB), wy[e], wz[e], f, A[i], -1), g.inv_npts);
-    }
-    __syncthreads();
-    lds_fft<1, true, VPT>(B, TB, fTB, 1, TB, g.fx, s_tw);
+  for (int f = 0; f < 6; ++f) {
     double2* dst = F + inst * Fs + f * Ffs + b0;
-#pragma unroll 4
-    for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+    for (int i = threadIdx.x; i < T; i += blockDim.x) {
       const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-      if (b < tbv) dst[(long long)x * nb + b] = B[i];
+      if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + f * TB + b];
     }
-    __syncthreads();
   }
 }
 
-// x pass of the back-convolution: 6 derivative fields -> x FFT each ->
-// weighted sum -> inverse x FFT -> F[0] (in place: every field's tile is read
-// before the sum is written).
-__global__ void __launch_bounds__(NT) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
-                                             int TB, FastDiv fTB, FastDiv fnzh, const int* __restrict__ active) {
+// back-convolution: the 6 derivative fields -> x FFT of all six -> weighted
+// sum -> inverse x FFT -> F[0] (in place: the block reads its whole tile of
+// every field before it writes field 0's tile).
+__global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
+                                              FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fT, FastDiv fnzh,
+                                              int per, long long ntiles, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
-  const int tile = g.nx * TB;
-  double2* A = sm;
-  double2* B = sm + tile;
+  const long long tile = xtile(per);
+  if (tile >= ntiles) return;
+  const int T = g.nx * TB;
+  const int S6 = 6 * TB;
+  double2* B = sm;             // [nx][6 TB]
+  double2* A = sm + 6 * T;     // [nx][TB]
   const long long nb = (long long)g.ny * g.nzh;
-  const long long b0 = (long long)blockIdx.x * TB;
+  const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
-  KW wr[EPTX];
-  int wy[EPTX], wz[EPTX];
-#pragma unroll
-  for (int e = 0; e < EPTX; ++e) {
-    const int i = threadIdx.x + e * NT;
-    if (i < tile) {
-      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-      const unsigned bb = (unsigned)min(b0 + b, nb - 1);
-      wy[e] = (int)fdiv(bb, fnzh);
-      wz[e] = (int)bb - wy[e] * g.nzh;
-      wr[e] = kweights(g, fp, x, wy[e], wz[e]);
-      A[i] = make_double2(0.0, 0.0);
-    }
+  const double2* src = F + inst * Fs + b0;
+  for (int i = threadIdx.x; i < 6 * T; i += blockDim.x) {
+    const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
+    const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
+    B[x * S6 + f * TB + b] = (b < tbv) ? src[f * Ffs + (long long)x * nb + b] : make_double2(0.0, 0.0);
   }
-  for (int f = 0; f < 6; ++f) {
-    const double2* src = F + inst * Fs + f * Ffs + b0;
-#pragma unroll 4
-    for (int i = threadIdx.x; i < tile; i += blockDim.x) {
-      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-      B[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    lds_fft<-1, true, VPT>(B, TB, fTB, 1, TB, g.fx, s_tw);
+  __syncthreads();
+  lds_fft<-1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
+  for (int i = threadIdx.x; i < T; i += blockDim.x) {
+    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+    const unsigned bb = (unsigned)min(b0 + b, nb - 1);
+    const int wy = (int)fdiv(bb, fnzh);
+    const int wz = (int)bb - wy * g.nzh;
+    const KW w = kweights(g, fp, x, wy, wz);
+    double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int e = 0; e < EPTX; ++e) {
-      const int i = threadIdx.x + e * NT;
-      if (i < tile) A[i] = cadd(A[i], cscale(apply_w(g, wr[e], (int)fdiv((unsigned)i, fTB), wy[e], wz[e], f, B[i], 1), g.inv_npts));
-    }
-    __syncthreads();
+    for (int f = 0; f < 6; ++f) acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, f, B[x * S6 + f * TB + b], 1), g.inv_npts));
+    A[i] = acc;
   }
+  __syncthreads();
   lds_fft<1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
   double2* dst = F + inst * Fs + b0;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tile; i += blockDim.x) {
+  for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     if (b < tbv) dst[(long long)x * nb + b] = A[i];
   }
