@@ -367,17 +367,26 @@ int cmm_set_instances(cmm_plan* p, const double* fug, const double* alpha) {
 // --------------------------------------------------------------------------
 // launch helpers
 // --------------------------------------------------------------------------
-static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
-  dim3 grid(p->nx, nf, p->nb);
+// y passes over nf fields (field stride nhalf, instance stride is)
+static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sgn) {
+  const int ntz = (p->nzh + p->tby - 1) / p->tby;
+  const long long ntiles = (long long)p->nx * ntz;
+  const int per = (int)((ntiles + 7) / 8);
+  dim3 grid((unsigned)(8 * per), nf, p->nb);
   size_t lds = (size_t)p->ny * p->tby * sizeof(double2);
-  long long is = (nf == 1 && F == p->rz) ? p->nhalf : 6 * p->nhalf;
   const FastDiv fT = make_fastdiv((unsigned)p->tby);
   if (sgn < 0)
-    hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, p->active);
+    hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per,
+                       p->active);
   else
-    hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, p->active);
+    hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per,
+                       p->active);
   LAUNCHCHK("k_ypass");
   return CMM_OK;
+}
+
+static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
+  return launch_ypass_is(p, F, (nf == 1 && F == p->rz) ? p->nhalf : 6 * p->nhalf, nf, sgn);
 }
 
 static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
@@ -429,12 +438,8 @@ int cmm_fft_r2c(cmm_plan* p, const double* x, double* X) {
                      ztile(p, p->pz, 1), p->active);
   LAUNCHCHK("k_zr2c");
   // y and x passes on the caller's buffer (instance stride nhalf)
-  {
-    dim3 gy(p->nx, 1, p->nb);
-    hipLaunchKernelGGL(k_ypass<-1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, Xc, p->nhalf,
-                       p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
-    LAUNCHCHK("k_ypass");
-  }
+  int rc = launch_ypass_is(p, Xc, p->nhalf, 1, -1);
+  if (rc) return rc;
   return launch_xpass(p, Xc, p->nhalf, -1);
 }
 
@@ -444,12 +449,8 @@ int cmm_fft_c2r(cmm_plan* p, const double* X, double* x) {
   HIPCHK(hipMemcpyAsync(p->rz, X, (size_t)p->nhalf * p->nb * sizeof(double2), hipMemcpyDeviceToDevice, p->stream));
   int rc = launch_xpass(p, p->rz, p->nhalf, 1);
   if (rc) return rc;
-  {
-    dim3 gy(p->nx, 1, p->nb);
-    hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->rz, p->nhalf,
-                       p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
-    LAUNCHCHK("k_ypass");
-  }
+  rc = launch_ypass_is(p, p->rz, p->nhalf, 1, 1);
+  if (rc) return rc;
   dim3 grid((unsigned)zblocks(p, p->pz), 1, p->nb);
   hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->rz, p->nhalf, 0LL, x, p->npts, 0LL,
                      p->g, ztile(p, p->pz, 1), p->active);
@@ -510,11 +511,7 @@ static int back_chain(cmm_plan* p) {
   if (rc) return rc;
   rc = launch_xback(p);
   if (rc) return rc;
-  dim3 gy(p->nx, 1, p->nb);
-  hipLaunchKernelGGL(k_ypass<1>, gy, dim3(NT), (size_t)p->ny * p->tby * sizeof(double2), p->stream, p->F,
-                     6 * p->nhalf, p->nhalf, p->g, p->tby, make_fastdiv((unsigned)p->tby), p->active);
-  LAUNCHCHK("k_ypass");
-  return CMM_OK;
+  return launch_ypass_is(p, p->F, 6 * p->nhalf, 1, 1);
 }
 
 int cmm_derive(cmm_plan* p) {

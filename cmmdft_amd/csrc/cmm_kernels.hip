@@ -169,37 +169,43 @@ __global__ void __launch_bounds__(NT) k_zc2r(const double2* __restrict__ X, long
   }
 }
 
-// y pass: a block owns one x plane of one field and walks its kz columns in
-// tiles of TB (all y each).  Keeping a plane's tiles on one block makes the
-// cache lines two neighbouring tiles share (rows of nz/2+1 complex are not
-// line aligned) hit in that CU's caches instead of being fetched twice.
+// Tile order of the y and x passes, XCD-aware: the hardware deals
+// consecutive workgroups to the 8 XCDs in turn, so workgroup w takes tile
+// (w % 8) * per + w / 8 (the grid is padded to 8 * per): runs of consecutive
+// tiles, which share the cache lines their column ranges straddle, stay on
+// one XCD's L2.
+__device__ __forceinline__ long long xcd_tile(int per) {
+  return (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
+// y pass: one block per tile of TB consecutive kz columns of one x plane (all
+// y) of one field; tiles t = x * ntz + kz-tile.
 template <int SGN>
 __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
-                                             FastDiv fTB, const int* __restrict__ active) {
+                                             FastDiv fTB, int ntz, int per, const int* __restrict__ active) {
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fy);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
-  const int x = blockIdx.x;
-  double2* plane = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh;
+  const long long t = xcd_tile(per);
+  if (t >= (long long)g.nx * ntz) return;
+  const int x = (int)(t / ntz);
+  const int k0 = (int)(t - (long long)x * ntz) * TB;
+  const int tbv = min(TB, g.nzh - k0);
+  double2* base = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh + k0;
   const int tot = g.ny * TB;
-  for (int k0 = 0; k0 < g.nzh; k0 += TB) {
-    const int tbv = min(TB, g.nzh - k0);
-    double2* base = plane + k0;
 #pragma unroll 4
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-      const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
-      sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fy, s_tw);
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
+    sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fy, s_tw);
 #pragma unroll 4
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-      const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
-      if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
-    }
-    __syncthreads();
+  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
+    if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
   }
 }
 
@@ -297,16 +303,9 @@ __device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix
 // nx * TB <= NTX, and holds all six fields of the tile in LDS at once
 // (element-major, [x][6 TB]), so that its global loads are all in flight
 // together and one Stockham pass transforms the 6 TB sequences.
-// Tiles are mapped XCD-aware: the hardware deals consecutive workgroups to
-// the 8 XCDs in turn, so workgroup w takes tile (w % 8) * per + w / 8 (the
-// grid is padded to 8 * per): runs of consecutive tiles, which share the
-// cache lines their column ranges straddle, stay on one XCD's L2.
+// Tiles are taken in the XCD-aware order of xcd_tile.
 // ---------------------------------------------------------------------------
 constexpr int NTX = 512;
-
-__device__ __forceinline__ long long xtile(int per) {
-  return (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
-}
 
 // forward chain: rho (z,y transformed) -> x FFT -> 6 weighted copies ->
 // inverse x FFT of all six -> F[0..5].  The 1/N of the inverse 3-D transform
@@ -320,7 +319,7 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
-  const long long tile = xtile(per);
+  const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
   const int S6 = 6 * TB;
@@ -369,7 +368,7 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
-  const long long tile = xtile(per);
+  const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
   const int S6 = 6 * TB;
