@@ -88,7 +88,7 @@ EXPORTS = {
     'cmm_adjoint_lambda': ([C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
     'cmm_lj_grad_workspace': ([C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_longlong)], C.c_int),
     'cmm_lj_grid_grad': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
-                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p,
+                          C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
                           C.c_void_p, C.c_void_p], C.c_int),
     'cmm_lj_points': ([C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                        C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
@@ -350,17 +350,19 @@ def lj_points(points_dev, M, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff
 
 def lj_grid_grad(npoints, rvecs, atoms_dev, types_dev, natom, ntypes, sigma_f, epsilon_f, cutoff, images, limit, w,
                  stream=None):
-    """Device tensor (ntypes, 2): sum_i w_i dV_i/dsigma_t, sum_i w_i dV_i/depsilon_t."""
+    """Device tensor (ntypes, 2): sum_i w_i dV_i/dsigma_t, sum_i w_i dV_i/depsilon_t;
+    ``w`` is one weight field (npts) or several (nw, npts), summed per point."""
+    nw = 1 if w.dim() == 1 else int(w.shape[0])
     torch = _torch()
     lib = load()
     rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
     s = stream if stream is not None else torch.cuda.current_stream()
-    nw = C.c_longlong()
-    check(lib.cmm_lj_grad_workspace(int(npoints[0]), int(npoints[1]), int(npoints[2]), int(ntypes), C.byref(nw)))
-    work = torch.empty(int(nw.value), dtype=torch.float64, device=w.device)
+    nwork = C.c_longlong()
+    check(lib.cmm_lj_grad_workspace(int(npoints[0]), int(npoints[1]), int(npoints[2]), int(ntypes), C.byref(nwork)))
+    work = torch.empty(int(nwork.value), dtype=torch.float64, device=w.device)
     grad = torch.empty((int(ntypes), 2), dtype=torch.float64, device=w.device)
     check(lib.cmm_lj_grid_grad(int(npoints[0]), int(npoints[1]), int(npoints[2]), rv.ctypes.data_as(C.c_void_p),
                                _ptr(atoms_dev), _ptr(types_dev), int(natom), int(ntypes), float(sigma_f),
-                               float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(w), _ptr(work),
+                               float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(w), nw, _ptr(work),
                                _ptr(grad), C.c_void_p(s.cuda_stream)))
     return grad

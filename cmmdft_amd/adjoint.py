@@ -26,7 +26,7 @@ from ._device import torch_mod, as_device
 from . import _lib as L
 from .solver import _Engine
 
-__all__ = ['AdjointSolver', 'uptake', 'set_host_parameters', 'host_parameters']
+__all__ = ['AdjointSolver', 'BatchedAdjoint', 'uptake', 'set_host_parameters', 'host_parameters']
 
 
 class AdjointSolver:
@@ -107,20 +107,7 @@ class AdjointSolver:
     def host_gradients(self, lam):
         """(ntypes, 2) array of dN/dsigma_t, dN/depsilon_t for the host
         force-field types (order of ``host.mol.ffatypes``), given lambda."""
-        torch = torch_mod()
-        ep = self.fener.part_dict.get('ExtPot')
-        if ep is None or getattr(ep, 'generated', None) is None:
-            raise ValueError('host-parameter gradients need an external potential generated from the host '
-                             '(add_external_potential without potential= or load_fn=)')
-        host, guest = ep.host, ep.guest
-        from .extpot import host_atom_table
-        tab = as_device(host_atom_table(host.ff, host.mol))
-        types = torch.from_numpy(np.asarray(host.mol.ffatype_ids, dtype=np.int32)).to(tab.device)
-        ntypes = len(host.mol.ffatypes)
-        g = L.lj_grid_grad(self.grid.npoints, np.asarray(self.grid.cell.rvecs, dtype=np.float64), tab, types,
-                           tab.shape[0], ntypes, guest.sigma, guest.epsilon, ep.cutoff, int(ep.generated['images']),
-                           ep.limit_potential, as_device(lam).reshape(-1).contiguous())
-        return -g.cpu().numpy()
+        return _host_grad(self.fener, self.grid, as_device(lam).reshape(-1).contiguous())
 
     def gradients(self, rho):
         """dict(dN_dmu, dN_dsigma (ntypes,), dN_depsilon (ntypes,), iters)."""
@@ -132,6 +119,93 @@ class AdjointSolver:
             out['dN_dsigma'] = g[:, 0]
             out['dN_depsilon'] = g[:, 1]
         return out
+
+
+def _host_grad(fener, grid, lam):
+    """-(sum over the weight fields of lam) . dV/dtheta for the host types:
+    (ntypes, 2) array of d/dsigma_t, d/depsilon_t."""
+    torch = torch_mod()
+    ep = fener.part_dict.get('ExtPot')
+    if ep is None or getattr(ep, 'generated', None) is None:
+        raise ValueError('host-parameter gradients need an external potential generated from the host '
+                         '(add_external_potential without potential= or load_fn=)')
+    host, guest = ep.host, ep.guest
+    from .extpot import host_atom_table
+    tab = as_device(host_atom_table(host.ff, host.mol))
+    types = torch.from_numpy(np.asarray(host.mol.ffatype_ids, dtype=np.int32)).to(tab.device)
+    g = L.lj_grid_grad(grid.npoints, np.asarray(grid.cell.rvecs, dtype=np.float64), tab, types, tab.shape[0],
+                       len(host.mol.ffatypes), guest.sigma, guest.epsilon, ep.cutoff, int(ep.generated['images']),
+                       ep.limit_potential, lam)
+    return -g.cpu().numpy()
+
+
+class BatchedAdjoint:
+    """Adjoint solves for a batch of converged densities of one program (the
+    state points of one framework), in lockstep: one plan of B instances, the
+    CG scalars per instance; iterates until every instance's relative residual
+    is below ``tol``.  ``gradients`` also returns the derivatives of the total
+    uptake sum_i N_i with respect to the host type parameters, from one LJ
+    backward pass over sum_i lambda_i."""
+
+    def __init__(self, program, tol=1e-10, maxiter=1000):
+        self.program = program
+        self.grid = program.grid
+        self.fener = program.fener
+        self.tol = tol
+        self.maxiter = maxiter
+        self.iters = 0
+        self.residual = None
+
+    def solve(self, rho):
+        """rho: (B, nx, ny, nz) device tensor.  Returns (lambda (B, npts),
+        dN_i/dmu_i (B,))."""
+        torch = torch_mod()
+        from ._lib import Plan
+        from .functionals import build_functional
+        hs = self.fener.part_dict.get('HardSphere')
+        wda = self.fener.part_dict.get('WDA-V')
+        r = as_device(rho).reshape(rho.shape[0], -1).contiguous()
+        B, n = r.shape
+        beta = self.fener.beta
+        lam3 = self.fener.wavelength ** 3
+        plan = Plan(self.grid.npoints, self.grid.cell.rvecs, nbatch=B)
+        plan.set_functional(build_functional(hs=hs, wda=wda, kT=1.0 / beta))
+        nfld = torch.empty((B, 6, n), dtype=torch.float64, device=r.device)
+        plan.seed(r, None, False, lam3)
+        plan.weighted_densities(nfld)
+        y, res, p, q = (torch.empty((B, n), dtype=torch.float64, device=r.device) for _ in range(4))
+        plan.adjoint_seed(r, beta, self.grid.dr, 1, 0, y, res, p)
+        bb = plan.scalars()[:, L.S_ADJ_RR].copy()
+        bb[bb == 0.0] = 1.0
+        self.iters = 0
+        for k in range(self.maxiter):
+            par = k & 1
+            plan.hessian_apply(nfld)
+            plan.adjoint_final(r, beta, p, q)
+            plan.adjoint_update(par, y, res, p, q)
+            s = plan.scalars()
+            self.iters = k + 1
+            rel = np.sqrt(s[:, L.S_ADJ_RR + (par ^ 1)] / bb)
+            if not np.all(np.isfinite(rel)):
+                raise FloatingPointError('non-finite adjoint residual')
+            self.residual = float(rel.max())
+            if self.residual <= self.tol:
+                break
+            plan.adjoint_seed(r, beta, self.grid.dr, 0, par ^ 1, y, res, p)
+        else:
+            raise RuntimeError('batched adjoint CG did not converge in %d iterations (residual %.3e)'
+                               % (self.maxiter, self.residual))
+        lam = torch.empty((B, n), dtype=torch.float64, device=r.device)
+        plan.adjoint_lambda(r, beta, y, lam)
+        return lam, plan.scalars()[:, L.S_ADJ_LAM].copy()
+
+    def gradients(self, rho):
+        """dict(dN_dmu (B,), dNtot_dsigma (ntypes,), dNtot_depsilon (ntypes,),
+        iters, residual)."""
+        lam, dmu = self.solve(rho)
+        g = _host_grad(self.fener, self.grid, lam)
+        return dict(dN_dmu=dmu, dNtot_dsigma=g[:, 0], dNtot_depsilon=g[:, 1], iters=self.iters,
+                    residual=self.residual)
 
 
 def host_parameters(program):

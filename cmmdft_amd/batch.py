@@ -41,9 +41,18 @@ from . import _lib as L
 from .functionals import build_functional
 from .solver import Anderson
 
-__all__ = ['BatchedAnderson']
+__all__ = ['BatchedAnderson', 'UnevenBatch', 'footprint_bytes']
 
 STATUS = {0: 'running', 1: 'converged', 2: 'step limit', 3: 'fallback', 4: 'non-finite'}
+
+
+def footprint_bytes(npoints, ninst, m=5):
+    """Device bytes of a batched solve: per instance 7 half spectra of the plan
+    workspace, the density (m + 1) and residual (m) history rings, plus the
+    density output and external potential of the grid."""
+    nx, ny, nz = (int(n) for n in npoints)
+    npts, nhalf = nx * ny * nz, nx * ny * (nz // 2 + 1)
+    return ninst * (7 * 16 * nhalf + (2 * m + 2) * 8 * npts) + 8 * npts
 
 
 class BatchedAnderson:
@@ -136,21 +145,24 @@ class BatchedAnderson:
         sts = [g['plan'].batch_state() for g in self._groups]
         return {k: np.concatenate([st[k] for st in sts]) for k in sts[0]}
 
-    def solve(self, rho0=None):
-        """Returns dict(N, rho (B, nx, ny, nz) device tensor, converged, iters,
-        status, omega, resolved)."""
+    def start(self, rho0=None):
+        """Upload the state points, seed every instance and run the first m
+        (warm-up) iterations; capture the periodic part into a HIP graph.
+        Work is enqueued on this solver's stream."""
         torch = torch_mod()
         self._setup()
         B, m = len(self.chempots), self.m
         self.stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(self.stream):
-            Vd = self.V if self.V is not None else torch.zeros(self.grid.npts, dtype=torch.float64, device='cuda')
+            self._Vd = self.V if self.V is not None else torch.zeros(self.grid.npts, dtype=torch.float64,
+                                                                     device='cuda')
+            self._rho0 = rho0
             for g in self._groups:
                 p, rr = g['plan'], g['rho_ring']
                 if rho0 is None:
                     # ideal-gas initial densities (program.py:296-305), one per state point
                     mu = torch.from_numpy(g['mu']).to('cuda')[:, None]
-                    rr[0].copy_(torch.exp(self.beta * (mu - Vd[None, :])) / self.lam3)
+                    rr[0].copy_(torch.exp(self.beta * (mu - self._Vd[None, :])) / self.lam3)
                 else:
                     rr[0].copy_(as_device(rho0).reshape(B, -1)[g['first']:g['first'] + g['n']])
                 g['res_ring'].zero_()
@@ -164,28 +176,52 @@ class BatchedAnderson:
             while s < m and s < self.nsteps:
                 s += 1
                 self._iteration(s)
-            st = self._state()
-            period = m * (m + 1)
-            if self.use_graph and st['active'].any() and s < self.nsteps:
+            self._s = s
+            self._st = self._state()
+            self._graphed = bool(self.use_graph and self._st['active'].any() and s < self.nsteps)
+            if self._graphed and self._graph is None:
                 p0 = self._groups[0]['plan']
-                if self._graph is None:
-                    p0.graph_begin()
-                    try:
-                        for k in range(period):
-                            self._iteration(m + 1 + k)
-                    finally:
-                        self._graph = p0.graph_end()
-                while st['active'].any() and s < self.nsteps:
-                    p0.graph_launch(self._graph)
-                    s += period
-                    st = self._state()
-            else:
-                while st['active'].any() and s < self.nsteps:
-                    s += 1
-                    self._iteration(s)
-                    if s % 8 == 0:
-                        st = self._state()
-                st = self._state()
+                p0.graph_begin()
+                try:
+                    for k in range(m * (m + 1)):
+                        self._iteration(m + 1 + k)
+                finally:
+                    self._graph = p0.graph_end()
+        return self
+
+    def running(self):
+        """True while some instance is active and the step limit is not reached
+        (as of the last poll)."""
+        return bool(self._st['active'].any()) and self._s < self.nsteps
+
+    def advance(self):
+        """Enqueue the next block of iterations (one graph replay of m (m+1)
+        iterations, or up to 8 eager iterations); does not wait."""
+        torch = torch_mod()
+        if self._graphed:
+            self._groups[0]['plan'].graph_launch(self._graph)
+            self._s += self.m * (self.m + 1)
+        else:
+            with torch.cuda.stream(self.stream):
+                for _ in range(8):
+                    if self._s >= self.nsteps:
+                        break
+                    self._s += 1
+                    self._iteration(self._s)
+
+    def poll(self):
+        """Read the device status block (waits for this solver's stream)."""
+        self._st = self._state()
+        return self.running()
+
+    def finish(self):
+        """Gather the converged densities and scalars; re-solve fallback
+        instances with the plain solver.  Returns dict(N, rho (B, nx, ny, nz)
+        device tensor, converged, iters, status, omega, resolved)."""
+        torch = torch_mod()
+        B = len(self.chempots)
+        st = self._st
+        with torch.cuda.stream(self.stream):
             scal = np.concatenate([g['plan'].scalars() for g in self._groups])
             rho = torch.empty((B,) + tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
             for g in self._groups:
@@ -200,12 +236,12 @@ class BatchedAnderson:
         iters = st['iters'].copy()
         redo = [i for i in range(B) if status[i] in (2, 3, 4)] if self.resolve_fallbacks else []
         for i in redo:
-            solver = Anderson(self.program, nsteps=self.nsteps, method='anderson', m=m, damping=self.damping,
+            solver = Anderson(self.program, nsteps=self.nsteps, method='anderson', m=self.m, damping=self.damping,
                               criterion='RIUE', threshold=self.threshold, adaptive_damping=self.adaptive,
                               damping_max=self.dmax, damping_min=self.dmin, damping_factors=self.factors)
             mu = self.chempots[i]
-            r0 = torch.exp(self.beta * (mu - Vd.reshape(self.grid.npoints))) / self.lam3 if rho0 is None \
-                else as_device(rho0)[i]
+            r0 = torch.exp(self.beta * (mu - self._Vd.reshape(self.grid.npoints))) / self.lam3 \
+                if self._rho0 is None else as_device(self._rho0)[i]
             Ni, ri, ci = solver.solve(mu, r0)
             rho[i].copy_(ri.reshape(self.grid.npoints))
             N[i] = Ni
@@ -214,7 +250,96 @@ class BatchedAnderson:
         return dict(N=N, rho=rho, converged=status == 1, iters=iters, status=status, omega=st['omega'],
                     resolved=redo)
 
+    def solve(self, rho0=None):
+        """Returns dict(N, rho (B, nx, ny, nz) device tensor, converged, iters,
+        status, omega, resolved)."""
+        self.start(rho0)
+        while self.running():
+            self.advance()
+            self.poll()
+        return self.finish()
+
+    def footprint(self):
+        """Device bytes this solver's groups hold (spectra, rings, scalars)."""
+        return footprint_bytes(self.grid.npoints, len(self.chempots), self.m)
+
     def close(self):
         if self._graph is not None and self._groups:
             self._groups[0]['plan'].graph_destroy(self._graph)
         self._graph = None
+
+
+class UnevenBatch:
+    """State points of many frameworks of different grid sizes, solved
+    concurrently on one GPU.
+
+    ``items`` is a list of (program, chempots): one program per framework (its
+    grid, host and functional).  Each framework's state points form one
+    BatchedAnderson (its own plan, stream and HIP graph); frameworks are
+    admitted by device footprint -- largest first, while the admitted total
+    stays within ``mem_cap`` bytes -- and run concurrently: every round
+    launches one graph replay per running framework on its own stream, then
+    polls them, retires the finished ones and admits more.
+    """
+
+    def __init__(self, items, m=5, damping=0.1, threshold=1e-8, nsteps=500, use_graph=True, mem_cap=None,
+                 chunk=None):
+        self.items = list(items)
+        self.m, self.damping, self.threshold, self.nsteps = m, damping, threshold, nsteps
+        self.use_graph = use_graph
+        self.chunk = chunk
+        if mem_cap is None:
+            torch = torch_mod()
+            free, _total = torch.cuda.mem_get_info()
+            mem_cap = int(0.6 * free)
+        self.mem_cap = int(mem_cap)
+        self.stats = {}
+
+    def footprints(self):
+        return [footprint_bytes(p.grid.npoints, len(mu), self.m) for p, mu in self.items]
+
+    def solve(self):
+        """Returns a list (one per item) of BatchedAnderson result dicts."""
+        fps = self.footprints()
+        order = sorted(range(len(self.items)), key=lambda i: -fps[i])
+        if order and fps[order[0]] > self.mem_cap:
+            raise ValueError('framework %d needs %d bytes, above the memory cap %d'
+                             % (order[0], fps[order[0]], self.mem_cap))
+        pending = list(order)
+        running = {}
+        results = [None] * len(self.items)
+        used = peak_used = peak_running = rounds = 0
+        admitted = []
+        while pending or running:
+            # admission: largest pending framework that fits, repeatedly
+            progress = True
+            while pending and progress:
+                progress = False
+                for j, i in enumerate(pending):
+                    if used + fps[i] <= self.mem_cap:
+                        prog, mus = self.items[i]
+                        b = BatchedAnderson(prog, mus, m=self.m, damping=self.damping, threshold=self.threshold,
+                                            nsteps=self.nsteps, use_graph=self.use_graph,
+                                            chunk=self.chunk or len(mus))
+                        b.start()
+                        running[i] = b
+                        used += fps[i]
+                        admitted.append(i)
+                        pending.pop(j)
+                        progress = True
+                        break
+            peak_used = max(peak_used, used)
+            peak_running = max(peak_running, len(running))
+            rounds += 1
+            for i, b in list(running.items()):
+                if b.running():
+                    b.advance()
+            for i, b in list(running.items()):
+                if not b.poll():
+                    results[i] = b.finish()
+                    b.close()
+                    del running[i]
+                    used -= fps[i]
+        self.stats = dict(admission_order=admitted, peak_bytes=peak_used, peak_concurrent=peak_running,
+                          rounds=rounds, mem_cap=self.mem_cap)
+        return results

@@ -836,15 +836,16 @@ int cmm_lj_grad_workspace(int nx, int ny, int nz, int ntypes, long long* ndouble
 
 int cmm_lj_grid_grad(int nx, int ny, int nz, const double* rvecs, const double* atoms, const int* types, int natom,
                      int ntypes, double sigma_f, double epsilon_f, double cutoff, int images, double limit,
-                     const double* w, double* work, double* grad, void* stream) {
-  if (!w || !work || !grad || (natom > 0 && (!atoms || !types)) || natom < 0 || ntypes < 1)
+                     const double* w, int nw, double* work, double* grad, void* stream) {
+  if (!w || !work || !grad || (natom > 0 && (!atoms || !types)) || natom < 0 || ntypes < 1 || nw < 1)
     return fail(CMM_E_ARG, "null argument");
   LJDev d;
   int rc = lj_setup(nx, ny, nz, rvecs, sigma_f, epsilon_f, cutoff, images, limit, d);
   if (rc) return rc;
   const dim3 grid = lj_grid_dims(nx, ny, nz);
   const long long nblocks = (long long)grid.x * grid.y * grid.z;
-  hipLaunchKernelGGL(k_lj_bwd, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, types, natom, ntypes, d, w, work);
+  hipLaunchKernelGGL(k_lj_bwd, grid, dim3(LJ_NT), 0, (hipStream_t)stream, atoms, types, natom, ntypes, d, w, nw,
+                     work);
   LAUNCHCHK("k_lj_bwd");
   hipLaunchKernelGGL(k_lj_bwd_reduce, dim3(2 * ntypes), dim3(LJ_NT), 0, (hipStream_t)stream, work, nblocks, grad);
   LAUNCHCHK("k_lj_bwd_reduce");

@@ -315,9 +315,10 @@ __device__ void lj_grad_type(const double* __restrict__ atoms, const int* __rest
   }
 }
 
-// partials: [2 ntypes][nblocks] (nblocks = gridDim.x*gridDim.y*gridDim.z)
+// partials: [2 ntypes][nblocks] (nblocks = gridDim.x*gridDim.y*gridDim.z);
+// w: nw weight fields of nx*ny*nz points
 __global__ void __launch_bounds__(LJ_NT) k_lj_bwd(const double* __restrict__ atoms, const int* __restrict__ types,
-                                                 int natom, int ntypes, LJDev d, const double* __restrict__ w,
+                                                 int natom, int ntypes, LJDev d, const double* __restrict__ w, int nw,
                                                  double* __restrict__ partials) {
   __shared__ LJShared S;
   __shared__ LJGradShared SG;
@@ -351,7 +352,14 @@ __global__ void __launch_bounds__(LJ_NT) k_lj_bwd(const double* __restrict__ ato
   const double reach = d.rc + hd + 1e-6;
   const double V = lj_sum(atoms, natom, d, p, c, reach, S);
   const bool clamped = d.limit > 0.0 && (!isfinite(V) || V > d.limit);
-  const double wi = (valid && !clamped) ? w[((long long)ix * d.ny + iy) * d.nz + iz] : 0.0;
+  // weight of the point: the sum of the nw weight fields (e.g. the adjoints of
+  // the state points of one framework), in field order
+  double wi = 0.0;
+  if (valid && !clamped) {
+    const long long idx = ((long long)ix * d.ny + iy) * d.nz + iz;
+    const long long np = (long long)d.nx * d.ny * d.nz;
+    for (int k = 0; k < nw; ++k) wi += w[k * np + idx];
+  }
   for (int t = 0; t < ntypes; ++t) {
     double gs, ge;
     lj_grad_type(atoms, types, natom, t, d, p, c, reach, SG, gs, ge);

@@ -170,3 +170,36 @@ def host_case5(k):
 
 def pressures_isotherm(n=32, lo=1e-2, hi=1e2):
     return np.logspace(np.log10(lo), np.log10(hi), n) * U.bar
+
+
+def pressures_uneven(n=8, lo=0.1, hi=50.0):
+    """The 8 pressures of each framework of the uneven batch, log-spaced."""
+    return np.logspace(np.log10(lo), np.log10(hi), n) * U.bar
+
+
+def framework_program(host, npoints, version='WBII', images=None):
+    """A Program for a synthetic host: united-atom methane at 300 K, FMT
+    ``version`` + mean-field WDA + LJ external potential (periodic images
+    summed when a cell edge is shorter than twice the cutoff, or as asked)."""
+    from .program import Program
+    from .system import NanoporousHost, SphericalLJGuest
+    from .eos import MFAEOS
+    prog = Program(write_files=False)
+    prog.set_system(NanoporousHost(host.name, host, host.ff),
+                    SphericalLJGuest('CH4_UA', GUEST_MASS, GUEST_SIGMA, GUEST_EPSILON))
+    prog.set_grid(npoints=list(npoints))
+    prog.init_free_energy(TEMPERATURE)
+    if version:
+        prog.fener.add_hard_sphere(version)
+    prog.fener.add_wdav(MFAEOS(GUEST_MASS, GUEST_SIGMA, GUEST_EPSILON))
+    if images is None:
+        images = bool(np.min(np.linalg.norm(host.rvecs, axis=1)) < 2 * CUTOFF)
+    prog.fener.add_external_potential(cutoff=CUTOFF, images=images)
+    return prog
+
+
+def chempots_at(pressures):
+    """mu(p) at 300 K from the MBWR equation of state (repo EOS)."""
+    from .eos import ModifiedBenedictWebbRubinEOS
+    eos = ModifiedBenedictWebbRubinEOS(GUEST_MASS, GUEST_SIGMA, GUEST_EPSILON)
+    return np.array([eos.calculate_mu(TEMPERATURE, p) for p in pressures])

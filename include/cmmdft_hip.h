@@ -213,13 +213,14 @@ int cmm_lj_points(const double* points, long long M, const double* rvecs, const 
 
 /* Backward of cmm_lj_grid: grad[2t] = sum_i w_i dV_i/dsigma_t,
  * grad[2t+1] = sum_i w_i dV_i/depsilon_t for host atom types t < ntypes
- * (types: device int[natom]); clamped points contribute nothing.  work is a
- * device buffer of cmm_lj_grad_workspace() doubles; grad is a device
+ * (types: device int[natom]), with w_i the sum of the nw weight fields
+ * w[nw][nx*ny*nz]; clamped points contribute nothing.  work is a device
+ * buffer of cmm_lj_grad_workspace() doubles; grad is a device
  * double[2 ntypes]. */
 int cmm_lj_grad_workspace(int nx, int ny, int nz, int ntypes, long long* ndoubles);
 int cmm_lj_grid_grad(int nx, int ny, int nz, const double* rvecs, const double* atoms, const int* types, int natom,
                      int ntypes, double sigma_f, double epsilon_f, double cutoff, int images, double limit,
-                     const double* w, double* work, double* grad, void* stream);
+                     const double* w, int nw, double* work, double* grad, void* stream);
 
 #ifdef __cplusplus
 }
