@@ -184,6 +184,52 @@ def adjoint_leg(shape, reps=3):
             'dN_depsilon': [float(v) for v in g['dN_depsilon']]}
 
 
+def uneven_leg(nframe=16, npress=8):
+    """The uneven batch: synthetic frameworks k = 0..nframe-1 (orthorhombic
+    edges 18-40 A, 0.25 A grids 72..160 per axis, 100-600 atoms of 4 LJ types,
+    periodic images), each at npress pressures 0.1..50 bar -- all solves
+    batched on one GPU (UnevenBatch, admission by device footprint), then the
+    batched adjoint of every framework: dN_i/dmu_i and the gradient of the
+    framework's total uptake with respect to its 8 host parameters.  External
+    potentials are generated before the timed region (vext_s)."""
+    import torch
+    from cmmdft_amd import workloads as W
+    from cmmdft_amd.batch import UnevenBatch
+    from cmmdft_amd.adjoint import BatchedAdjoint
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    items = []
+    for k in range(nframe):
+        host = W.host_case5(k)
+        prog = W.framework_program(host, host.notes['npoints'], images=True)
+        items.append((prog, W.chempots_at(W.pressures_uneven(npress))))
+    torch.cuda.synchronize()
+    vext_s = time.perf_counter() - t0
+    ub = UnevenBatch(items, m=5, damping=0.1, threshold=1e-8, nsteps=500)
+    t0 = time.perf_counter()
+    res = ub.solve()
+    torch.cuda.synchronize()
+    solve_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grads = [BatchedAdjoint(p, tol=1e-10).gradients(r['rho']) for (p, _), r in zip(items, res)]
+    torch.cuda.synchronize()
+    adj_s = time.perf_counter() - t0
+    nsolve = sum(len(r['N']) for r in res)
+    npts = [int(np.prod(p.grid.npoints)) for p, _ in items]
+    iters = np.concatenate([r['iters'] for r in res])
+    return {'workload': '%d synthetic frameworks (grids %s..%s points) x %d pressures 0.1..50 bar, WBII + MFA-WDA, '
+                        'periodic-image LJ' % (nframe, min(npts), max(npts), npress),
+            'solves': nsolve, 'solve_s': solve_s, 'points_per_s': nsolve / solve_s,
+            'all_converged': bool(all(np.all(r['converged']) for r in res)),
+            'resolved_points': int(sum(len(r['resolved']) for r in res)),
+            'iterations_mean': float(iters.mean()), 'iterations_max': int(iters.max()),
+            'peak_concurrent_frameworks': ub.stats['peak_concurrent'], 'peak_bytes': ub.stats['peak_bytes'],
+            'adjoint_s': adj_s, 'adjoint_vs_solve': adj_s / solve_s,
+            'cg_iterations_max': int(max(g['iters'] for g in grads)), 'vext_s': vext_s,
+            'total_uptake_grad_norm': [float(np.linalg.norm(np.concatenate([g['dNtot_dsigma'], g['dNtot_depsilon']])))
+                                       for g in grads][:4]}
+
+
 def passes_only(shape, reps=5):
     """Functional chain of one realistic density (ideal-gas start at 10 bar),
     each pass launched ``reps`` times, then a plain 256 MB device-to-device
@@ -227,6 +273,7 @@ def main():
     ap.add_argument('--grid', type=int, default=NSIDE)
     ap.add_argument('--cpu-sample', type=int, default=4, help='oracle iterations timed (0 skips the CPU leg)')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward-vs-adjoint timing')
+    ap.add_argument('--no-uneven', action='store_true', help='skip the uneven-batch (16 frameworks) leg')
     ap.add_argument('--passes-only', action='store_true',
                     help='short run for hardware counters: the functional passes at the bench grid, then a 256 MB '
                          'device copy as the byte-count reference')
@@ -319,6 +366,7 @@ def main():
     iter_gbps = b_iter * int(sum(int(v) for v in (biters if biters else iters))) / dt / 1e9
 
     adjoint = None if args.no_adjoint else adjoint_leg(shape)
+    uneven = None if args.no_uneven else uneven_leg()
 
     cpu = None
     if args.cpu_sample > 0:
@@ -353,6 +401,7 @@ def main():
         'vext_ms': vext_ms,
         'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
         'adjoint': adjoint,
+        'uneven_batch': uneven,
     }
     print(json.dumps(line))
 
