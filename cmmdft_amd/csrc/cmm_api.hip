@@ -126,7 +126,7 @@ RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->scal}; }
 
 // totals of the partials the previous kernel stored (see k_reduce_partials)
 int launch_reduce(cmm_plan* p, long long nblocks, int nq, unsigned maxmask, int slot) {
-  hipLaunchKernelGGL(k_reduce_partials, dim3(p->nb), dim3(NT), 0, p->stream, red_of(p), (int)nblocks, nq, maxmask,
+  hipLaunchKernelGGL(k_reduce_partials, dim3(p->nb, nq), dim3(NT), 0, p->stream, red_of(p), (int)nblocks, maxmask,
                      slot, p->active);
   hipError_t e_ = hipGetLastError();
   if (e_ != hipSuccess) return fail(CMM_E_HIP, std::string("k_reduce_partials: ") + hipGetErrorString(e_));
@@ -496,12 +496,12 @@ static int forward_chain(cmm_plan* p) {
   return launch_ypass(p, p->F, 6, 1);
 }
 
-static int launch_zphi(cmm_plan* p, int store) {
+static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
   dim3 grid((unsigned)zblocks(p, p->pphi), p->nb);
   hipLaunchKernelGGL(k_zphi, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g,
                      p->fp, ztile(p, p->pphi, 6), store, p->active, red_of(p));
   LAUNCHCHK("k_zphi");
-  return launch_reduce(p, grid.x, 3, 1u << 2, S_PHI);
+  return reduce ? launch_reduce(p, grid.x, 3, 1u << 2, S_PHI) : CMM_OK;
 }
 
 // back-convolution of the 6 derivative fields in F: inverse y, the x pass
@@ -545,7 +545,7 @@ int cmm_launch_pass(cmm_plan* p, int which) {
   if (!p) return fail(CMM_E_ARG, "null plan");
   if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
   switch (which) {
-    case 0: return launch_zphi(p, 1);
+    case 0: return launch_zphi(p, 1, false);
     case 1: return launch_xweights(p);
     case 2: return launch_xback(p);
     case 3: return launch_ypass(p, p->F, 6, -1);

@@ -836,31 +836,36 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
 // Per-instance totals of the block partials a producer kernel stored
 // (grid_reduce_store): out[q] = sum (or max, bit q of maxmask) over the
 // nblocks partials of quantity q, in a fixed order (lane-strided, then the
-// wavefront tree, then the waves in order).  One block per instance.
-__global__ void __launch_bounds__(NT) k_reduce_partials(RedDev red, int nblocks, int nq, unsigned maxmask, int slot,
+// wavefront tree, then the waves in order).  One block per (instance,
+// quantity); each thread issues its loads in groups of 8 before summing them
+// in order, so the latency of one load is paid per group, not per partial.
+__global__ void __launch_bounds__(NT) k_reduce_partials(RedDev red, int nblocks, unsigned maxmask, int slot,
                                                        const int* __restrict__ active) {
   __shared__ double sh[NT / 64];
-  const int inst = blockIdx.x;
+  const int inst = blockIdx.x, q = blockIdx.y;
   if (active && !active[inst]) return;
-  const double* part = red.partials + (long long)inst * red.pstride;
-  double* out = red.scal + (long long)inst * NSCAL + slot;
+  const double* part = red.partials + (long long)inst * red.pstride + (long long)q * nblocks;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-  for (int q = 0; q < nq; ++q) {
-    const bool mx = (maxmask >> q) & 1u;
-    double a = mx ? -INFINITY : 0.0;
-    for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
-      const double x = part[(long long)q * nblocks + i];
-      a = mx ? fmax(a, x) : a + x;
+  const bool mx = (maxmask >> q) & 1u;
+  const double id = mx ? -INFINITY : 0.0;
+  double a = id;
+  for (int i0 = threadIdx.x; i0 < nblocks; i0 += 8 * blockDim.x) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * blockDim.x;
+      x[u] = (i < nblocks) ? part[i] : id;
     }
-    const double r = mx ? wave_max(a) : wave_sum(a);
-    if (lane == 0) sh[wid] = r;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = sh[0];
-      for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[w]) : t + sh[w];
-      out[q] = t;
-    }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a = mx ? fmax(a, x[u]) : a + x[u];
+  }
+  const double r = mx ? wave_max(a) : wave_sum(a);
+  if (lane == 0) sh[wid] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = sh[0];
+    for (int w = 1; w < nw; ++w) t = mx ? fmax(t, sh[w]) : t + sh[w];
+    red.scal[(long long)inst * NSCAL + slot + q] = t;
   }
 }
 
@@ -872,51 +877,72 @@ __global__ void __launch_bounds__(NT) k_reduce_partials(RedDev red, int nblocks,
 // minimiser).  One wavefront per instance.  Also updates the Gram matrix ring
 // from this step's row and the adaptive damping.
 // ---------------------------------------------------------------------------
+// Minimiser of a^T G a on the face {a_i = 0 for i not in S, sum a = 1} for the
+// support S = mask: a_S = G_S^-1 1 / (1^T G_S^-1 1).  Solved as the fixed-size
+// system A y = P 1 with A = P G P + (I - P) (P = diag(mask)), so every loop
+// has a compile-time bound and the matrix lives in registers: Cholesky
+// factorisation (G_S is a Gram matrix), then the two triangular solves.
+// Returns false if G_S is numerically singular (pivot <= 1e-13 of the unit
+// diagonal scale) or the minimiser leaves the simplex.
 __device__ bool kkt_solve(const double* Gm, int mk, unsigned mask, double* alpha) {
-  // unknowns: alpha_S (|S| of them) and lambda; rows: G_S a - lambda 1 = 0, 1^T a = 1
-  int idx[CMM_MMAX];
-  int s = 0;
-  for (int i = 0; i < mk; ++i)
-    if (mask & (1u << i)) idx[s++] = i;
-  const int dim = s + 1;
-  double M[CMM_MMAX + 1][CMM_MMAX + 2];
-  for (int r = 0; r < s; ++r) {
-    for (int c = 0; c < s; ++c) M[r][c] = Gm[idx[r] * CMM_MMAX + idx[c]];
-    M[r][s] = -1.0;
-    M[r][s + 1] = 0.0;
-  }
-  for (int c = 0; c < s; ++c) M[s][c] = 1.0;
-  M[s][s] = 0.0;
-  M[s][s + 1] = 1.0;
-  for (int c = 0; c < dim; ++c) {
-    int piv = c;
-    double best = fabs(M[c][c]);
-    for (int r = c + 1; r < dim; ++r)
-      if (fabs(M[r][c]) > best) { best = fabs(M[r][c]); piv = r; }
-    if (best < 1e-13) return false;
-    if (piv != c)
-      for (int k = 0; k <= dim; ++k) { double t = M[c][k]; M[c][k] = M[piv][k]; M[piv][k] = t; }
-    for (int r = c + 1; r < dim; ++r) {
-      const double f = M[r][c] / M[c][c];
-      for (int k = c; k <= dim; ++k) M[r][k] -= f * M[c][k];
+  double Lm[CMM_MMAX][CMM_MMAX];
+#pragma unroll
+  for (int i = 0; i < CMM_MMAX; ++i) {
+#pragma unroll
+    for (int j = 0; j < CMM_MMAX; ++j) {
+      const bool in = (i < mk) && (j < mk) && ((mask >> i) & 1u) && ((mask >> j) & 1u);
+      Lm[i][j] = in ? Gm[i * CMM_MMAX + j] : (i == j ? 1.0 : 0.0);
     }
   }
-  double x[CMM_MMAX + 1];
-  for (int r = dim - 1; r >= 0; --r) {
-    double t = M[r][dim];
-    for (int k = r + 1; k < dim; ++k) t -= M[r][k] * x[k];
-    x[r] = t / M[r][r];
+#pragma unroll
+  for (int j = 0; j < CMM_MMAX; ++j) {
+    double d = Lm[j][j];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= Lm[j][k] * Lm[j][k];
+    if (!(d > 1e-13)) return false;
+    const double ljj = sqrt(d);
+    Lm[j][j] = ljj;
+    const double inv = 1.0 / ljj;
+#pragma unroll
+    for (int i = j + 1; i < CMM_MMAX; ++i) {
+      double t = Lm[i][j];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= Lm[i][k] * Lm[j][k];
+      Lm[i][j] = t * inv;
+    }
   }
-  for (int i = 0; i < mk; ++i) alpha[i] = 0.0;
+  double y[CMM_MMAX];
+#pragma unroll
+  for (int i = 0; i < CMM_MMAX; ++i) {          // L z = P 1
+    double t = (i < mk && ((mask >> i) & 1u)) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < i; ++k) t -= Lm[i][k] * y[k];
+    y[i] = t / Lm[i][i];
+  }
+#pragma unroll
+  for (int i = CMM_MMAX - 1; i >= 0; --i) {     // L^T y = z
+    double t = y[i];
+#pragma unroll
+    for (int k = i + 1; k < CMM_MMAX; ++k) t -= Lm[k][i] * y[k];
+    y[i] = t / Lm[i][i];
+  }
   double sum = 0.0;
-  for (int r = 0; r < s; ++r) {
-    if (x[r] < -1e-12) return false;
-    const double a = fmax(x[r], 0.0);
-    alpha[idx[r]] = a;
-    sum += a;
-  }
+#pragma unroll
+  for (int i = 0; i < CMM_MMAX; ++i) sum += (i < mk && ((mask >> i) & 1u)) ? y[i] : 0.0;
   if (!(sum > 0.0)) return false;
-  for (int i = 0; i < mk; ++i) alpha[i] /= sum;
+  const double isum = 1.0 / sum;
+#pragma unroll
+  for (int i = 0; i < CMM_MMAX; ++i) {
+    const bool in = i < mk && ((mask >> i) & 1u);
+    const double a = in ? y[i] * isum : 0.0;
+    if (a < -1e-12) return false;
+    alpha[i] = fmax(a, 0.0);
+  }
+  double s2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < CMM_MMAX; ++i) s2 += alpha[i];
+#pragma unroll
+  for (int i = 0; i < CMM_MMAX; ++i) alpha[i] /= s2;
   return true;
 }
 
@@ -955,9 +981,11 @@ __global__ void __launch_bounds__(64) k_anderson_coef(AndDev ad, const int* __re
     for (unsigned m = 1 + lane; m <= nsub; m += 64) {
       if (!kkt_solve(Gm, mk, m, alpha)) continue;
       double f = 0.0;
-      for (int i = 0; i < mk; ++i) {
+#pragma unroll
+      for (int i = 0; i < CMM_MMAX; ++i) {
         double t = 0.0;
-        for (int j = 0; j < mk; ++j) t += Gm[i * CMM_MMAX + j] * alpha[j];
+#pragma unroll
+        for (int j = 0; j < CMM_MMAX; ++j) t += (i < mk && j < mk) ? Gm[i * CMM_MMAX + j] * alpha[j] : 0.0;
         f += alpha[i] * t;
       }
       if (f < best || (f == best && m < bestmask)) { best = f; bestmask = m; }

@@ -129,8 +129,9 @@ int cmm_value(cmm_plan* plan);
 int cmm_weighted_densities(cmm_plan* plan, double* out);
 
 /* launch one pass of the functional chain on the current workspace
- * (measurement aid): 0 functional z pass, 1 forward x pass with weights,
- * 2 back-convolution x pass, 3 y pass over the 6 fields, 4 y pass over rho */
+ * (measurement aid, kernel only, no reduction of its partials): 0 functional
+ * z pass, 1 forward x pass with weights, 2 back-convolution x pass, 3 y pass
+ * over the 6 fields, 4 y pass over rho */
 int cmm_launch_pass(cmm_plan* plan, int which);
 
 /* final passes after cmm_derive; rho is the density the derivative belongs to */
