@@ -7,10 +7,11 @@ The search space per grid shape:
   * the tiles: x-pass columns tb_x, y-pass columns tb_y, z-pass pencils p_z
     (density producer and final passes), and p_phi (the 6-field functional
     z pass), within the LDS and register-staging limits the library checks.
-The cost is the device time of the functional chain of one Euler-Lagrange
-iteration (density transform, weighted densities, pointwise derivatives,
-back-convolution, final pass), measured with HIP events on the plan's stream;
-the search is a coordinate descent over the seven parameters.
+The cost is the device time of one steady-state batched Anderson iteration
+(residual and Gram row, coefficients, mixing with the new density's
+transform, weighted densities, pointwise derivatives, back-convolution),
+measured with HIP events on the plan's stream; the search is a coordinate
+descent over the seven parameters.
 
 Results are kept per grid shape (not per batch size) in ``tuned_gfx950.json``
 beside this file, and every plan of that shape picks them up when it is
@@ -98,20 +99,45 @@ def radix_candidates(n, extra_stages=1, max_stages=8):
     return sorted(seqs, key=lambda s: (len(s), s))
 
 
+class _IterationBench:
+    """One steady-state batched Anderson iteration (m = 5) on the plan:
+    residual + Gram row, coefficients, mixing with the new density's z
+    transform, and the functional chain -- the launch sequence of the batched
+    solver, on history rings filled with the given density."""
+
+    def __init__(self, plan, rho, m=5):
+        import torch
+        self.plan, self.m = plan, m
+        nb, n = rho.shape
+        self.rr = rho.reshape(1, nb, n).repeat(m + 1, 1, 1).contiguous()
+        self.rs = torch.zeros((m, nb, n), dtype=torch.float64, device=rho.device)
+        plan.set_instances(np.full(nb, 1e-3), np.zeros(nb))
+        plan.anderson_init(np.full(nb, 0.1), np.ones(nb))
+
+    def run(self, s):
+        m, p = self.m, self.plan
+        cur, rslot, new = (s - 1) % (m + 1), (s - 1) % m, s % (m + 1)
+        ts = range(s - m, s)
+        order_r = [t % m for t in ts]
+        order_rho = [t % (m + 1) for t in ts]
+        p.finish_anderson(self.rr[cur], None, False, self.rs, m, rslot, None, 1.0, 1.0)
+        p.anderson_coef(m, order_r, rslot, (s - 2) % m, 1, 1.5, 0.5, 0.01, 0.8)
+        p.anderson_mix(self.rr, order_rho, self.rs, order_r, self.rr[cur], self.rr[new], None, False, 1.0)
+        p.derive()
+
+
 def _chain_time(plan, rho, reps):
     import torch
-    out = torch.empty_like(rho)
+    it = _IterationBench(plan, rho)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     with torch.cuda.stream(plan.stream):
         plan.seed(rho, None, False, 1.0)
         plan.derive()
-        plan.finish_c1(rho, None, False, out, 1.0)
+        it.run(6)
         ev0.record(plan.stream)
-        for _ in range(reps):
-            plan.seed(rho, None, False, 1.0)
-            plan.derive()
-            plan.finish_c1(rho, None, False, out, 1.0)
+        for k in range(reps):
+            it.run(7 + k)
         ev1.record(plan.stream)
     ev1.synchronize()
     return ev0.elapsed_time(ev1) / reps

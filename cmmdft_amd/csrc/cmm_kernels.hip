@@ -799,10 +799,13 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
       const double r0 = rcur[idx];
       if (MODE == MODE_MIX) {
         double s = 0.0;
-        for (int j = 0; j < el.mk; ++j) {
-          const int so = el.order[j];
-          s += coef[j] * el.rhohist[(long long)el.order_rho[j] * el.hstride2 + inst * el.fstride + idx];
-          s += coef[CMM_MMAX + j] * el.rhist[(long long)so * el.hstride + inst * el.fstride + idx];
+#pragma unroll
+        for (int j = 0; j < CMM_MMAX; ++j) {     // compile-time bound: all history loads in flight together
+          if (j < el.mk) {
+            const int so = el.order[j];
+            s += coef[j] * el.rhohist[(long long)el.order_rho[j] * el.hstride2 + inst * el.fstride + idx];
+            s += coef[CMM_MMAX + j] * el.rhist[(long long)so * el.hstride + inst * el.fstride + idx];
+          }
         }
         rn = (s < 1e-12) ? 1e-30 : s;
       } else if (MODE == MODE_TRIAL) {
