@@ -74,7 +74,7 @@ def measured_traffic(kernel, n):
         return None
     for line in open(fn):
         w = line.split()
-        if len(w) == 4 and w[0] == 'traffic_bytes_per_launch' and w[1] == kernel and int(w[2]) == n:
+        if len(w) == 4 and w[0] == 'traffic_bytes_per_launch' and w[1] == kernel.replace(' ', '_') and int(w[2]) == n:
             return int(w[3])
     return None
 
