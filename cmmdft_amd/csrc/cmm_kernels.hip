@@ -31,14 +31,32 @@ __device__ __forceinline__ double& z_val(double2* lds, const ZTile& t, int f, in
   return reinterpret_cast<double*>(lds)[2 * (e * t.ES + f * t.B + (pl >> 1)) + (pl & 1)];
 }
 
+// Loaders issue the global loads of LU elements per thread before any of
+// their LDS stores, with addresses clamped into range and out-of-range values
+// replaced afterwards: no branch separates the loads, so they are all in
+// flight together.
+constexpr int LU = 4;
+
 __device__ __forceinline__ void z_load_real(double2* lds, const double* src, long long p0, const ZTile& t, long long npen,
                                             int n, int f) {
   const int tot = t.P * n;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
-    const long long p = p0 + pl;
-    z_val(lds, t, f, pl, e) = (p < npen) ? src[p * n + e] : 0.0;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
+    double v[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, tot - 1);
+      const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
+      const long long p = min(p0 + pl, npen - 1);
+      v[u] = src[p * n + e];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < tot) {
+        const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
+        z_val(lds, t, f, pl, e) = (p0 + pl < npen) ? v[u] : 0.0;
+      }
+    }
   }
 }
 
@@ -66,38 +84,67 @@ __device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, l
 __device__ __forceinline__ void z_load_half(double2* lds, const double2* src, long long p0, const ZTile& t, long long npen,
                                             int n, int nzh, int f) {
   const int tot = t.B * nzh;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
-    const long long pa = p0 + 2 * q, pb = pa + 1;
-    double2 A = (pa < npen) ? src[pa * nzh + k] : make_double2(0.0, 0.0);
-    double2 B = (pb < npen) ? src[pb * nzh + k] : make_double2(0.0, 0.0);
-    const bool selfconj = (k == 0) || (2 * k == n);
-    if (selfconj) { A.y = 0.0; B.y = 0.0; }
-    lds[k * t.ES + f * t.B + q] = make_double2(A.x - B.y, A.y + B.x);
-    if (!selfconj) lds[(n - k) * t.ES + f * t.B + q] = make_double2(A.x + B.y, B.x - A.y);
+  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
+    double2 A[LU], B[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, tot - 1);
+      const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
+      const long long pa = p0 + 2 * q;
+      A[u] = src[min(pa, npen - 1) * nzh + k];
+      B[u] = src[min(pa + 1, npen - 1) * nzh + k];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < tot) {
+        const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
+        const long long pa = p0 + 2 * q;
+        double2 a = (pa < npen) ? A[u] : make_double2(0.0, 0.0);
+        double2 b = (pa + 1 < npen) ? B[u] : make_double2(0.0, 0.0);
+        const bool selfconj = (k == 0) || (2 * k == n);
+        if (selfconj) { a.y = 0.0; b.y = 0.0; }
+        lds[k * t.ES + f * t.B + q] = make_double2(a.x - b.y, a.y + b.x);
+        if (!selfconj) lds[(n - k) * t.ES + f * t.B + q] = make_double2(a.x + b.y, b.x - a.y);
+      }
+    }
   }
 }
 
-// nf half-spectrum fields (field stride fs) into the tile in one loop, so
-// every thread has several independent loads in flight
+// nf half-spectrum fields (field stride fs) into the tile in one loop
 __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* src, long long fs, int nf, long long p0,
                                                   const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn) {
   const int per = t.B * nzh;
   const int tot = nf * per;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int f = (int)fdiv((unsigned)i, fBn);
-    const int r = i - f * per;
-    const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
-    const long long pa = p0 + 2 * q, pb = pa + 1;
-    const double2* sf = src + f * fs;
-    double2 A = (pa < npen) ? sf[pa * nzh + k] : make_double2(0.0, 0.0);
-    double2 B = (pb < npen) ? sf[pb * nzh + k] : make_double2(0.0, 0.0);
-    const bool selfconj = (k == 0) || (2 * k == n);
-    if (selfconj) { A.y = 0.0; B.y = 0.0; }
-    lds[k * t.ES + f * t.B + q] = make_double2(A.x - B.y, A.y + B.x);
-    if (!selfconj) lds[(n - k) * t.ES + f * t.B + q] = make_double2(A.x + B.y, B.x - A.y);
+  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
+    double2 A[LU], B[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, tot - 1);
+      const int f = (int)fdiv((unsigned)i, fBn);
+      const int r = i - f * per;
+      const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
+      const long long pa = p0 + 2 * q;
+      const double2* sf = src + f * fs;
+      A[u] = sf[min(pa, npen - 1) * nzh + k];
+      B[u] = sf[min(pa + 1, npen - 1) * nzh + k];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < tot) {
+        const int f = (int)fdiv((unsigned)i, fBn);
+        const int r = i - f * per;
+        const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
+        const long long pa = p0 + 2 * q;
+        double2 a = (pa < npen) ? A[u] : make_double2(0.0, 0.0);
+        double2 b = (pa + 1 < npen) ? B[u] : make_double2(0.0, 0.0);
+        const bool selfconj = (k == 0) || (2 * k == n);
+        if (selfconj) { a.y = 0.0; b.y = 0.0; }
+        lds[k * t.ES + f * t.B + q] = make_double2(a.x - b.y, a.y + b.x);
+        if (!selfconj) lds[(n - k) * t.ES + f * t.B + q] = make_double2(a.x + b.y, b.x - a.y);
+      }
+    }
   }
 }
 
@@ -195,10 +242,22 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   const int tbv = min(TB, g.nzh - k0);
   double2* base = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh + k0;
   const int tot = g.ny * TB;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
-    sm[i] = (b < tbv) ? base[(long long)y * g.nzh + b] : make_double2(0.0, 0.0);
+  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
+    double2 v[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, tot - 1);
+      const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
+      v[u] = base[(long long)y * g.nzh + min(b, tbv - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < tot) {
+        const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
+        sm[i] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+      }
+    }
   }
   __syncthreads();
   lds_fft<SGN, true, VPT>(sm, TB, fTB, 1, TB, g.fy, s_tw);
@@ -329,9 +388,22 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = rz + inst * rzs + b0;
-  for (int i = threadIdx.x; i < T; i += blockDim.x) {
-    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-    A[i] = (b < tbv) ? src[(long long)x * nb + b] : make_double2(0.0, 0.0);
+  for (int i0 = threadIdx.x; i0 < T; i0 += LU * blockDim.x) {
+    double2 v[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, T - 1);
+      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+      v[u] = src[(long long)x * nb + min(b, tbv - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < T) {
+        const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+        A[i] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+      }
+    }
   }
   __syncthreads();
   lds_fft<-1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
@@ -378,10 +450,24 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = F + inst * Fs + b0;
-  for (int i = threadIdx.x; i < 6 * T; i += blockDim.x) {
-    const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
-    const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
-    B[x * S6 + f * TB + b] = (b < tbv) ? src[f * Ffs + (long long)x * nb + b] : make_double2(0.0, 0.0);
+  for (int i0 = threadIdx.x; i0 < 6 * T; i0 += LU * blockDim.x) {
+    double2 v[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, 6 * T - 1);
+      const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
+      const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
+      v[u] = src[f * Ffs + (long long)x * nb + min(b, tbv - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < 6 * T) {
+        const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
+        const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
+        B[x * S6 + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+      }
+    }
   }
   __syncthreads();
   lds_fft<-1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
