@@ -796,44 +796,79 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
   double* rnew_hist = nullptr;
   if (MODE == MODE_ANDRES) rnew_hist = el.rhist + (long long)el.slot_new * el.hstride + inst * el.fstride;
   const int tot = t.P * n;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
-    const long long p = p0 + pl;
-    if (p >= npen) continue;
-    const long long idx = p * n + e;
-    double& cv = z_val(sm, t, 0, pl, e);
-    const double vv = V ? V[idx] : 0.0;
-    const double C1 = cv + vv;
-    const double r = rho[idx];
-    // ideal-gas and external terms of the grand potential of rho
-    const double rc = (r < 1e-12) ? 1e-30 : r;
-    acc[0] += r;
-    acc[1] += rc * (log(el.lam3 * rc) - 1.0);
-    acc[2] += r * vv;
-    if (MODE == MODE_C1) {
-      el.c1out[inst * el.fstride + idx] = C1;
-    } else {
-      if (el.c1out) el.c1out[inst * el.fstride + idx] = C1;
-      const double G = el.beta * exp(-el.beta * C1) * ip.fug;
-      if (MODE == MODE_PICARD) {
-        double rn = (1.0 - ip.alpha) * r + ip.alpha * G;
-        if (rn < el.zero_thr) rn = 0.0;
-        el.rho_out[inst * el.fstride + idx] = rn;
-        cv = rn;
-        const double d = fabs(rn - r);
-        acc[3] += rn;
-        acc[4] += d;
-        acc[5] = fmax(acc[5], d);
-        acc[6] += isfinite(rn) ? 0.0 : 1.0;
-      } else {
-        const double rr = G - r;
-        rnew_hist[idx] = rr;
-        acc[3] += isfinite(rr) ? 0.0 : 1.0;
+  // EC elements per thread at a time, loads grouped (addresses clamped into
+  // range); the Gram row walks the residual slots in the outer loop so each
+  // slot's EC loads are in flight together -- per quantity the sums keep the
+  // element order
+  constexpr int EC = 8;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += EC * blockDim.x) {
+    long long idx[EC];
+    bool ok[EC];
+    double vv[EC], r[EC], rr[EC];
 #pragma unroll
-        for (int j = 0; j < CMM_MMAX; ++j) {
-          if (j < el.m) {
-            const double other = (j == el.slot_new) ? rr : el.rhist[(long long)j * el.hstride + inst * el.fstride + idx];
-            acc[4 + j] += rr * other;
+    for (int u = 0; u < EC; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      const int ic = min(i, tot - 1);
+      const int pl = (int)fdiv((unsigned)ic, t.fn), e = ic - pl * n;
+      ok[u] = (i < tot) && (p0 + pl < npen);
+      idx[u] = min(p0 + pl, npen - 1) * n + e;
+    }
+#pragma unroll
+    for (int u = 0; u < EC; ++u) {
+      vv[u] = V ? V[idx[u]] : 0.0;
+      r[u] = rho[idx[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < EC; ++u) {
+      rr[u] = 0.0;
+      if (!ok[u]) continue;
+      const int i = i0 + u * (int)blockDim.x;
+      const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
+      double& cv = z_val(sm, t, 0, pl, e);
+      const double C1 = cv + vv[u];
+      // ideal-gas and external terms of the grand potential of rho
+      const double rc = (r[u] < 1e-12) ? 1e-30 : r[u];
+      acc[0] += r[u];
+      acc[1] += rc * (log(el.lam3 * rc) - 1.0);
+      acc[2] += r[u] * vv[u];
+      if (MODE == MODE_C1) {
+        el.c1out[inst * el.fstride + idx[u]] = C1;
+      } else {
+        if (el.c1out) el.c1out[inst * el.fstride + idx[u]] = C1;
+        const double G = el.beta * exp(-el.beta * C1) * ip.fug;
+        if (MODE == MODE_PICARD) {
+          double rn = (1.0 - ip.alpha) * r[u] + ip.alpha * G;
+          if (rn < el.zero_thr) rn = 0.0;
+          el.rho_out[inst * el.fstride + idx[u]] = rn;
+          cv = rn;
+          const double d = fabs(rn - r[u]);
+          acc[3] += rn;
+          acc[4] += d;
+          acc[5] = fmax(acc[5], d);
+          acc[6] += isfinite(rn) ? 0.0 : 1.0;
+        } else {
+          rr[u] = G - r[u];
+          rnew_hist[idx[u]] = rr[u];
+          acc[3] += isfinite(rr[u]) ? 0.0 : 1.0;
+        }
+      }
+    }
+    if (MODE == MODE_ANDRES) {
+#pragma unroll
+      for (int j = 0; j < CMM_MMAX; ++j) {
+        if (j < el.m) {
+          if (j == el.slot_new) {
+#pragma unroll
+            for (int u = 0; u < EC; ++u)
+              if (ok[u]) acc[4 + j] += rr[u] * rr[u];
+          } else {
+            const double* hs = el.rhist + (long long)j * el.hstride + inst * el.fstride;
+            double o[EC];
+#pragma unroll
+            for (int u = 0; u < EC; ++u) o[u] = hs[idx[u]];
+#pragma unroll
+            for (int u = 0; u < EC; ++u)
+              if (ok[u]) acc[4 + j] += rr[u] * o[u];
           }
         }
       }
@@ -876,43 +911,78 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
   const double* coef = el.coef + inst * 2 * CMM_MMAX;
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
   const int tot = t.P * n;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
-    const long long p = p0 + pl;
-    double rn = 0.0;
-    if (p < npen) {
-      const long long idx = p * n + e;
-      const double r0 = rcur[idx];
-      if (MODE == MODE_MIX) {
-        double s = 0.0;
+  // EC elements per thread at a time; the history fields are walked in the
+  // outer loop so that the EC loads of one field are in flight together (the
+  // per-element sums keep the field order)
+  constexpr int EC = 8;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += EC * blockDim.x) {
+    long long idx[EC];
+    double r0[EC], rn[EC];
 #pragma unroll
-        for (int j = 0; j < CMM_MMAX; ++j) {     // compile-time bound: all history loads in flight together
-          if (j < el.mk) {
-            const int so = el.order[j];
-            s += coef[j] * el.rhohist[(long long)el.order_rho[j] * el.hstride2 + inst * el.fstride + idx];
-            s += coef[CMM_MMAX + j] * el.rhist[(long long)so * el.hstride + inst * el.fstride + idx];
-          }
-        }
-        rn = (s < 1e-12) ? 1e-30 : s;
-      } else if (MODE == MODE_TRIAL) {
-        const double rr = el.rhist[(long long)el.slot_new * el.hstride + inst * el.fstride + idx];
-        rn = r0 + el.inst[inst].alpha * rr;
-        if (el.clip_trial && rn < 1e-12) rn = 1e-30;
-      } else {
-        rn = r0;
-      }
-      if (MODE != MODE_SEED) rout[idx] = rn;
-      const double vv = V ? V[idx] : 0.0;
-      const double rc = (rn < 1e-12) ? 1e-30 : rn;
-      const double d = fabs(rn - r0);
-      acc[0] += rn;
-      acc[1] += d;
-      acc[2] = fmax(acc[2], d);
-      acc[3] += isfinite(rn) ? 0.0 : 1.0;
-      acc[4] += rc * (log(el.lam3 * rc) - 1.0);
-      acc[5] += rn * vv;
+    for (int u = 0; u < EC; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, tot - 1);
+      const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
+      idx[u] = min(p0 + pl, npen - 1) * n + e;
     }
-    z_val(sm, t, 0, pl, e) = rn;
+#pragma unroll
+    for (int u = 0; u < EC; ++u) r0[u] = rcur[idx[u]];
+    if (MODE == MODE_MIX) {
+#pragma unroll
+      for (int u = 0; u < EC; ++u) rn[u] = 0.0;
+      for (int j = 0; j < el.mk; ++j) {
+        const double a = coef[j], b = coef[CMM_MMAX + j];
+        const double* hr = el.rhohist + (long long)el.order_rho[j] * el.hstride2 + inst * el.fstride;
+        const double* hs = el.rhist + (long long)el.order[j] * el.hstride + inst * el.fstride;
+        double x[EC], y[EC];
+#pragma unroll
+        for (int u = 0; u < EC; ++u) {
+          x[u] = hr[idx[u]];
+          y[u] = hs[idx[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < EC; ++u) {
+          rn[u] += a * x[u];
+          rn[u] += b * y[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < EC; ++u) rn[u] = (rn[u] < 1e-12) ? 1e-30 : rn[u];
+    } else if (MODE == MODE_TRIAL) {
+      const double* hs = el.rhist + (long long)el.slot_new * el.hstride + inst * el.fstride;
+      const double al = el.inst[inst].alpha;
+#pragma unroll
+      for (int u = 0; u < EC; ++u) {
+        rn[u] = r0[u] + al * hs[idx[u]];
+        if (el.clip_trial && rn[u] < 1e-12) rn[u] = 1e-30;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < EC; ++u) rn[u] = r0[u];
+    }
+    double vv[EC];
+#pragma unroll
+    for (int u = 0; u < EC; ++u) vv[u] = V ? V[idx[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < EC; ++u) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < tot) {
+        const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
+        double v = 0.0;
+        if (p0 + pl < npen) {
+          v = rn[u];
+          if (MODE != MODE_SEED) rout[idx[u]] = v;
+          const double rc = (v < 1e-12) ? 1e-30 : v;
+          const double d = fabs(v - r0[u]);
+          acc[0] += v;
+          acc[1] += d;
+          acc[2] = fmax(acc[2], d);
+          acc[3] += isfinite(v) ? 0.0 : 1.0;
+          acc[4] += rc * (log(el.lam3 * rc) - 1.0);
+          acc[5] += v * vv[u];
+        }
+        z_val(sm, t, 0, pl, e) = v;
+      }
+    }
   }
   __syncthreads();
   z_fft<-1>(sm, t, g.fz, s_tw);
