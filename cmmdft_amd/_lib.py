@@ -29,7 +29,8 @@ S_ADJ_PQ, S_ADJ_RR, S_ADJ_LAM = 24, 25, 27
 
 class CmmFunctional(C.Structure):
     _fields_ = [
-        ('hs_base', C.c_int), ('hs_antisym', C.c_int), ('use_wda', C.c_int), ('n_eos', C.c_int),
+        ('hs_base', C.c_int), ('hs_antisym', C.c_int), ('hs_tensor', C.c_int), ('use_wda', C.c_int),
+        ('n_eos', C.c_int),
         ('eos_kind', C.c_int * 4), ('eos_factor', C.c_double * 4), ('eos_par', (C.c_double * 16) * 4),
         ('R', C.c_double), ('D', C.c_double), ('kT', C.c_double),
     ]
@@ -178,10 +179,16 @@ class Plan:
 
     @property
     def info(self):
-        out = np.zeros(10, dtype=np.int64)
-        check(self.lib.cmm_plan_get_info(self.h, out.ctypes.data_as(C.c_void_p), 10))
-        return dict(zip(['nx', 'ny', 'nz', 'nbatch', 'tb_x', 'tb_y', 'p_z', 'p_phi', 'workspace_bytes', 'nzh'],
+        out = np.zeros(11, dtype=np.int64)
+        check(self.lib.cmm_plan_get_info(self.h, out.ctypes.data_as(C.c_void_p), 11))
+        return dict(zip(['nx', 'ny', 'nz', 'nbatch', 'tb_x', 'tb_y', 'p_z', 'p_phi', 'workspace_bytes', 'nzh', 'nf'],
                         [int(v) for v in out]))
+
+    @property
+    def nfields(self):
+        """Weighted-density fields of the functional set: 6, or 12 for the
+        tensor variant."""
+        return self.info['nf']
 
     def set_tiles(self, tb_x=0, tb_y=0, p_z=0, p_phi=0):
         check(self.lib.cmm_plan_set_tiles(self.h, tb_x, tb_y, p_z, p_phi))
@@ -230,6 +237,10 @@ class Plan:
         check(self.lib.cmm_launch_pass(self.h, int(which)))
 
     def weighted_densities(self, out):
+        """Weighted densities of the seeded density into out
+        (nbatch x nfields x npts values)."""
+        if out.numel() != self.nbatch * self.nfields * int(np.prod(self.npoints)):
+            raise ValueError('weighted_densities: out must hold nbatch x %d x npts values' % self.nfields)
         check(self.lib.cmm_weighted_densities(self.h, _ptr(out)))
 
     def finish_c1(self, rho, vext, vext_batched, c1, lam3=1.0):

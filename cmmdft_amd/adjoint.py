@@ -48,7 +48,7 @@ class AdjointSolver:
     def _prepare(self, eng, rho):
         torch = torch_mod()
         r = as_device(rho).reshape(-1).contiguous()
-        nfld = torch.empty(6 * self.grid.npts, dtype=torch.float64, device=r.device)
+        nfld = torch.empty(eng.plan.nfields * self.grid.npts, dtype=torch.float64, device=r.device)
         eng.plan.seed(r, None, False, eng.lam3)
         eng.plan.weighted_densities(nfld)
         return r, nfld
@@ -170,7 +170,7 @@ class BatchedAdjoint:
         lam3 = self.fener.wavelength ** 3
         plan = Plan(self.grid.npoints, self.grid.cell.rvecs, nbatch=B)
         plan.set_functional(build_functional(hs=hs, wda=wda, kT=1.0 / beta))
-        nfld = torch.empty((B, 6, n), dtype=torch.float64, device=r.device)
+        nfld = torch.empty((B, plan.nfields, n), dtype=torch.float64, device=r.device)
         plan.seed(r, None, False, lam3)
         plan.weighted_densities(nfld)
         y, res, p, q = (torch.empty((B, n), dtype=torch.float64, device=r.device) for _ in range(4))

@@ -46,13 +46,19 @@ __all__ = ['BatchedAnderson', 'UnevenBatch', 'footprint_bytes']
 STATUS = {0: 'running', 1: 'converged', 2: 'step limit', 3: 'fallback', 4: 'non-finite'}
 
 
-def footprint_bytes(npoints, ninst, m=5):
-    """Device bytes of a batched solve: per instance 7 half spectra of the plan
-    workspace, the density (m + 1) and residual (m) history rings, plus the
-    density output and external potential of the grid."""
+def _nfields(fener):
+    hs = fener.part_dict.get('HardSphere') if fener is not None else None
+    return 12 if hs is not None and 't' in hs.version else 6
+
+
+def footprint_bytes(npoints, ninst, m=5, nfields=6):
+    """Device bytes of a batched solve: per instance nfields + 1 half spectra
+    of the plan workspace (nfields = 6, or 12 for the tensor FMT variant), the
+    density (m + 1) and residual (m) history rings, plus the density output
+    and external potential of the grid."""
     nx, ny, nz = (int(n) for n in npoints)
     npts, nhalf = nx * ny * nz, nx * ny * (nz // 2 + 1)
-    return ninst * (7 * 16 * nhalf + (2 * m + 2) * 8 * npts) + 8 * npts
+    return ninst * ((nfields + 1) * 16 * nhalf + (2 * m + 2) * 8 * npts) + 8 * npts
 
 
 class BatchedAnderson:
@@ -75,7 +81,7 @@ class BatchedAnderson:
         self.resolve_fallbacks = resolve_fallbacks
         if chunk is None:
             nx, ny, nz = self.grid.npoints
-            per_inst = 7 * 16 * nx * ny * (nz // 2 + 1) + (2 * m + 1) * 8 * nx * ny * nz
+            per_inst = (_nfields(self.fener) + 1) * 16 * nx * ny * (nz // 2 + 1) + (2 * m + 1) * 8 * nx * ny * nz
             chunk = max(1, min(len(self.chempots), int(mem_cap // per_inst)))
         self.chunk = int(chunk)
         self._groups = None
@@ -261,7 +267,7 @@ class BatchedAnderson:
 
     def footprint(self):
         """Device bytes this solver's groups hold (spectra, rings, scalars)."""
-        return footprint_bytes(self.grid.npoints, len(self.chempots), self.m)
+        return footprint_bytes(self.grid.npoints, len(self.chempots), self.m, _nfields(self.fener))
 
     def close(self):
         if self._graph is not None and self._groups:
@@ -296,7 +302,7 @@ class UnevenBatch:
         self.stats = {}
 
     def footprints(self):
-        return [footprint_bytes(p.grid.npoints, len(mu), self.m) for p, mu in self.items]
+        return [footprint_bytes(p.grid.npoints, len(mu), self.m, _nfields(p.fener)) for p, mu in self.items]
 
     def solve(self):
         """Returns a list (one per item) of BatchedAnderson result dicts."""

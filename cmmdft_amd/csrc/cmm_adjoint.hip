@@ -28,7 +28,7 @@ constexpr int S_ADJ_LAM = 27;
 
 struct AdjDev {
   const double* rho;    // converged density [inst][npts]
-  const double* nfld;   // weighted densities of rho [inst][6][npts] (n3 unclipped)
+  const double* nfld;   // weighted densities of rho [inst][NF][npts] (n3 unclipped)
   long long fstride;    // npts
   double beta, dr;
   double* y;
@@ -39,9 +39,10 @@ struct AdjDev {
   int parity;
 };
 
-// z pass of the Hessian product: 6 weighted densities of v (x,y inverse-
+// z pass of the Hessian product: NF weighted densities of v (x,y inverse-
 // transformed) -> C2R -> directional derivative of the pointwise derivative
-// fields at the weighted densities of rho -> R2C -> F[0..5] in place.
+// fields at the weighted densities of rho -> R2C -> F[0..NF) in place.
+template <int NF>
 __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long Fs, long long Ffs,
                                              const double* __restrict__ nfld, long long npts, GridDev g, FuncDev fp,
                                              ZTile t, const int* __restrict__ active) {
@@ -54,10 +55,10 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
-  z_load_half_multi(sm, F + inst * Fs, Ffs, 6, p0, t, npen, n, g.nzh, fBn);
+  z_load_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
   __syncthreads();
   z_fft<1>(sm, t, g.fz, s_tw);
-  const double* nb = nfld + (long long)inst * 6 * npts;
+  const double* nb = nfld + (long long)inst * NF * npts;
   const int tot = t.P * n;
   const FastDiv fP = make_fastdiv((unsigned)t.P);
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
@@ -71,10 +72,21 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
     double& r4 = z_val(sm, t, 4, pl, e);
     double& r5 = z_val(sm, t, 5, pl, e);
     double g0 = 0.0, g1 = 0.0, gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g5 = 0.0;
+    double gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (fp.use_hs) {
-      const PhiOut<Dual> o = fmt_point<Dual>(fp, Dual(nb[idx], r0), Dual(nb[npts + idx], r1),
-                                             Dual(nb[2 * npts + idx], r2), Dual(nb[3 * npts + idx], r3),
-                                             Dual(nb[4 * npts + idx], r4));
+      const Dual a0(nb[idx], r0), a1(nb[npts + idx], r1), a2(nb[2 * npts + idx], r2), a3(nb[3 * npts + idx], r3),
+          a4(nb[4 * npts + idx], r4);
+      PhiOut<Dual> o;
+      if constexpr (NF == 12) {
+        Dual tn[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tn[c] = Dual(nb[(6 + c) * npts + idx], z_val(sm, t, 6 + c, pl, e));
+        o = fmt_point<Dual, true>(fp, a0, a1, a2, a3, a4, tn);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) gt[c] = o.gt[c].d;
+      } else {
+        o = fmt_point<Dual>(fp, a0, a1, a2, a3, a4);
+      }
       g0 = o.g0.d;
       g1 = o.g1.d;
       gv0 = o.gv[0].d;
@@ -92,10 +104,14 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
     r3 = gv1;
     r4 = gv2;
     r5 = g5;
+    if constexpr (NF == 12) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) z_val(sm, t, 6 + c, pl, e) = gt[c];
+    }
   }
   __syncthreads();
   z_fft<-1>(sm, t, g.fz, s_tw);
-  z_store_half_multi(sm, F + inst * Fs, Ffs, 6, p0, t, npen, n, g.nzh, fBn);
+  z_store_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
 }
 
 // CG direction and its transform:

@@ -59,6 +59,8 @@ struct cmm_plan {
   double* alpha_out;
   const int* active;
   int tbx, tby, pz, pphi;
+  int nf;           // weighted-density fields of the functional: 6, or 12 with the tensor variant
+  int fcap;         // fields the workspace F holds per instance
   size_t ws_bytes;
   // batched-solve bookkeeping (allocated on first use)
   int* b_active;
@@ -108,7 +110,8 @@ int largest_even_le(int v) { return std::max(2, v - (v & 1)); }
 
 void default_tiles(cmm_plan* p) {
   const int budget = VPT * NT;               // complex values a block stages
-  // x passes: one tile of nx * tb_x columns per NTX threads (all 6 fields in LDS)
+  // x passes: one tile of nx * tb_x columns per NTX threads (all fields in LDS;
+  // launches with twelve fields cap tb_x by LDS)
   p->tbx = std::max(1, std::min(32, NTX / p->nx));
   int tby = std::min(32, std::min(budget / p->ny, 4096 / p->ny));
   p->tby = std::max(1, tby);
@@ -215,6 +218,8 @@ int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm
   hipError_t e = hipSuccess;
   e = (e == hipSuccess) ? hipMalloc(&p->rz, half_bytes) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->F, 6 * half_bytes) : e;
+  p->nf = 6;
+  p->fcap = 6;
   e = (e == hipSuccess) ? hipMalloc(&p->partials, (size_t)p->pstride * nbatch * sizeof(double)) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->scal, (size_t)NSCAL * nbatch * sizeof(double)) : e;
   e = (e == hipSuccess) ? hipMalloc(&p->inst, (size_t)nbatch * sizeof(InstDev)) : e;
@@ -314,8 +319,8 @@ int cmm_plan_get_radix(cmm_plan* p, int axis, int* nstages, int* radices) {
 
 int cmm_plan_get_info(cmm_plan* p, long long* info, int n) {
   if (!p || !info) return fail(CMM_E_ARG, "null argument");
-  long long v[10] = {p->nx, p->ny, p->nz, p->nb, p->tbx, p->tby, p->pz, p->pphi, (long long)p->ws_bytes, p->nzh};
-  for (int i = 0; i < n && i < 10; ++i) info[i] = v[i];
+  long long v[11] = {p->nx, p->ny, p->nz, p->nb, p->tbx, p->tby, p->pz, p->pphi, (long long)p->ws_bytes, p->nzh, p->nf};
+  for (int i = 0; i < n && i < 11; ++i) info[i] = v[i];
   return CMM_OK;
 }
 
@@ -336,6 +341,21 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
   d.use_hs = f->hs_base >= 0;
   d.base = std::max(0, f->hs_base);
   d.anti = f->hs_antisym ? 1 : 0;
+  d.tensor = (d.use_hs && f->hs_tensor) ? 1 : 0;
+  const int nf = d.tensor ? 12 : 6;
+  if (nf == 12 && (12 * p->nz > VPT * NT || 13 * 16 * p->nx > 65536 - CMM_MAX_N * 16))
+    return fail(CMM_E_UNSUPPORTED, "the tensor variant needs nz <= 170 (twelve fields per z tile)");
+  if (nf > p->fcap) {
+    // grow the workspace to twelve half spectra per instance
+    (void)hipDeviceSynchronize();
+    double2* Fn = nullptr;
+    HIPCHK(hipMalloc(&Fn, (size_t)nf * p->nhalf * p->nb * sizeof(double2)));
+    if (p->F) (void)hipFree(p->F);
+    p->F = Fn;
+    p->fcap = nf;
+    p->ws_bytes = (size_t)(nf + 1) * p->nhalf * p->nb * sizeof(double2);
+  }
+  p->nf = nf;
   d.use_wda = f->use_wda && f->n_eos > 0;
   d.neos = f->n_eos;
   for (int t = 0; t < 4; ++t) {
@@ -351,6 +371,7 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
   d.c_w3 = 4.0 * M_PI * d.R * d.R * d.R / 3.0;
   d.inv4piR2 = 1.0 / (4.0 * M_PI * d.R * d.R);
   d.inv4piR = 1.0 / (4.0 * M_PI * d.R);
+  d.c_tor = 3.0 / d.R;
   p->has_func = 1;
   return CMM_OK;
 }
@@ -386,7 +407,7 @@ static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sg
 }
 
 static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
-  return launch_ypass_is(p, F, (nf == 1 && F == p->rz) ? p->nhalf : 6 * p->nhalf, nf, sgn);
+  return launch_ypass_is(p, F, (nf == 1 && F == p->rz) ? p->nhalf : (long long)p->fcap * p->nhalf, nf, sgn);
 }
 
 static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
@@ -402,32 +423,64 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
   return CMM_OK;
 }
 
-// x passes of the functional chain (see k_xweights / k_xback)
+// x passes of the functional chain (see k_xweights / k_xback): tiles of
+// nx * tb columns with all nf fields in LDS ((nf + 1) nx tb complex values
+// plus the static twiddle table, at most 64 KB together)
+static int xpass_tb(cmm_plan* p) {
+  const int lim = (65536 - CMM_MAX_N * 16) / (16 * (p->nf + 1) * p->nx);
+  return std::max(1, std::min(p->tbx, lim));
+}
+
+template <int NF>
+static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
+  const int T = p->nx * tb;
+  dim3 grid((unsigned)(8 * per), 1, p->nb);
+  hipLaunchKernelGGL(k_xweights<NF>, grid, dim3(NTX), (size_t)(NF + 1) * T * sizeof(double2), p->stream, p->rz,
+                     p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
+                     make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)p->nzh), per,
+                     ntiles, p->active);
+}
+
 static int launch_xweights(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
-  const long long ntiles = (nb + p->tbx - 1) / p->tbx;
+  const int tb = xpass_tb(p);
+  const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
-  const int T = p->nx * p->tbx;
-  dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL(k_xweights, grid, dim3(NTX), 7 * (size_t)T * sizeof(double2), p->stream, p->rz, p->nhalf, p->F,
-                     6 * p->nhalf, p->nhalf, p->g, p->fp, p->tbx, make_fastdiv((unsigned)p->tbx),
-                     make_fastdiv((unsigned)(6 * p->tbx)), make_fastdiv((unsigned)p->nzh), per, ntiles, p->active);
+  if (p->nf == 12) launch_xweights_nf<12>(p, tb, ntiles, per);
+  else launch_xweights_nf<6>(p, tb, ntiles, per);
   LAUNCHCHK("k_xweights");
   return CMM_OK;
 }
 
+template <int NF>
+static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
+  const int T = p->nx * tb;
+  dim3 grid((unsigned)(8 * per), 1, p->nb);
+  hipLaunchKernelGGL(k_xback<NF>, grid, dim3(NTX), (size_t)(NF + 1) * T * sizeof(double2), p->stream, p->F,
+                     (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb, make_fastdiv((unsigned)tb),
+                     make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh), per,
+                     ntiles, p->active);
+}
+
 static int launch_xback(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
-  const long long ntiles = (nb + p->tbx - 1) / p->tbx;
+  const int tb = xpass_tb(p);
+  const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
-  const int T = p->nx * p->tbx;
-  dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL(k_xback, grid, dim3(NTX), 7 * (size_t)T * sizeof(double2), p->stream, p->F, 6 * p->nhalf,
-                     p->nhalf, p->g, p->fp, p->tbx, make_fastdiv((unsigned)p->tbx),
-                     make_fastdiv((unsigned)(6 * p->tbx)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh),
-                     per, ntiles, p->active);
+  if (p->nf == 12) launch_xback_nf<12>(p, tb, ntiles, per);
+  else launch_xback_nf<6>(p, tb, ntiles, per);
   LAUNCHCHK("k_xback");
   return CMM_OK;
+}
+
+// z-pass pencils of the nf-field functional pass: the plan's p_phi for 6
+// fields; for 12, half of it (at least 2) within the staging and LDS limits
+static int zphi_p(cmm_plan* p) {
+  if (p->nf != 12) return p->pphi;
+  int P = std::max(2, p->pphi / 2);
+  P -= P & 1;
+  while (P > 2 && (6 * P * p->nz > VPT * NT || zlds(p, P, 12) > 65536)) P -= 2;
+  return P;
 }
 
 int cmm_fft_r2c(cmm_plan* p, const double* x, double* X) {
@@ -493,25 +546,30 @@ static int forward_chain(cmm_plan* p) {
   if (rc) return rc;
   rc = launch_xweights(p);
   if (rc) return rc;
-  return launch_ypass(p, p->F, 6, 1);
+  return launch_ypass(p, p->F, p->nf, 1);
 }
 
 static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
-  dim3 grid((unsigned)zblocks(p, p->pphi), p->nb);
-  hipLaunchKernelGGL(k_zphi, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, p->g,
-                     p->fp, ztile(p, p->pphi, 6), store, p->active, red_of(p));
+  const int P = zphi_p(p);
+  dim3 grid((unsigned)zblocks(p, P), p->nb);
+  if (p->nf == 12)
+    hipLaunchKernelGGL(k_zphi<12>, grid, dim3(NT), zlds(p, P, 12), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                       p->nhalf, p->g, p->fp, ztile(p, P, 12), store, p->active, red_of(p));
+  else
+    hipLaunchKernelGGL(k_zphi<6>, grid, dim3(NT), zlds(p, P, 6), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                       p->nhalf, p->g, p->fp, ztile(p, P, 6), store, p->active, red_of(p));
   LAUNCHCHK("k_zphi");
   return reduce ? launch_reduce(p, grid.x, 3, 1u << 2, S_PHI) : CMM_OK;
 }
 
-// back-convolution of the 6 derivative fields in F: inverse y, the x pass
+// back-convolution of the nf derivative fields in F: inverse y, the x pass
 // with the weighted sum, inverse y of the sum (left in field 0 of F)
 static int back_chain(cmm_plan* p) {
-  int rc = launch_ypass(p, p->F, 6, -1);
+  int rc = launch_ypass(p, p->F, p->nf, -1);
   if (rc) return rc;
   rc = launch_xback(p);
   if (rc) return rc;
-  return launch_ypass_is(p, p->F, 6 * p->nhalf, 1, 1);
+  return launch_ypass_is(p, p->F, (long long)p->fcap * p->nhalf, 1, 1);
 }
 
 int cmm_derive(cmm_plan* p) {
@@ -527,9 +585,14 @@ int cmm_hessian_apply(cmm_plan* p, const double* nfld) {
   if (!p || !nfld) return fail(CMM_E_ARG, "null argument");
   int rc = forward_chain(p);
   if (rc) return rc;
-  dim3 grid((unsigned)zblocks(p, p->pphi), p->nb);
-  hipLaunchKernelGGL(k_zhess, grid, dim3(NT), zlds(p, p->pphi, 6), p->stream, p->F, 6 * p->nhalf, p->nhalf, nfld,
-                     p->npts, p->g, p->fp, ztile(p, p->pphi, 6), p->active);
+  const int P = zphi_p(p);
+  dim3 grid((unsigned)zblocks(p, P), p->nb);
+  if (p->nf == 12)
+    hipLaunchKernelGGL(k_zhess<12>, grid, dim3(NT), zlds(p, P, 12), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                       p->nhalf, nfld, p->npts, p->g, p->fp, ztile(p, P, 12), p->active);
+  else
+    hipLaunchKernelGGL(k_zhess<6>, grid, dim3(NT), zlds(p, P, 6), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                       p->nhalf, nfld, p->npts, p->g, p->fp, ztile(p, P, 6), p->active);
   LAUNCHCHK("k_zhess");
   return back_chain(p);
 }
@@ -548,7 +611,7 @@ int cmm_launch_pass(cmm_plan* p, int which) {
     case 0: return launch_zphi(p, 1, false);
     case 1: return launch_xweights(p);
     case 2: return launch_xback(p);
-    case 3: return launch_ypass(p, p->F, 6, -1);
+    case 3: return launch_ypass(p, p->F, p->nf, -1);
     case 4: return launch_ypass(p, p->rz, 1, -1);
     default: return fail(CMM_E_ARG, "unknown pass");
   }
@@ -558,9 +621,9 @@ int cmm_weighted_densities(cmm_plan* p, double* out) {
   if (!p || !out) return fail(CMM_E_ARG, "null argument");
   int rc = forward_chain(p);
   if (rc) return rc;
-  dim3 grid((unsigned)zblocks(p, p->pz), 6, p->nb);
-  hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->nhalf, out,
-                     6 * p->npts, p->npts, p->g, ztile(p, p->pz, 1), p->active);
+  dim3 grid((unsigned)zblocks(p, p->pz), p->nf, p->nb);
+  hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                     p->nhalf, out, (long long)p->nf * p->npts, p->npts, p->g, ztile(p, p->pz, 1), p->active);
   LAUNCHCHK("k_zc2r");
   return CMM_OK;
 }
@@ -568,7 +631,8 @@ int cmm_weighted_densities(cmm_plan* p, double* out) {
 template <int MODE>
 static int launch_final(cmm_plan* p, const ElDev& el, const char* name) {
   dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
-  hipLaunchKernelGGL(k_zfinal<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, el,
+  hipLaunchKernelGGL(k_zfinal<MODE>, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                     p->g, el,
                      ztile(p, p->pz, 1), p->active, red_of(p));
   LAUNCHCHK(name);
   constexpr int NQ = (MODE == MODE_ANDRES) ? (4 + CMM_MMAX) : 8;
@@ -734,7 +798,8 @@ int cmm_adjoint_final(cmm_plan* p, const double* rho, double beta, const double*
   a.p = const_cast<double*>(pvec);
   a.q = q;
   dim3 grid((unsigned)zblocks(p, p->pz), p->nb);
-  hipLaunchKernelGGL(k_adj_final, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, 6 * p->nhalf, p->g, a,
+  hipLaunchKernelGGL(k_adj_final, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                     p->g, a,
                      ztile(p, p->pz, 1), p->active, red_of(p));
   LAUNCHCHK("k_adj_final");
   return launch_reduce(p, grid.x, 1, 0u, S_ADJ_PQ);

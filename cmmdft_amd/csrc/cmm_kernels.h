@@ -40,11 +40,13 @@ struct GridDev {
 // field, 1 Carnahan-Starling, 2 MBWR).
 struct FuncDev {
   int use_hs, use_wda, base, anti, neos;
+  int tensor;       // the 't' variant: six tensor weighted densities (fields 6..11)
   int eos_kind[4];
   double eos_fac[4];
   double eos_par[4][16];
   double R, D, kT;
   double c_w2, c_w3, inv4piR2, inv4piR;
+  double c_tor;     // 3 / R: tensor weight B = w2 - (3/R) w3
 };
 
 struct InstDev {

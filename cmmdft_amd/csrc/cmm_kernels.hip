@@ -304,7 +304,8 @@ __global__ void __launch_bounds__(NT) k_xpass(double2* __restrict__ F, long long
 // ---------------------------------------------------------------------------
 struct KW {
   double w2, w3, wD;
-  bool k0;   // omega = |k| R <= 1e-8: the k -> 0 limits apply, vector weights vanish
+  double wt;  // tensor weight amplitude B = -4 pi R^2 (j_ball - j0) (Lanczos included) = w2 - (3/R) w3
+  bool k0;    // omega = |k| R <= 1e-8: the k -> 0 limits apply, vector and tensor weights vanish
 };
 
 __device__ __forceinline__ double j_ball(double w, double s, double c) {
@@ -334,6 +335,7 @@ __device__ __forceinline__ KW kweights(const GridDev& g, const FuncDev& fp, int 
     sincos(wd, &s, &c);
     r.wD = j_ball(wd, s, c) * sl;
   }
+  r.wt = r.w2 - fp.c_tor * r.w3;
   return r;
 }
 
@@ -343,6 +345,24 @@ __device__ __forceinline__ KW kweights(const GridDev& g, const FuncDev& fp, int 
 // weights change sign).  Each vector component is zero on the Nyquist plane of
 // its own axis: there its spectrum has no Hermitian part, so the real part of
 // the inverse transform does not see it.  At k = 0 the vector weights vanish.
+//
+// Tensor fields 6..11 (xx, xy, xz, yy, yz, zz; the 't' variant): real, even
+// weights B (k_i k_j / k^2 - delta_ij / 3), zero at k = 0, the same in both
+// directions.  An off-diagonal component is odd in k_i and in k_j, so where
+// exactly one of the two indices is on its axis' Nyquist plane its spectrum
+// has no Hermitian part and it is zero there (on both planes it is even).
+__device__ __forceinline__ double2 tensor_w(const GridDev& g, const KW& w, int ix, int iy, int iz, int c, double2 v) {
+  if (w.k0) return make_double2(0.0, 0.0);
+  const double k[3] = {g.kx.k[ix], g.ky.k[iy], g.kz.k[iz]};
+  const bool nq[3] = {ix == g.kx.nyq, iy == g.ky.nyq, iz == g.kz.nyq};
+  const int a = (c < 3) ? 0 : (c < 5 ? 1 : 2);
+  const int b = (c < 3) ? c : (c < 5 ? c - 2 : 2);
+  if (a != b && nq[a] != nq[b]) return make_double2(0.0, 0.0);
+  const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+  const double h = k[a] * k[b] / k2 - (a == b ? 1.0 / 3.0 : 0.0);
+  return cscale(v, w.wt * h);
+}
+
 __device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix, int iy, int iz, int f, double2 v,
                                            int dir) {
   const bool k0 = w.k0;
@@ -352,23 +372,26 @@ __device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix
     case 2: return (k0 || ix == g.kx.nyq) ? make_double2(0.0, 0.0) : cmul_i(cscale(v, w.w3 * g.kx.k[ix]), (double)dir);
     case 3: return (k0 || iy == g.ky.nyq) ? make_double2(0.0, 0.0) : cmul_i(cscale(v, w.w3 * g.ky.k[iy]), (double)dir);
     case 4: return (k0 || iz == g.kz.nyq) ? make_double2(0.0, 0.0) : cmul_i(cscale(v, w.w3 * g.kz.k[iz]), (double)dir);
-    default: return cscale(v, w.wD);
+    case 5: return cscale(v, w.wD);
+    default: return tensor_w(g, w, ix, iy, iz, f - 6, v);
   }
 }
 
 // ---------------------------------------------------------------------------
 // x passes of the functional chain.  A block (NTX threads) owns a tile of TB
 // consecutive (ky, kz) columns of the flattened (y, z) plane, all nx, with
-// nx * TB <= NTX, and holds all six fields of the tile in LDS at once
-// (element-major, [x][6 TB]), so that its global loads are all in flight
-// together and one Stockham pass transforms the 6 TB sequences.
+// nx * TB <= NTX, and holds all NF fields of the tile in LDS at once
+// (element-major, [x][NF TB]), so that its global loads are all in flight
+// together and one Stockham pass transforms the NF TB sequences.
 // Tiles are taken in the XCD-aware order of xcd_tile.
 // ---------------------------------------------------------------------------
 constexpr int NTX = 512;
 
-// forward chain: rho (z,y transformed) -> x FFT -> 6 weighted copies ->
-// inverse x FFT of all six -> F[0..5].  The 1/N of the inverse 3-D transform
-// is folded into the weights.
+// forward chain: rho (z,y transformed) -> x FFT -> NF weighted copies ->
+// inverse x FFT of all of them -> F[0..NF).  The 1/N of the inverse 3-D
+// transform is folded into the weights.  NF = 6 (fields n2, n3, nv2 x/y/z,
+// rho_bar) or 12 (plus the six tensor components).
+template <int NF>
 __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
                                                  long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
                                                  FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
@@ -381,9 +404,9 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
   const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
-  const int S6 = 6 * TB;
+  const int S6 = NF * TB;
   double2* A = sm;        // [nx][TB]
-  double2* B = sm + T;    // [nx][6 TB]
+  double2* B = sm + T;    // [nx][NF TB]
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
@@ -415,12 +438,12 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
     const KW w = kweights(g, fp, x, wy, wz);
     const double2 a = A[i];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) B[x * S6 + f * TB + b] = cscale(apply_w(g, w, x, wy, wz, f, a, -1), g.inv_npts);
+    for (int f = 0; f < NF; ++f) B[x * S6 + f * TB + b] = cscale(apply_w(g, w, x, wy, wz, f, a, -1), g.inv_npts);
   }
   __syncthreads();
   lds_fft<1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
 #pragma unroll
-  for (int f = 0; f < 6; ++f) {
+  for (int f = 0; f < NF; ++f) {
     double2* dst = F + inst * Fs + f * Ffs + b0;
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
       const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
@@ -429,9 +452,10 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
   }
 }
 
-// back-convolution: the 6 derivative fields -> x FFT of all six -> weighted
-// sum -> inverse x FFT -> F[0] (in place: the block reads its whole tile of
-// every field before it writes field 0's tile).
+// back-convolution: the NF derivative fields -> x FFT of all of them ->
+// weighted sum -> inverse x FFT -> F[0] (in place: the block reads its whole
+// tile of every field before it writes field 0's tile).
+template <int NF>
 __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
                                               FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fT, FastDiv fnzh,
                                               int per, long long ntiles, const int* __restrict__ active) {
@@ -443,18 +467,18 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
   const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
-  const int S6 = 6 * TB;
-  double2* B = sm;             // [nx][6 TB]
-  double2* A = sm + 6 * T;     // [nx][TB]
+  const int S6 = NF * TB;
+  double2* B = sm;             // [nx][NF TB]
+  double2* A = sm + NF * T;    // [nx][TB]
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = F + inst * Fs + b0;
-  for (int i0 = threadIdx.x; i0 < 6 * T; i0 += LU * blockDim.x) {
+  for (int i0 = threadIdx.x; i0 < NF * T; i0 += LU * blockDim.x) {
     double2 v[LU];
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
-      const int i = min(i0 + u * (int)blockDim.x, 6 * T - 1);
+      const int i = min(i0 + u * (int)blockDim.x, NF * T - 1);
       const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
       const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
       v[u] = src[f * Ffs + (long long)x * nb + min(b, tbv - 1)];
@@ -462,7 +486,7 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = i0 + u * (int)blockDim.x;
-      if (i < 6 * T) {
+      if (i < NF * T) {
         const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
         const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
         B[x * S6 + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
@@ -479,7 +503,7 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
     const KW w = kweights(g, fp, x, wy, wz);
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int f = 0; f < 6; ++f) acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, f, B[x * S6 + f * TB + b], 1), g.inv_npts));
+    for (int f = 0; f < NF; ++f) acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, f, B[x * S6 + f * TB + b], 1), g.inv_npts));
     A[i] = acc;
   }
   __syncthreads();
@@ -549,6 +573,7 @@ __device__ __forceinline__ Dual clip(Dual x, double lo, double hi) {
 template <typename T>
 struct PhiOut {
   T phi, g0, g1, gv[3];
+  T gt[6];   // tensor derivative fields (the 't' variant)
 };
 
 template <typename T>
@@ -598,8 +623,14 @@ __device__ __forceinline__ void phi23(int base, T n3, T ln3, T& p2, T& p3, T& d2
 //   g0 = kT (dPhi/dn0 / 4piR^2 + dPhi/dn1 / 4piR + dPhi/dn2)    with w2
 //   g1 = kT dPhi/dn3                                           with w3
 //   gv = kT (dPhi/dnv1 / 4piR + dPhi/dnv2)                      with -wv2
-template <typename T>
-__device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw, T vx, T vy, T vz) {
+//   gt = kT dPhi/dn_t (TEN: the 't' variant, tensor tn = xx xy xz yy yz zz;
+//        off-diagonal derivatives counted twice, as the symmetric tensor
+//        holds each of them twice)                             with w_t
+// The 't' terms (functionals.py:636-640, 673-717) add 9/2 (nv2 . T . nv2 -
+// tr T^3) to the coefficient of phi3.
+template <typename T, bool TEN = false>
+__device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw, T vx, T vy, T vz,
+                                               const T* tn = nullptr) {
   PhiOut<T> o;
   const double R = fp.R;
   const double i4pR2 = fp.inv4piR2, i4pR = fp.inv4piR;
@@ -623,6 +654,16 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
     dq3_n2 = 3.0 * (n2 * n2 - v2);
     fv = -6.0 * n2 * p3;
   }
+  if constexpr (TEN) {
+    const T xx = tn[0], xy = tn[1], xz = tn[2], yy = tn[3], yz = tn[4], zz = tn[5];
+    const T quad = xx * vx * vx + yy * vy * vy + zz * vz * vz + 2.0 * xy * vx * vy + 2.0 * xz * vx * vz +
+                   2.0 * yz * vy * vz;
+    const T tr3 = xx * xx * xx + yy * yy * yy + zz * zz * zz +
+                  3.0 * (xx * xy * xy + xx * xz * xz + yy * xy * xy + yy * yz * yz + zz * xz * xz + zz * yz * yz) +
+                  6.0 * xy * xz * yz;
+    q3 = q3 + 4.5 * quad;
+    q3 = q3 - 4.5 * tr3;
+  }
   o.phi = n0 * (-ln3) + (n1 * n2 - dot12) * p2 + q3 * p3;
   const T dn0 = -ln3;
   const T dn1 = n2 * p2;
@@ -632,9 +673,23 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
   o.g0 = kT * (dn0 * i4pR2 + dn1 * i4pR + dn2);
   o.g1 = kT * dn3;
   // dPhi/dnv1 = -nv2 p2 ; dPhi/dnv2 = -nv1 p2 + nv2 fv
-  o.gv[0] = kT * ((-vx * p2) * i4pR + (-ux * p2 + vx * fv));
-  o.gv[1] = kT * ((-vy * p2) * i4pR + (-uy * p2 + vy * fv));
-  o.gv[2] = kT * ((-vz * p2) * i4pR + (-uz * p2 + vz * fv));
+  if constexpr (TEN) {
+    const T xx = tn[0], xy = tn[1], xz = tn[2], yy = tn[3], yz = tn[4], zz = tn[5];
+    const T p3t = 4.5 * p3;
+    o.gv[0] = kT * ((-vx * p2) * i4pR + (-ux * p2 + vx * fv + p3t * (2.0 * (xx * vx + xy * vy + xz * vz))));
+    o.gv[1] = kT * ((-vy * p2) * i4pR + (-uy * p2 + vy * fv + p3t * (2.0 * (xy * vx + yy * vy + yz * vz))));
+    o.gv[2] = kT * ((-vz * p2) * i4pR + (-uz * p2 + vz * fv + p3t * (2.0 * (xz * vx + yz * vy + zz * vz))));
+    o.gt[0] = kT * (p3t * (vx * vx - 3.0 * (xx * xx + xy * xy + xz * xz)));
+    o.gt[1] = kT * (p3t * ((vx * vy - 3.0 * (xx * xy + yy * xy + xz * yz)) * 2.0));
+    o.gt[2] = kT * (p3t * ((vx * vz - 3.0 * (xx * xz + zz * xz + xy * yz)) * 2.0));
+    o.gt[3] = kT * (p3t * (vy * vy - 3.0 * (yy * yy + xy * xy + yz * yz)));
+    o.gt[4] = kT * (p3t * ((vy * vz - 3.0 * (yy * yz + zz * yz + xy * xz)) * 2.0));
+    o.gt[5] = kT * (p3t * (vz * vz - 3.0 * (zz * zz + xz * xz + yz * yz)));
+  } else {
+    o.gv[0] = kT * ((-vx * p2) * i4pR + (-ux * p2 + vx * fv));
+    o.gv[1] = kT * ((-vy * p2) * i4pR + (-uy * p2 + vy * fv));
+    o.gv[2] = kT * ((-vz * p2) * i4pR + (-uz * p2 + vz * fv));
+  }
   (void)R;
   return o;
 }
@@ -696,9 +751,11 @@ __device__ __forceinline__ void wda_point(const FuncDev& fp, T rb, T& f, T& df) 
   }
 }
 
-// z pass of the functional: 6 weighted densities (x,y inverse-transformed)
-// -> C2R -> pointwise Phi / WDA derivatives -> R2C -> F[0..5] in place.
+// z pass of the functional: NF weighted densities (x,y inverse-transformed)
+// -> C2R -> pointwise Phi / WDA derivatives -> R2C -> F[0..NF) in place
+// (NF = 12: the tensor variant's six more fields).
 // Reduces sum Phi, sum f_ex(rho_bar), max n3 (unclipped) per instance.
+template <int NF>
 __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
                                             ZTile t, int store, const int* __restrict__ active, RedDev red) {
   extern __shared__ double2 sm[];
@@ -711,7 +768,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
-  z_load_half_multi(sm, F + inst * Fs, Ffs, 6, p0, t, npen, n, g.nzh, fBn);
+  z_load_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
   __syncthreads();
   z_fft<1>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
@@ -729,8 +786,18 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
     double& r4 = z_val(sm, t, 4, pl, e);
     double& r5 = z_val(sm, t, 5, pl, e);
     double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
+    double tn[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (fp.use_hs) {
-      const PhiOut<double> o = fmt_point<double>(fp, r0, r1, r2, r3, r4);
+      PhiOut<double> o;
+      if constexpr (NF == 12) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tn[c] = z_val(sm, t, 6 + c, pl, e);
+        o = fmt_point<double, true>(fp, r0, r1, r2, r3, r4, tn);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) gt[c] = o.gt[c];
+      } else {
+        o = fmt_point<double>(fp, r0, r1, r2, r3, r4);
+      }
       acc[0] += o.phi;
       acc[2] = fmax(acc[2], r1);
       g0 = o.g0;
@@ -751,11 +818,15 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
     r3 = gv1;
     r4 = gv2;
     r5 = g5;
+    if constexpr (NF == 12) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) z_val(sm, t, 6 + c, pl, e) = gt[c];
+    }
   }
   if (store) {
     __syncthreads();
     z_fft<-1>(sm, t, g.fz, s_tw);
-    z_store_half_multi(sm, F + inst * Fs, Ffs, 6, p0, t, npen, n, g.nzh, fBn);
+    z_store_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);

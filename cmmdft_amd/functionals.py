@@ -47,10 +47,9 @@ def build_functional(hs=None, wda=None, kT=None):
     f.hs_base = -1
     f.R = 1.0
     if hs is not None:
-        if 't' in hs.version:
-            raise NotImplementedError('tensor FMT versions are not implemented on the device')
         f.hs_base = hs_base_of(hs.version)
         f.hs_antisym = 1 if 'a' in hs.version else 0
+        f.hs_tensor = 1 if 't' in hs.version else 0
         f.R = hs.R
     if wda is not None:
         terms = wda.eos.device_terms()
@@ -152,13 +151,14 @@ class HardSphereFunctional(Functional):
         return s[S_PHI] * self.grid.dr / self.beta
 
     def weighted_densities_dev(self, krho):
-        """n2, n3 (unclipped), nv2 (3) and the WDA density as a (6, N) tensor."""
+        """n2, n3 (unclipped), nv2 (3), the WDA density and, for the tensor
+        variant, n_t (xx, xy, xz, yy, yz, zz) as a (6 or 12, N) tensor."""
         torch = torch_mod()
         rho = density_from(self.grid, krho)
         plan = self.grid.plan()
         plan.set_functional(self._func())
         plan.seed(rho)
-        out = torch.empty((6,) + tuple(self.grid.npoints), dtype=torch.float64, device=rho.device)
+        out = torch.empty((plan.nfields,) + tuple(self.grid.npoints), dtype=torch.float64, device=rho.device)
         plan.weighted_densities(out)
         return out
 
@@ -205,7 +205,7 @@ class WDAVFunctional(Functional):
         f.R = self.R
         plan.set_functional(f)
         plan.seed(rho)
-        out = torch.empty((6,) + tuple(self.grid.npoints), dtype=torch.float64, device=rho.device)
+        out = torch.empty((plan.nfields,) + tuple(self.grid.npoints), dtype=torch.float64, device=rho.device)
         plan.weighted_densities(out)
         return like_input(out[5], krho)
 

@@ -68,12 +68,14 @@ extern "C" {
 typedef struct cmm_plan cmm_plan;
 
 /* Functional definition.  hs_base: -1 none, 0 Rosenfeld FMT, 1 MFMT,
- * 2 White-Bear II; hs_antisym: the 'a' variant.  WDA EOS terms: kind 0 mean
+ * 2 White-Bear II; hs_antisym: the 'a' variant; hs_tensor: the 't' variant
+ * (twelve weighted-density fields; nz <= 170).  WDA EOS terms: kind 0 mean
  * field (par[0] = a), 1 Carnahan-Starling (par = {eta, kT}), 2 MBWR
  * (par = {a1..a8, b1..b6, sigma^3, epsilon}); factor multiplies the term. */
 typedef struct {
   int hs_base;
   int hs_antisym;
+  int hs_tensor;
   int use_wda;
   int n_eos;
   int eos_kind[4];
@@ -102,7 +104,8 @@ int cmm_plan_set_tiles(cmm_plan* plan, int tb_x, int tb_y, int p_z, int p_phi);
  * the default is 5s, 3s, 8s, 4s, 2s) */
 int cmm_plan_set_radix(cmm_plan* plan, int axis, int nstages, const int* radices);
 int cmm_plan_get_radix(cmm_plan* plan, int axis, int* nstages, int* radices);
-/* info[0..9] = nx, ny, nz, nbatch, tb_x, tb_y, p_z, p_phi, workspace bytes, nzh */
+/* info[0..10] = nx, ny, nz, nbatch, tb_x, tb_y, p_z, p_phi, workspace bytes, nzh,
+ * nf (weighted-density fields of the functional set: 6 or 12) */
 int cmm_plan_get_info(cmm_plan* plan, long long* info, int n);
 /* device int[nbatch] mask of instances to process (NULL = all) */
 int cmm_plan_set_active(cmm_plan* plan, const int* dev_mask);
@@ -124,8 +127,9 @@ int cmm_seed(cmm_plan* plan, const double* rho, const double* vext, int vext_bat
 int cmm_derive(cmm_plan* plan);
 /* value-only chain (no back-convolution): reduces sum Phi, sum f_ex, max n3 */
 int cmm_value(cmm_plan* plan);
-/* weighted densities n2, n3, nv2x, nv2y, nv2z, rho_bar of the seeded density
- * into out[nbatch][6][npts] (n3 unclipped) */
+/* weighted densities n2, n3, nv2x, nv2y, nv2z, rho_bar (then, for the tensor
+ * variant, n_t xx xy xz yy yz zz) of the seeded density into
+ * out[nbatch][nf][npts], nf = 6 or 12 (n3 unclipped) */
 int cmm_weighted_densities(cmm_plan* plan, double* out);
 
 /* launch one pass of the functional chain on the current workspace

@@ -89,13 +89,14 @@ def _check_hvp(prog, rho, seed, tol):
 
 
 @pytest.mark.parametrize('version,eos', [('WBII', 'MFA'), ('FMT', 'MFA'), ('aWBII', 'MFA'), ('MFMT', None),
-                                         (None, 'corr'), ('aFMT', 'corr'), ('WBII', 'corr')])
+                                         (None, 'corr'), ('aFMT', 'corr'), ('WBII', 'corr'), ('tFMT', 'MFA'),
+                                         ('atWBII', 'MFA'), ('tMFMT', 'corr')])
 def test_hessian_vector_matches_finite_differences(version, eos):
     prog, host = _program(version, eos)
     _check_hvp(prog, _dense_density(prog.grid), 11, 1e-8)
 
 
-@pytest.mark.parametrize('version,eos', [('FMT', 'MFA'), ('aFMT', 'corr'), (None, 'corr')])
+@pytest.mark.parametrize('version,eos', [('FMT', 'MFA'), ('aFMT', 'corr'), (None, 'corr'), ('tFMT', 'MFA')])
 def test_hessian_vector_near_empty_regions(version, eos):
     """Densities ~1e-12 near the host atoms (n3 down to the clip).  The White-
     Bear forms are left out here: at n3 << 1 they lose ~1e-9 of relative
@@ -104,7 +105,8 @@ def test_hessian_vector_near_empty_regions(version, eos):
     _check_hvp(prog, _structured_density(prog.grid, host), 13, 1e-8)
 
 
-@pytest.mark.parametrize('version,eos', [('WBII', 'corr'), ('FMT', 'MFA'), ('aWBII', 'MFA')])
+@pytest.mark.parametrize('version,eos', [('WBII', 'corr'), ('FMT', 'MFA'), ('aWBII', 'MFA'), ('atWBII', 'MFA'),
+                                         ('tFMT', 'corr')])
 def test_hessian_is_symmetric(version, eos):
     """u.Mv = v.Mu.  (Where the reference's clips act -- n3 below 1e-30, or
     xi above 1 for the 'a' forms -- its c_ex is not the gradient of F_ex and M
@@ -222,6 +224,11 @@ def adjoint_case_fmt():
     return _adjoint_case('FMT')
 
 
+@pytest.fixture(scope='module')
+def adjoint_case_tensor():
+    return _adjoint_case('tFMT')
+
+
 def _fd_host(prog, mu, sig, eps, t, j, rel=1e-4):
     """Central differences of N in parameter j (0 sigma, 1 epsilon) of host
     type t with relative steps rel and 2 rel: (plain, Richardson)."""
@@ -241,7 +248,7 @@ def _fd_host(prog, mu, sig, eps, t, j, rel=1e-4):
     return d1, (4 * d1 - d2) / 3
 
 
-@pytest.mark.parametrize('case', ['adjoint_case', 'adjoint_case_fmt'])
+@pytest.mark.parametrize('case', ['adjoint_case', 'adjoint_case_fmt', 'adjoint_case_tensor'])
 def test_dN_dmu_matches_finite_differences(case, request):
     prog, host, mu, N, g, sig, eps = request.getfixturevalue(case)
     h = 1e-4 * abs(mu)
@@ -263,6 +270,17 @@ def test_host_parameter_gradients_match_finite_differences(adjoint_case_fmt):
             d1, rich = _fd_host(prog, mu, sig, eps, t, j)
             assert abs(grad[t, j] - d1) <= 1e-5 * abs(d1), (t, j, grad[t, j], d1)
             assert abs(grad[t, j] - rich) <= 2e-6 * abs(rich), (t, j, grad[t, j], rich)
+
+
+def test_host_parameter_gradients_tensor(adjoint_case_tensor):
+    """Tensor Rosenfeld FMT ('tFMT') + mean-field WDA: epsilon and sigma of two
+    host types against central differences (relative step 1e-4)."""
+    prog, host, mu, N, g, sig, eps = adjoint_case_tensor
+    for t in (0, 2):
+        for j in range(2):
+            d1, rich = _fd_host(prog, mu, sig, eps, t, j)
+            grad = (g['dN_dsigma'], g['dN_depsilon'])[j][t]
+            assert abs(grad - d1) <= 1e-5 * abs(d1), (t, j, grad, d1)
 
 
 def test_host_parameter_gradients_white_bear(adjoint_case):

@@ -72,6 +72,29 @@ def test_batch_matches_oracle(setup):
     b.close()
 
 
+def test_batch_tensor_functional_equals_plain():
+    """The reference's default hard-sphere functional (atWBII, twelve
+    weighted-density fields) batched: bit-identical to the plain solver."""
+    import torch
+    from cmmdft_amd.batch import BatchedAnderson
+    from cmmdft_amd.solver import Anderson
+    host = W.host_case1()
+    prog = product_program(host, (20, 20, 20), 'atWBII')
+    mus = _mus(4)
+    b = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400)
+    res = b.solve()
+    assert res['converged'].all(), res['status']
+    for i, mu in enumerate(mus):
+        if i in res['resolved']:
+            continue
+        s = Anderson(prog, nsteps=400, method='anderson', m=5, damping=0.1, criterion='RIUE', threshold=1e-10)
+        r0 = prog._set_initial_density(chempot=mu, temperature=T)
+        N, rho, conv = s.solve(mu, r0)
+        assert conv and s.curr_step == res['iters'][i]
+        assert torch.equal(rho.reshape(-1), res['rho'][i].reshape(-1))
+    b.close()
+
+
 def test_batch_equals_plain_larger_grid():
     """Multi-instance groups at 64^3 (hundreds of blocks per reduction): every
     instance must follow the plain solver's trajectory exactly."""

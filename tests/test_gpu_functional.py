@@ -34,7 +34,7 @@ def _product(shape, cell):
 SHAPES = [(24, 24, 24), (20, 18, 25), (16, 15, 12)]
 
 
-@pytest.mark.parametrize('version', ['FMT', 'WBII', 'aWBII', 'MFMT', 'aFMT'])
+@pytest.mark.parametrize('version', ['FMT', 'WBII', 'aWBII', 'MFMT', 'aFMT', 'tFMT', 'atWBII', 'tMFMT', 'atFMT'])
 @pytest.mark.parametrize('shape', SHAPES)
 def test_hard_sphere_derive_and_value(version, shape):
     import torch
@@ -58,7 +58,7 @@ def test_hard_sphere_derive_and_value(version, shape):
     # of n3 moves c by ~1e-9 relative (test_oracle_conditioning shows it on the
     # CPU).  There the bound is on beta*|dc|, the relative error it puts on the
     # Boltzmann factor exp(-beta c) of the Euler-Lagrange update.
-    if version in ('FMT', 'aFMT'):
+    if version in ('FMT', 'aFMT', 'tFMT', 'atFMT'):
         tol = 1e-12 * scale
     else:
         tol = 1e-8 * U.boltzmann * T
@@ -70,25 +70,35 @@ def test_hard_sphere_derive_and_value(version, shape):
     assert abs(F - F_ref) <= 1e-11 * abs(F_ref)
 
 
-@pytest.mark.parametrize('shape', SHAPES)
-def test_weighted_densities(shape):
+@pytest.mark.parametrize('version', ['WBII', 'atWBII'])
+@pytest.mark.parametrize('shape', SHAPES + [(18, 16, 27)])
+def test_weighted_densities(shape, version):
     import torch
     from cmmdft_amd.functionals import HardSphereFunctional
     host = W.host_case1()
     og = O.Grid(host.rvecs, shape)
     rho, _ = _density(og, host)
     R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
-    ohs = O.HardSphere(og, R, 'WBII', T)
+    ohs = O.HardSphere(og, R, version, T)
     krho = og.fft(rho)
     n3_ref = og.ifft(krho * ohs.w['scalar'][3])
     n2_ref = og.ifft(krho * ohs.w['scalar'][2])
     nv2_ref = og.ifftn(krho[..., None] * ohs.w['vector'][1])
-    hs = HardSphereFunctional(_product(shape, host.rvecs), R, 'WBII')
+    hs = HardSphereFunctional(_product(shape, host.rvecs), R, version)
     hs.set_temperature(T, R)
     n = hs.weighted_densities_dev(torch.from_numpy(rho).cuda()).cpu().numpy()
-    for got, ref in [(n[0], n2_ref), (n[1], n3_ref), (n[2], nv2_ref[..., 0]), (n[3], nv2_ref[..., 1]),
-                     (n[4], nv2_ref[..., 2])]:
-        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    pairs = [(n[0], n2_ref), (n[1], n3_ref), (n[2], nv2_ref[..., 0]), (n[3], nv2_ref[..., 1]),
+             (n[4], nv2_ref[..., 2])]
+    if 't' in version:
+        assert n.shape[0] == 12
+        # n_t xx, xy, xz, yy, yz, zz: the scale is that of n2 (|w_t| <= w2 bound)
+        pairs += [(n[6 + c], og.ifft(krho * ohs.w['tensor'][c])) for c in range(6)]
+        tscale = np.abs(n2_ref).max()
+    else:
+        assert n.shape[0] == 6
+    for i, (got, ref) in enumerate(pairs):
+        scale = np.abs(ref).max() if i < 5 else tscale
+        assert np.abs(got - ref).max() <= 1e-12 * scale, i
 
 
 def _eos(kind, R):

@@ -104,6 +104,33 @@ def test_rosenfeld_anderson_vs_oracle_picard():
     assert abs(N - Nref) <= 1e-8 * Nref
 
 
+@pytest.mark.parametrize('version', ['atWBII', 'tFMT'])
+def test_tensor_anderson_vs_oracle_picard(version):
+    """The tensor variants (atWBII is the reference's default hard-sphere
+    functional): GPU Anderson against the oracle's static Picard."""
+    from cmmdft_amd.solver import Anderson, grand_potential
+    host = W.host_case1()
+    shape = (20, 20, 20)
+    og, parts, beta, lam, V = oracle_system(host, shape, version)
+    mu = mu_at(10.0)
+    rho0 = O.initial_density(og, mu, beta, lam, V=V)
+    s = O.Picard(og, parts, beta, lam, nsteps=5000, alpha_mix=0.05, method='static', criterion='RIUE',
+                 threshold=1e-13)
+    Nref, rho_ref, conv = s.solve(mu, rho0)
+    assert conv
+    omega_ref = s.omega(rho_ref, og.fft(rho_ref))
+    prog = product_program(host, shape, version)
+    prog.set_solver(Anderson(prog, nsteps=1000, method='anderson', m=5, damping=0.1, criterion='RIUE',
+                             threshold=1e-13))
+    N, rho, conv = prog.solve(mu)
+    assert conv
+    rel, mism = rho_rel_err(rho, rho_ref)
+    assert rel <= 1e-8, rel
+    assert abs(N - Nref) <= 1e-8 * Nref
+    om = grand_potential(prog.fener, mu, rho)
+    assert abs(om - omega_ref) <= 1e-8 * abs(omega_ref), (om, omega_ref)
+
+
 def test_consecutive_solves_do_not_leak_state():
     """Two solves at different chemical potentials on one program: the second
     must not reuse the first one's device state (fugacity, history, damping),

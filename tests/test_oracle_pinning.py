@@ -129,14 +129,16 @@ def _uniform(version, rho_b, shape=(8, 8, 8)):
 def test_bulk_limits_fmt(eta):
     rho_b = eta / (4 * np.pi * R ** 3 / 3)
     # Rosenfeld FMT -> Percus-Yevick compressibility route
-    g, hs, rho = _uniform('FMT', rho_b)
-    phi = hs.value(g.fft(rho), local=True) * BETA
-    f_py = -np.log(1 - eta) + 1.5 * eta * (2 - eta) / (1 - eta) ** 2
-    assert np.allclose(phi, rho_b * f_py, rtol=1e-12)
-    mu_py = -np.log(1 - eta) + (14 * eta - 13 * eta ** 2 + 5 * eta ** 3) / (2 * (1 - eta) ** 3)
-    assert np.allclose(hs.derive(g.fft(rho)) * BETA, mu_py, rtol=1e-11)
+    # (the tensor weights vanish at k = 0, so the 't' variants share the limit)
+    for v in ['FMT', 'tFMT']:
+        g, hs, rho = _uniform(v, rho_b)
+        phi = hs.value(g.fft(rho), local=True) * BETA
+        f_py = -np.log(1 - eta) + 1.5 * eta * (2 - eta) / (1 - eta) ** 2
+        assert np.allclose(phi, rho_b * f_py, rtol=1e-12), v
+        mu_py = -np.log(1 - eta) + (14 * eta - 13 * eta ** 2 + 5 * eta ** 3) / (2 * (1 - eta) ** 3)
+        assert np.allclose(hs.derive(g.fft(rho)) * BETA, mu_py, rtol=1e-11), v
     # White-Bear II and MFMT -> Carnahan-Starling
-    for v in ['WBII', 'MFMT', 'aWBII']:
+    for v in ['WBII', 'MFMT', 'aWBII', 'atWBII', 'tMFMT']:
         g, hs, rho = _uniform(v, rho_b)
         phi = hs.value(g.fft(rho), local=True) * BETA
         f_cs = (4 * eta - 3 * eta ** 2) / (1 - eta) ** 2
@@ -199,7 +201,7 @@ def _smooth_density(shape, seed=4):
     return g, rho
 
 
-@pytest.mark.parametrize('version', ['FMT', 'WBII', 'aWBII'])
+@pytest.mark.parametrize('version', ['FMT', 'WBII', 'aWBII', 'tFMT', 'atWBII', 'tMFMT'])
 def test_derive_is_functional_derivative(version):
     """[F(rho + h e_j) - F(rho - h e_j)] / (2 h dr) = c_j."""
     g, rho = _smooth_density((12, 12, 12))
@@ -367,3 +369,67 @@ def test_wbii_formulas_are_ill_conditioned_at_low_n3():
     rel = np.abs(c1 - c0).max() / np.abs(c0).max()
     assert rel > 1e-12            # far above the 1e-17 perturbation
     assert np.abs(c1 - c0).max() < 1e-8 * U.boltzmann * T
+
+
+def _sphere_average(fn, nt=96, nphi=192):
+    """Average of fn(n) over unit vectors n: Gauss-Legendre in cos(theta),
+    uniform in phi (exact for the band-limited integrands used here)."""
+    x, wx = np.polynomial.legendre.leggauss(nt)
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    st = np.sqrt(1 - x ** 2)
+    n = np.stack([st[:, None] * np.cos(ph)[None, :], st[:, None] * np.sin(ph)[None, :],
+                  np.broadcast_to(x[:, None], (nt, nphi))], axis=-1)
+    vals = fn(n)
+    return np.tensordot(wx, vals.mean(axis=1), axes=(0, 0)) / 2
+
+
+def test_tensor_weight_is_traceless_shell_dyad():
+    """The 't' weights equal the Fourier transform of the traceless part of
+    delta(R - r) r_i r_j / r^2: computed here by quadrature over the sphere at
+    a few wave vectors, and as -4 pi R^2 j_2(kR) (k_i k_j / k^2 - delta_ij / 3)
+    with scipy's spherical Bessel function, on the Lanczos-damped grid."""
+    # quadrature: FT of the shell dyad at q = k R
+    for q in [np.array([0.7, -0.2, 0.4]), np.array([0.0, 2.3, 0.0]), np.array([1.9, 1.1, -2.6])]:
+        qn = np.linalg.norm(q)
+        dyad = _sphere_average(lambda n: n[..., :, None] * n[..., None, :] * np.exp(-1j * n @ q)[..., None, None])
+        tl = dyad - np.trace(dyad) / 3 * np.eye(3)
+        ref = -sp.spherical_jn(2, qn) * (np.outer(q, q) / qn ** 2 - np.eye(3) / 3)
+        assert np.abs(tl - ref).max() < 1e-13
+    g = O.Grid(np.diag([14.0, 11.0, 17.0]), (10, 8, 12))
+    w = O.hs_weights(g, R, 'tWBII')
+    k = g.kpoints[..., :3]
+    K = g.kpoints[..., 3]
+    sl = g.sigma_lanczos
+    nz = K > 0
+    comps = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for c, (i, j) in enumerate(comps):
+        ref = np.zeros_like(K)
+        ref[nz] = (-4 * np.pi * R ** 2 * sp.spherical_jn(2, K[nz] * R) * sl[nz]
+                   * (k[..., i][nz] * k[..., j][nz] / K[nz] ** 2 - (i == j) / 3))
+        assert np.abs(w['tensor'][c] - ref).max() < 1e-12 * 4 * np.pi * R ** 2, (i, j)
+        assert np.all(w['tensor'][c][~nz] == 0)
+    # the six weights sum to a traceless tensor
+    assert np.abs(w['tensor'][0] + w['tensor'][3] + w['tensor'][5]).max() < 1e-13 * 4 * np.pi * R ** 2
+
+
+def test_tensor_phi3_is_tarazona_form():
+    """With a traceless n_t, the oracle's Phi_3 numerator (n2^3 - 3 n2 v.v +
+    9/2 (v.T.v - tr T^3)) equals Tarazona's form with the full tensor
+    T = n_t + n2 I / 3 (Phys. Rev. Lett. 84, 694 (2000)):
+    9/2 (v.T.v - n2 v.v - tr T^3 + n2 tr T^2), by matrix products."""
+    rng = np.random.default_rng(4)
+    for _ in range(5):
+        A = rng.standard_normal((3, 3))
+        Tf = A + A.T
+        n2 = np.trace(Tf)
+        v = rng.standard_normal(3)
+        Tt = Tf - n2 / 3 * np.eye(3)
+        tara = 4.5 * (v @ Tf @ v - n2 * v @ v - np.trace(Tf @ Tf @ Tf) + n2 * np.trace(Tf @ Tf))
+        xx, xy, xz, yy, yz, zz = Tt[0, 0], Tt[0, 1], Tt[0, 2], Tt[1, 1], Tt[1, 2], Tt[2, 2]
+        tr2 = np.trace(Tt @ Tt)
+        tr3 = (xx ** 3 + yy ** 3 + zz ** 3 + 3 * (xx * xy * xy + xx * xz * xz + yy * xy * xy + yy * yz * yz
+               + zz * xz * xz + zz * yz * yz) + 6 * xy * xz * yz)
+        assert np.isclose(tr3, np.trace(Tt @ Tt @ Tt), rtol=1e-12)
+        d = dict(n2=np.array(n2), nv2=v[None, :], xi=None, nt=[xx, xy, xz, yy, yz, zz, tr2, tr3])
+        q = O.HardSphere._q3(d, 'tFMT')
+        assert np.isclose(float(np.squeeze(q)), tara, rtol=1e-12, atol=1e-12)
