@@ -25,6 +25,7 @@ S_PIC_NNEW, S_PIC_IUE, S_PIC_DMAX, S_PIC_NONFIN = 6, 7, 8, 9
 S_AND_NONFIN, S_GRAM = 6, 7
 S_PN, S_PIUE, S_PDMAX, S_PNONFIN, S_PFID, S_PEXT = 16, 17, 18, 19, 20, 21
 S_ADJ_PQ, S_ADJ_RR, S_ADJ_LAM = 24, 25, 27
+S_FMFA = 28
 
 
 class CmmFunctional(C.Structure):
@@ -45,6 +46,11 @@ EXPORTS = {
     'cmm_plan_set_tiles': ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     'cmm_plan_get_info': ([C.c_void_p, C.c_void_p, C.c_int], C.c_int),
     'cmm_plan_set_radix': ([C.c_void_p, C.c_int, C.c_int, C.c_void_p], C.c_int),
+    'cmm_plan_set_mfa': ([C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_mfa_lj_potential': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_fold': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_mfa_table': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p], C.c_int),
     'cmm_plan_get_radix': ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p], C.c_int),
     'cmm_plan_set_active': ([C.c_void_p, C.c_void_p], C.c_int),
     'cmm_set_functional': ([C.c_void_p, C.POINTER(CmmFunctional)], C.c_int),
@@ -209,7 +215,26 @@ class Plan:
 
     def set_functional(self, f: CmmFunctional):
         self._func = f
+        self._mfa = None
         check(self.lib.cmm_set_functional(self.h, C.byref(f)))
+
+    def set_mfa(self, kd):
+        """Mean-field kernel table (complex half spectrum from ``mfa_table``)
+        or None; kept referenced while set."""
+        nx, ny, nz = self.npoints
+        if kd is not None and kd.numel() * kd.element_size() != 16 * nx * ny * (nz // 2 + 1):
+            raise ValueError('set_mfa: table must hold nx x ny x (nz/2+1) complex values')
+        check(self.lib.cmm_plan_set_mfa(self.h, _ptr(kd)))
+        self._mfa = kd
+
+    @property
+    def has_mfa(self):
+        return getattr(self, '_mfa', None) is not None
+
+    def mfa_table(self, uhat, reps, rvecs_sc, lanczos, scale, kd):
+        rv = np.ascontiguousarray(np.asarray(rvecs_sc, dtype=np.float64).reshape(9))
+        check(self.lib.cmm_mfa_table(self.h, _ptr(uhat), _ints([int(r) for r in reps]),
+                                     rv.ctypes.data_as(C.c_void_p), int(bool(lanczos)), float(scale), _ptr(kd)))
 
     def set_instances(self, fug, alpha):
         fug = np.ascontiguousarray(np.broadcast_to(np.asarray(fug, dtype=np.float64), (self.nbatch,)))
@@ -377,3 +402,33 @@ def lj_grid_grad(npoints, rvecs, atoms_dev, types_dev, natom, ntypes, sigma_f, e
                                float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(w), nw, _ptr(work),
                                _ptr(grad), C.c_void_p(s.cuda_stream)))
     return grad
+
+
+def mfa_lj_potential(ns, rvecs, reps, sigma, epsilon, rmin, limit, cutoff, stream=None):
+    """Device tensor (ns / reps): the mean-field LJ potential on the grid of
+    ``ns`` points of the cell ``rvecs``, folded over ``reps`` images
+    (cutoff None: no cutoff)."""
+    torch = _torch()
+    lib = load()
+    n = [int(a) // int(r) for a, r in zip(ns, reps)]
+    out = torch.empty(n, dtype=torch.float64, device='cuda')
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_mfa_lj_potential(_ints([int(a) for a in ns]), rv.ctypes.data_as(C.c_void_p),
+                                   _ints([int(r) for r in reps]), float(sigma), float(epsilon), float(rmin),
+                                   float(limit), float(cutoff if cutoff is not None else 1.0),
+                                   int(cutoff is not None), _ptr(out), C.c_void_p(s.cuda_stream)))
+    return out
+
+
+def fold(field, reps, stream=None):
+    """Device tensor: ``field`` (3-D, device) summed over its reps images."""
+    torch = _torch()
+    lib = load()
+    ns = [int(a) for a in field.shape]
+    n = [a // int(r) for a, r in zip(ns, reps)]
+    out = torch.empty(n, dtype=torch.float64, device=field.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_fold(_ints(ns), _ints([int(r) for r in reps]), _ptr(field.contiguous()), _ptr(out),
+                       C.c_void_p(s.cuda_stream)))
+    return out

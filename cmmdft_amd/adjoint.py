@@ -29,6 +29,16 @@ from .solver import _Engine
 __all__ = ['AdjointSolver', 'BatchedAdjoint', 'uptake', 'set_host_parameters', 'host_parameters']
 
 
+def _check_symmetric(fener):
+    """The conjugate-gradient adjoint needs a symmetric Hessian of the
+    excess free energy.  The reference's mean-field convolution keeps one
+    factor of its transform phase exp(i pi scalprod), which displaces the
+    pair potential, so its operator is not symmetric: not supported here."""
+    if 'MFA' in fener.part_names:
+        raise NotImplementedError('adjoint with the mean-field (MFA) part: its convolution operator is not '
+                                  'symmetric (the reference phase displaces the pair potential)')
+
+
 class AdjointSolver:
     """Linear response of a converged density of ``program`` (its grid and
     free energy)."""
@@ -37,6 +47,7 @@ class AdjointSolver:
         self.program = program
         self.grid = program.grid
         self.fener = program.fener
+        _check_symmetric(self.fener)
         self.tol = tol
         self.maxiter = maxiter
         self.iters = 0
@@ -151,6 +162,7 @@ class BatchedAdjoint:
         self.program = program
         self.grid = program.grid
         self.fener = program.fener
+        _check_symmetric(self.fener)
         self.tol = tol
         self.maxiter = maxiter
         self.iters = 0

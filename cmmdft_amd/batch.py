@@ -38,7 +38,7 @@ import numpy as np
 from . import units as U
 from ._device import torch_mod, as_device
 from . import _lib as L
-from .functionals import build_functional
+from .functionals import configure_plan, device_parts
 from .solver import Anderson
 
 __all__ = ['BatchedAnderson', 'UnevenBatch', 'footprint_bytes']
@@ -92,17 +92,10 @@ class BatchedAnderson:
         torch = torch_mod()
         fener, grid = self.fener, self.grid
         B = len(self.chempots)
-        hs = wda = None
+        hs, wda, mfa, eps = device_parts(fener.parts)
         V = None
-        for part in fener.parts:
-            if part.name == 'HardSphere':
-                hs = part
-            elif part.name == 'WDA-V':
-                wda = part
-            elif part.name == 'ExtPot':
-                V = part.potential_dev.reshape(-1) if V is None else V + part.potential_dev.reshape(-1)
-            else:
-                raise NotImplementedError('functional part %r has no device implementation' % part.name)
+        for part in eps:
+            V = part.potential_dev.reshape(-1) if V is None else V + part.potential_dev.reshape(-1)
         self.beta = fener.beta
         self.lam3 = fener.wavelength ** 3
         self.V = V.contiguous() if V is not None else None
@@ -122,11 +115,10 @@ class BatchedAnderson:
                     rho_ring=torch.empty((self.m + 1, nb, n), dtype=torch.float64, device='cuda'),
                     res_ring=torch.zeros((self.m, nb, n), dtype=torch.float64, device='cuda')))
                 first += nb
-        func = build_functional(hs=hs, wda=wda, kT=1.0 / self.beta)
         for g in self._groups:
             mu = self.chempots[g['first']:g['first'] + g['n']]
             g['mu'] = mu
-            g['plan'].set_functional(func)
+            configure_plan(g['plan'], hs, wda, mfa, 1.0 / self.beta)
             g['plan'].set_instances(np.exp(self.beta * mu) / self.beta / self.lam3, np.zeros(g['n']))
 
     def _iteration(self, s):

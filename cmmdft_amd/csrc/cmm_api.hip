@@ -4,6 +4,7 @@
 #include "cmm_kernels.hip"
 #include "cmm_lj.hip"
 #include "cmm_adjoint.hip"
+#include "cmm_mfa.hip"
 #include "../../include/cmmdft_hip.h"
 
 #include <algorithm>
@@ -213,7 +214,8 @@ int cmm_plan_create(int nx, int ny, int nz, const double* rvecs, int nbatch, cmm
   default_tiles(p);
   // workspace: rz (1) + F (6) half spectra per instance
   size_t half_bytes = (size_t)p->nhalf * sizeof(double2) * nbatch;
-  long long maxblocks = zblocks(p, 2);
+  // (x-pass reductions have one partial per tile: at most ny * nzh)
+  long long maxblocks = std::max(zblocks(p, 2), (long long)ny * p->nzh);
   p->pstride = 16 * maxblocks;
   hipError_t e = hipSuccess;
   e = (e == hipSuccess) ? hipMalloc(&p->rz, half_bytes) : e;
@@ -376,6 +378,94 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
   return CMM_OK;
 }
 
+int cmm_plan_set_mfa(cmm_plan* p, const double* kd) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (kd && !p->has_func) return fail(CMM_E_ARG, "set the functional before the mean-field table");
+  p->fp.kmfa = reinterpret_cast<const double2*>(kd);
+  return CMM_OK;
+}
+
+int cmm_mfa_lj_potential(const int* ns, const double* rvecs, const int* reps, double sigma, double epsilon,
+                         double rmin, double limit, double cutoff, int use_cutoff, double* out, void* stream) {
+  if (!ns || !rvecs || !reps || !out) return fail(CMM_E_ARG, "null argument");
+  int n[3];
+  for (int a = 0; a < 3; ++a) {
+    if (ns[a] < 1 || reps[a] < 1 || ns[a] % reps[a]) return fail(CMM_E_ARG, "grid points must be a multiple of reps");
+    n[a] = ns[a] / reps[a];
+  }
+  if (!(sigma > 0.0)) return fail(CMM_E_ARG, "sigma must be positive");
+  Cell9 c;
+  for (int k = 0; k < 9; ++k) c.rv[k] = rvecs[k];
+  MfaLJ lj;
+  lj.sigma = sigma;
+  lj.eps4 = 4.0 * epsilon;
+  lj.rmin = rmin;
+  lj.limit = limit;
+  lj.cutoff = cutoff;
+  lj.use_cutoff = use_cutoff ? 1 : 0;
+  const double rc6 = std::pow(sigma / cutoff, 6.0);
+  lj.shift = use_cutoff ? 4.0 * epsilon * (rc6 * rc6 - rc6) : 0.0;
+  const long long npts = (long long)n[0] * n[1] * n[2];
+  hipLaunchKernelGGL(k_mfa_lj, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream, out, n[0],
+                     n[1], n[2], reps[0], reps[1], reps[2], c, lj);
+  LAUNCHCHK("k_mfa_lj");
+  return CMM_OK;
+}
+
+int cmm_fold(const int* ns, const int* reps, const double* in, double* out, void* stream) {
+  if (!ns || !reps || !in || !out) return fail(CMM_E_ARG, "null argument");
+  int n[3];
+  for (int a = 0; a < 3; ++a) {
+    if (ns[a] < 1 || reps[a] < 1 || ns[a] % reps[a]) return fail(CMM_E_ARG, "grid points must be a multiple of reps");
+    n[a] = ns[a] / reps[a];
+  }
+  const long long npts = (long long)n[0] * n[1] * n[2];
+  hipLaunchKernelGGL(k_fold, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in, out, n[0],
+                     n[1], n[2], reps[0], reps[1], reps[2]);
+  LAUNCHCHK("k_fold");
+  return CMM_OK;
+}
+
+int cmm_mfa_table(cmm_plan* p, const double* uhat, const int* reps, const double* rvecs_sc, int lanczos,
+                  double scale, double* kd) {
+  if (!p || !uhat || !reps || !rvecs_sc || !kd) return fail(CMM_E_ARG, "null argument");
+  const int n[3] = {p->nx, p->ny, p->nz};
+  const int nh[3] = {p->nx, p->ny, p->nzh};
+  std::vector<double2> ph(nh[0] + nh[1] + nh[2]);
+  int off = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (reps[a] < 1) return fail(CMM_E_ARG, "reps must be positive");
+    // supercell axis a: N_sc = reps n, spacing from the column norm of the
+    // supercell vectors (system.py:262-266); the grid's wave vector m is the
+    // supercell's reps m, fftfreq index reps fm (system.py:284-291)
+    const long long Nsc = (long long)reps[a] * n[a];
+    const double col = std::sqrt(rvecs_sc[0 * 3 + a] * rvecs_sc[0 * 3 + a] + rvecs_sc[1 * 3 + a] * rvecs_sc[1 * 3 + a] +
+                                 rvecs_sc[2 * 3 + a] * rvecs_sc[2 * 3 + a]);
+    const double h = col / (double)Nsc;
+    const double val = 1.0 / ((double)Nsc * h);
+    for (int m = 0; m < nh[a]; ++m) {
+      const long long fm = (m <= (n[a] - 1) / 2) ? m : m - n[a];
+      const double k = 2.0 * M_PI * ((double)(reps[a] * fm) * val);
+      const double theta = M_PI * (k * h * (double)Nsc);
+      ph[off + m] = make_double2(std::cos(theta), std::sin(theta));
+    }
+    off += nh[a];
+  }
+  double2* dph = nullptr;
+  HIPCHK(hipMalloc(&dph, ph.size() * sizeof(double2)));
+  hipError_t e = hipMemcpyAsync(dph, ph.data(), ph.size() * sizeof(double2), hipMemcpyHostToDevice, p->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mfa_table, dim3((unsigned)((p->nhalf + 255) / 256)), dim3(256), 0, p->stream,
+                       reinterpret_cast<const double2*>(uhat), reinterpret_cast<double2*>(kd), p->g, dph,
+                       lanczos ? 1 : 0, scale);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+  (void)hipFree(dph);
+  if (e != hipSuccess) return fail(CMM_E_HIP, std::string("k_mfa_table: ") + hipGetErrorString(e));
+  return CMM_OK;
+}
+
 int cmm_set_instances(cmm_plan* p, const double* fug, const double* alpha) {
   if (!p || !fug || !alpha) return fail(CMM_E_ARG, "null argument");
   std::vector<InstDev> iv(p->nb);
@@ -424,51 +514,64 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
 }
 
 // x passes of the functional chain (see k_xweights / k_xback): tiles of
-// nx * tb columns with all nf fields in LDS ((nf + 1) nx tb complex values
-// plus the static twiddle table, at most 64 KB together)
-static int xpass_tb(cmm_plan* p) {
-  const int lim = (65536 - CMM_MAX_N * 16) / (16 * (p->nf + 1) * p->nx);
+// nx * tb columns with all fields in LDS ((nl + 1) nx tb complex values for
+// nl fields, plus the static twiddle table, at most 64 KB together)
+static int xpass_tb(cmm_plan* p, int nl) {
+  const int lim = (65536 - CMM_MAX_N * 16) / (16 * (nl + 1) * p->nx);
   return std::max(1, std::min(p->tbx, lim));
 }
 
-template <int NF>
+template <int NF, bool MFA>
 static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   const int T = p->nx * tb;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL(k_xweights<NF>, grid, dim3(NTX), (size_t)(NF + 1) * T * sizeof(double2), p->stream, p->rz,
-                     p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
+  hipLaunchKernelGGL((k_xweights<NF, MFA>), grid, dim3(NTX), (size_t)(NF + 1) * T * sizeof(double2), p->stream,
+                     p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
                      make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)p->nzh), per,
-                     ntiles, p->active);
+                     ntiles, p->active, red_of(p));
 }
 
 static int launch_xweights(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
-  const int tb = xpass_tb(p);
+  const int tb = xpass_tb(p, p->nf);
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
-  if (p->nf == 12) launch_xweights_nf<12>(p, tb, ntiles, per);
-  else launch_xweights_nf<6>(p, tb, ntiles, per);
+  const bool mfa = p->fp.kmfa != nullptr;
+  if (p->nf == 12) {
+    if (mfa) launch_xweights_nf<12, true>(p, tb, ntiles, per);
+    else launch_xweights_nf<12, false>(p, tb, ntiles, per);
+  } else {
+    if (mfa) launch_xweights_nf<6, true>(p, tb, ntiles, per);
+    else launch_xweights_nf<6, false>(p, tb, ntiles, per);
+  }
   LAUNCHCHK("k_xweights");
-  return CMM_OK;
+  return mfa ? launch_reduce(p, ntiles, 1, 0u, S_FMFA) : CMM_OK;
 }
 
-template <int NF>
+template <int NF, bool MFA>
 static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
+  constexpr int NL = NF + (MFA ? 1 : 0);
   const int T = p->nx * tb;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL(k_xback<NF>, grid, dim3(NTX), (size_t)(NF + 1) * T * sizeof(double2), p->stream, p->F,
+  hipLaunchKernelGGL((k_xback<NF, MFA>), grid, dim3(NTX), (size_t)(NL + 1) * T * sizeof(double2), p->stream, p->F,
                      (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb, make_fastdiv((unsigned)tb),
-                     make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh), per,
-                     ntiles, p->active);
+                     make_fastdiv((unsigned)(NL * tb)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh), per,
+                     ntiles, p->active, p->rz, p->nhalf);
 }
 
 static int launch_xback(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
-  const int tb = xpass_tb(p);
+  const bool mfa = p->fp.kmfa != nullptr;
+  const int tb = xpass_tb(p, p->nf + (mfa ? 1 : 0));
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
-  if (p->nf == 12) launch_xback_nf<12>(p, tb, ntiles, per);
-  else launch_xback_nf<6>(p, tb, ntiles, per);
+  if (p->nf == 12) {
+    if (mfa) launch_xback_nf<12, true>(p, tb, ntiles, per);
+    else launch_xback_nf<12, false>(p, tb, ntiles, per);
+  } else {
+    if (mfa) launch_xback_nf<6, true>(p, tb, ntiles, per);
+    else launch_xback_nf<6, false>(p, tb, ntiles, per);
+  }
   LAUNCHCHK("k_xback");
   return CMM_OK;
 }
@@ -953,6 +1056,7 @@ int cmm_batch_reset(cmm_plan* p, const double* mu, double beta, double lam3, dou
   b.tol = tol;
   b.nsteps = nsteps;
   b.nb = nb;
+  b.mfa = p->fp.kmfa != nullptr;
   p->active = p->b_active;
   return CMM_OK;
 }

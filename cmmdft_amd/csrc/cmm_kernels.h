@@ -17,6 +17,7 @@ constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (fi
                             //   picard: N', sum|d|, max|d|, nonfinite
                             //   anderson: nonfinite, Gram row[CMM_MMAX]
 constexpr int S_GRAM = S_NCUR + 4;
+constexpr int S_FMFA = 28;  // (1/2) sum rho c_MFA (mean-field energy / dr)   (x weights pass)
 constexpr int S_PN = 16;    // N', sum|d|, max|d|, nonfinite, ideal', ext'  (density producers)
 
 // A z-pass tile: P pencils (B = P/2 complex sequences per field), S = nf*B
@@ -47,6 +48,7 @@ struct FuncDev {
   double R, D, kT;
   double c_w2, c_w3, inv4piR2, inv4piR;
   double c_tor;     // 3 / R: tensor weight B = w2 - (3/R) w3
+  const double2* kmfa;   // mean-field (MFA) kernel, half spectrum [nx][ny][nz/2+1]; null when off
 };
 
 struct InstDev {
@@ -114,6 +116,7 @@ struct BatchDev {
   double beta, kT, lam3, dr, tol;
   int nsteps;
   int nb;
+  int mfa;        // the functional has the mean-field part (energy in S_FMFA)
 };
 
 }  // namespace cmm

@@ -391,11 +391,15 @@ constexpr int NTX = 512;
 // inverse x FFT of all of them -> F[0..NF).  The 1/N of the inverse 3-D
 // transform is folded into the weights.  NF = 6 (fields n2, n3, nv2 x/y/z,
 // rho_bar) or 12 (plus the six tensor components).
-template <int NF>
+// MFA: also the mean-field energy (1/2) sum_x rho c_MFA = (1/2) sum_k
+// |rho^(k)|^2 K(k) over the full spectrum (K Hermitian, so the sum is real):
+// on the half spectrum the planes kz = 0 and kz = Nyquist count once, the
+// others twice.  Block partials are indexed by tile (ntiles of them).
+template <int NF, bool MFA>
 __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
                                                  long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
                                                  FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
-                                                 const int* __restrict__ active) {
+                                                 const int* __restrict__ active, RedDev red) {
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fx);
@@ -430,6 +434,7 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
   }
   __syncthreads();
   lds_fft<-1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
+  double emfa = 0.0;
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
@@ -437,8 +442,19 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
     const int wz = (int)bb - wy * g.nzh;
     const KW w = kweights(g, fp, x, wy, wz);
     const double2 a = A[i];
+    if (MFA && b < tbv) {
+      const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
+      emfa += half * (a.x * a.x + a.y * a.y) * fp.kmfa[(long long)x * nb + bb].x;
+    }
 #pragma unroll
     for (int f = 0; f < NF; ++f) B[x * S6 + f * TB + b] = cscale(apply_w(g, w, x, wy, wz, f, a, -1), g.inv_npts);
+  }
+  if (MFA) {
+    __shared__ double sh[CMM_RED_SH];
+    double acc[1] = {emfa};
+    const int ops[1] = {0};
+    block_reduce<1>(acc, ops, sh);
+    grid_reduce_store<1>(acc, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
   }
   __syncthreads();
   lds_fft<1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
@@ -455,10 +471,14 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
 // back-convolution: the NF derivative fields -> x FFT of all of them ->
 // weighted sum -> inverse x FFT -> F[0] (in place: the block reads its whole
 // tile of every field before it writes field 0's tile).
-template <int NF>
+// MFA: the density's spectrum (rz, z and y transformed) is loaded as field NF
+// and transformed with the others; K rho^ joins the weighted sum.
+template <int NF, bool MFA>
 __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
                                               FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fT, FastDiv fnzh,
-                                              int per, long long ntiles, const int* __restrict__ active) {
+                                              int per, long long ntiles, const int* __restrict__ active,
+                                              const double2* __restrict__ rz, long long rzs) {
+  constexpr int NL = NF + (MFA ? 1 : 0);    // fields held in LDS
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fx);
@@ -467,26 +487,28 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
   const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
-  const int S6 = NF * TB;
-  double2* B = sm;             // [nx][NF TB]
-  double2* A = sm + NF * T;    // [nx][TB]
+  const int S6 = NL * TB;
+  double2* B = sm;             // [nx][NL TB]
+  double2* A = sm + NL * T;    // [nx][TB]
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = F + inst * Fs + b0;
-  for (int i0 = threadIdx.x; i0 < NF * T; i0 += LU * blockDim.x) {
+  const double2* rsrc = MFA ? rz + inst * rzs + b0 : nullptr;
+  for (int i0 = threadIdx.x; i0 < NL * T; i0 += LU * blockDim.x) {
     double2 v[LU];
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
-      const int i = min(i0 + u * (int)blockDim.x, NF * T - 1);
+      const int i = min(i0 + u * (int)blockDim.x, NL * T - 1);
       const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
       const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
-      v[u] = src[f * Ffs + (long long)x * nb + min(b, tbv - 1)];
+      const long long o = (long long)x * nb + min(b, tbv - 1);
+      v[u] = (MFA && f == NF) ? rsrc[o] : src[f * Ffs + o];
     }
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = i0 + u * (int)blockDim.x;
-      if (i < NF * T) {
+      if (i < NL * T) {
         const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
         const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
         B[x * S6 + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
@@ -504,6 +526,7 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, f, B[x * S6 + f * TB + b], 1), g.inv_npts));
+    if (MFA) acc = cadd(acc, cmul(B[x * S6 + NF * TB + b], fp.kmfa[(long long)x * nb + bb]));
     A[i] = acc;
   }
   __syncthreads();
@@ -1254,7 +1277,8 @@ __global__ void __launch_bounds__(64) k_anderson_coef(AndDev ad, const int* __re
 // iteration (Solver._check_convergence with RIUE, solver.py:167-234, and the
 // grand-potential test of Anderson.update_rho, solver.py:567-574).  Reads the
 // scalars of the density just produced (N', sum|drho|, non-finite count,
-// ideal/external sums) and of its functional chain (sum Phi, sum f_ex).
+// ideal/external sums) and of its functional chain (sum Phi, sum f_ex and the
+// mean-field energy).
 // init = 1 only records Omega of the seeded densities.
 __global__ void k_batch_check(BatchDev b, RedDev red, int new_slot, int init) {
   const int inst = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1262,7 +1286,7 @@ __global__ void k_batch_check(BatchDev b, RedDev red, int new_slot, int init) {
   const double* s = red.scal + inst * NSCAL;
   const double N = s[S_PN];
   const double om = s[S_PN + 4] * b.dr * b.kT - b.mu[inst] * N * b.dr + s[S_PHI] * b.dr * b.kT + s[S_PHI + 1] * b.dr +
-                    s[S_PN + 5] * b.dr;
+                    s[S_PN + 5] * b.dr + (b.mfa ? s[S_FMFA] * b.dr : 0.0);
   b.omega[inst] = om;
   if (init) {
     b.omega_prev[inst] = om;

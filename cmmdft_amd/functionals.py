@@ -20,11 +20,11 @@ import numpy as np
 
 from . import units as U
 from ._device import torch_mod, as_device, like_input
-from ._lib import CmmFunctional, S_PHI, S_FEX, S_N3MAX
+from ._lib import CmmFunctional, S_PHI, S_FEX, S_N3MAX, S_FMFA
 from .system import NanoporousHost, SphericalLJGuest
 
-__all__ = ['FreeEnergy', 'Functional', 'HardSphereFunctional', 'WDAVFunctional', 'ExternalPotential',
-           'density_from', 'build_functional']
+__all__ = ['FreeEnergy', 'Functional', 'HardSphereFunctional', 'WDAVFunctional', 'MFAFunctional',
+           'ExternalPotential', 'density_from', 'build_functional', 'device_parts', 'configure_plan']
 
 FMT_NAMES = ['FMT', 'aFMT', 'tFMT', 'atFMT', 'taFMT']
 MFMT_NAMES = ['MFMT', 'aMFMT', 'tMFMT', 'atMFMT', 'taMFMT']
@@ -92,16 +92,49 @@ class Functional:
         pass
 
 
+def device_parts(parts):
+    """(hard sphere, WDA, MFA, [external potentials]) of a list of
+    functional parts; parts with no device implementation raise."""
+    hs = wda = mfa = None
+    eps = []
+    for part in parts:
+        if part.name == 'HardSphere':
+            if hs is not None:
+                raise NotImplementedError('more than one hard-sphere part')
+            hs = part
+        elif part.name == 'WDA-V':
+            if wda is not None:
+                raise NotImplementedError('more than one WDA part on the device')
+            wda = part
+        elif part.name == 'MFA':
+            if mfa is not None:
+                raise NotImplementedError('more than one mean-field part')
+            mfa = part
+        elif part.name == 'ExtPot':
+            eps.append(part)
+        else:
+            raise NotImplementedError('functional part %r has no device implementation' % part.name)
+    return hs, wda, mfa, eps
+
+
+def configure_plan(plan, hs, wda, mfa, kT):
+    """Set a plan's functional: hard sphere and WDA, then the mean-field
+    kernel table (or none)."""
+    plan.set_functional(build_functional(hs=hs, wda=wda, kT=kT))
+    plan.set_mfa(mfa.table() if mfa is not None else None)
+
+
 class _PartEngine:
     """Runs the fused pipeline of one grid with a single part enabled."""
 
     def __init__(self, grid):
         self.grid = grid
 
-    def run(self, func, rho, want_c1=True):
+    def run(self, func, rho, want_c1=True, mfa=None):
         torch = torch_mod()
         plan = self.grid.plan()
         plan.set_functional(func)
+        plan.set_mfa(mfa)
         plan.seed(rho)
         if want_c1:
             plan.derive()
@@ -224,6 +257,148 @@ class WDAVFunctional(Functional):
         return s[S_FEX] * self.grid.dr
 
 
+class MFAFunctional(Functional):
+    """Mean-field attraction between guests, c_MFA = U * rho
+    (functionals.py:770-939).
+
+    The convolution is the reference's ifft(krho kpotential) V, phase
+    included (its phased transforms leave one factor exp(i pi scalprod) in
+    the product); the device keeps its kernel table as a half spectrum (see
+    cmm_mfa_table) and the derivative chain adds it to c_ex.  With tail
+    corrections the pair potential lives on a supercell of ``repetitions``
+    times the grid; the reference samples every repetitions-th supercell
+    wave vector, which is the transform of the potential folded onto the
+    grid: the device folds it in real space.  The potential is generated on
+    the device; ``potential`` (the reference's array on ``self.grid``, the
+    supercell with tail corrections) is materialised on first access.
+    ``kpotential`` is not materialised (None).
+    """
+    name = 'MFA'
+
+    def __init__(self, grid, tailcorrections=False, repetitions=[2, 2, 2]):
+        self.tailcorrections = tailcorrections
+        self.repetitions = repetitions
+        if tailcorrections:
+            self.small_grid = grid
+            self.grid = grid.supercell(repetitions)
+        else:
+            self.grid = grid
+        self.base_grid = grid
+        self.reps = [int(r) for r in repetitions] if tailcorrections else [1, 1, 1]
+        self.kpotential = None
+        self._potential = None      # numpy, on self.grid
+        self._lj = None             # parameters of a potential generated on the device
+        self._fold = None           # device: potential folded onto the base grid
+        self._kd = None             # device: kernel table (half spectrum)
+        self._engine = _PartEngine(grid)
+
+    def copy(self, grid=None):
+        mfa = type(self)(grid if grid is not None else self.base_grid.copy(), self.tailcorrections, self.repetitions)
+        mfa._potential = None if self._potential is None else self._potential.copy()
+        mfa._lj = None if self._lj is None else dict(self._lj)
+        mfa._fold, mfa._kd = self._fold, self._kd
+        return mfa
+
+    def set_temperature(self, temperature, **kwargs):
+        pass
+
+    # -- the pair potential ------------------------------------------------
+    def _lj_field(self, reps):
+        from ._lib import mfa_lj_potential
+        lj = self._lj
+        return mfa_lj_potential(self.grid.npoints, self.grid.cell.rvecs, reps, lj['sigma'], lj['epsilon'], lj['rmin'],
+                                lj['limit'], lj['cutoff'])
+
+    @property
+    def potential(self):
+        if self._potential is None and self._lj is not None:
+            self._potential = self._lj_field([1, 1, 1]).cpu().numpy()
+        return self._potential
+
+    @potential.setter
+    def potential(self, value):
+        self._potential = value
+        self._lj = None
+
+    def generate_potential_lj(self, sigma, epsilon, rmin=None, limit_potential=0, cutoff=None, **kwargs):
+        """U(r) on the (super)grid (functionals.py:878-910); the kernel table
+        takes the Lanczos factor."""
+        if rmin is None:
+            rmin = sigma
+        self._lj = dict(sigma=float(sigma), epsilon=float(epsilon), rmin=float(rmin), limit=float(limit_potential),
+                        cutoff=None if cutoff is None else float(cutoff))
+        self._potential = None
+        self._fold = self._lj_field(self.reps)
+        self._build_table(lanczos=True)
+
+    def generate_potential(self, *args, **kwargs):
+        raise NotImplementedError('MFA potentials from a force field need yaff; use generate_potential_lj')
+
+    def load_potential(self, fn):
+        """(functionals.py:809-812): the kernel table of a loaded potential
+        has no Lanczos factor."""
+        from ._lib import fold
+        pot = np.load(fn)
+        assert tuple(self.grid.npoints) == pot.shape
+        self.potential = pot
+        dev = as_device(pot)
+        self._fold = dev if self.reps == [1, 1, 1] else fold(dev, self.reps)
+        self._build_table(lanczos=False)
+
+    def dump_potential(self, fn):
+        assert self.potential is not None
+        dn = os.path.dirname(fn)
+        if dn and not os.path.exists(dn):
+            os.makedirs(dn)
+        np.save(fn, self.potential)
+
+    def compute_vdw_a(self):
+        """a = (1/2) int U (functionals.py:814-822), over the supercell with
+        tail corrections (the sum of the folded potential)."""
+        if self._fold is None:
+            raise AssertionError('no MFA potential')
+        self.a = 0.5 * float(self._fold.sum()) * self.base_grid.dr
+        return self.a
+
+    def _build_table(self, lanczos):
+        torch = torch_mod()
+        g = self.base_grid
+        nx, ny, nz = g.npoints
+        plan = g.plan()
+        uhat = torch.empty((nx, ny, nz // 2 + 1), dtype=torch.complex128, device='cuda')
+        plan.fft_r2c(self._fold.contiguous(), torch.view_as_real(uhat))
+        kd = torch.empty_like(uhat)
+        plan.mfa_table(torch.view_as_real(uhat), self.reps, self.grid.cell.rvecs, lanczos, g.dr / g.npts,
+                       torch.view_as_real(kd))
+        self._kd = torch.view_as_real(kd)
+
+    def table(self):
+        """Device kernel table (real view of the complex half spectrum)."""
+        if self._kd is None:
+            raise AssertionError('the MFA potential has not been generated or loaded')
+        return self._kd
+
+    # -- functional --------------------------------------------------------
+    def _func(self):
+        # no hard-sphere or WDA part: only the mean-field table acts (kT unused)
+        return build_functional(kT=1.0)
+
+    def derive(self, krho):
+        """c_MFA (functionals.py:912-926)."""
+        rho = density_from(self.base_grid, krho)
+        c1, _ = self._engine.run(self._func(), rho, mfa=self.table())
+        return like_input(c1.reshape(self.base_grid.npoints), krho)
+
+    def value(self, krho, local=False):
+        """(1/2) int rho c_MFA (functionals.py:928-939)."""
+        rho = density_from(self.base_grid, krho)
+        if local:
+            c1, _ = self._engine.run(self._func(), rho, mfa=self.table())
+            return like_input(0.5 * rho * c1.reshape(rho.shape), krho)
+        _, s = self._engine.run(self._func(), rho, want_c1=False, mfa=self.table())
+        return s[S_FMFA] * self.base_grid.dr
+
+
 class ExternalPotential(Functional):
     """External potential on the grid (functionals.py:941-1083), spherical LJ
     guests; the grid sum runs on the device (see extpot.get_external_potential)."""
@@ -306,7 +481,7 @@ class ExternalPotential(Functional):
 class FreeEnergy:
     """Container of functional parts (functionals.py:25-362)."""
 
-    def __init__(self, grid, system, temperature, workdir='.', name_dict=None, overwrite=False):
+    def __init__(self, grid, system, temperature, workdir='.', name_dict=None, overwrite=False, write_files=True):
         self.grid = grid
         self.system = system
         self.temperature = None
@@ -318,6 +493,7 @@ class FreeEnergy:
         self.workdir = Path(workdir)
         self.name_dict = name_dict or {}
         self.overwrite = overwrite
+        self.write_files = write_files
         self.fn_tracking = None
         self.set_temperature(temperature)
 
@@ -367,6 +543,47 @@ class FreeEnergy:
         epot.clamp()
         self.add_part(epot)
         return epot
+
+    def add_mean_field(self, tailcorrections=False, repetitions=[2, 2, 2], cutoff=12 * U.angstrom, **kwargs):
+        """Mean-field attraction of spherical LJ guests
+        (functionals.py:301-337): the pair potential is generated, or
+        loaded from ``workdir/mfa.npy`` when that exists and neither
+        ``overwrite`` nor ``rewrite`` is set, and a generated one is saved
+        there; a free energy made with ``write_files=False`` does no file
+        I/O and always generates."""
+        fn = self.workdir / 'mfa.npy'
+        # as in the reference (functionals.py:322-325), ``repetitions`` is
+        # passed on only when it also appears in kwargs, which a named
+        # parameter never does: the supercell is the default 2 x 2 x 2
+        if 'repetitions' in kwargs:
+            mfa = MFAFunctional(self.grid, tailcorrections=tailcorrections, repetitions=repetitions)
+        else:
+            mfa = MFAFunctional(self.grid, tailcorrections=tailcorrections)
+        if not self.write_files or not os.path.isfile(fn) or self.overwrite or kwargs.get('rewrite', False):
+            if not isinstance(self.system.guest, SphericalLJGuest):
+                raise NotImplementedError('MFA potential generation only implemented for spherical LJ particles')
+            gen = {k: v for k, v in kwargs.items() if k in ('rmin', 'limit_potential')}
+            mfa.generate_potential_lj(self.system.guest.sigma, self.system.guest.epsilon, cutoff=cutoff, **gen)
+            if self.write_files:
+                mfa.dump_potential(str(fn))
+        else:
+            mfa.load_potential(fn)
+        self.add_part(mfa)
+        return mfa
+
+    def add_correlation_wda_lj(self, **kwargs):
+        """Correlation WDA: MBWR - CS - MFA (functionals.py:339-362), with the
+        mean-field a of the MFA part when there is one."""
+        from .eos import ModifiedBenedictWebbRubinEOS, CarnahanStarlingEOS, MFAEOS, SumOfEOS
+        g = self.system.guest
+        a = self.part_dict['MFA'].compute_vdw_a() if 'MFA' in self.part_names else None
+        mbwr = ModifiedBenedictWebbRubinEOS(g.mass, g.sigma, g.epsilon)
+        cs = CarnahanStarlingEOS(g.mass, g.Rhs)
+        mfa = MFAEOS(g.mass, g.sigma, g.epsilon, a=a)
+        corr = WDAVFunctional(self.grid, g.Rhs, SumOfEOS(g.mass, [mbwr, cs, mfa], factors=[1, -1, -1]))
+        corr.set_temperature(self.temperature, g.Rhs)
+        self.add_part(corr)
+        return corr
 
     def track_omega(self, chempot, rho):
         """Grand potential of a density (FreeEnergy.track without file output,

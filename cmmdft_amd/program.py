@@ -55,7 +55,7 @@ class Program:
     def init_free_energy(self, temperature):
         assert self.system is not None and self.grid is not None
         self.fener = FreeEnergy(self.grid, self.system, temperature, workdir=self.workdir,
-                                overwrite=self.overwrite, name_dict=self.name_dict)
+                                overwrite=self.overwrite, name_dict=self.name_dict, write_files=self.write_files)
 
     def set_temperature(self, temperature):
         self.fener.set_temperature(temperature)

@@ -22,7 +22,7 @@ import scipy.optimize as opt
 from . import units as U
 from ._device import torch_mod, as_device, like_input
 from . import _lib as L
-from .functionals import build_functional
+from .functionals import build_functional, configure_plan, device_parts
 
 __all__ = ['Solver', 'Picard', 'Anderson', 'NoSolutionError', 'grand_potential']
 
@@ -38,32 +38,26 @@ class _Engine:
         torch = torch_mod()
         self.grid = grid
         self.fener = fener
-        hs = wda = None
+        hs, wda, mfa, eps = device_parts(fener.parts)
         V = None
-        for part in fener.parts:
-            if part.name == 'HardSphere':
-                if hs is not None:
-                    raise NotImplementedError('more than one hard-sphere part')
-                hs = part
-            elif part.name == 'WDA-V':
-                if wda is not None:
-                    raise NotImplementedError('more than one WDA part on the device')
-                wda = part
-            elif part.name == 'ExtPot':
-                V = part.potential_dev.reshape(-1) if V is None else V + part.potential_dev.reshape(-1)
-            else:
-                raise NotImplementedError('functional part %r has no device implementation' % part.name)
-        self.hs, self.wda = hs, wda
+        for part in eps:
+            V = part.potential_dev.reshape(-1) if V is None else V + part.potential_dev.reshape(-1)
+        self.hs, self.wda, self.mfa = hs, wda, mfa
         self.beta = fener.beta
         self.kT = 1.0 / fener.beta
         self.lam3 = fener.wavelength ** 3
         self.func = build_functional(hs=hs, wda=wda, kT=self.kT)
         self.V = V.contiguous() if V is not None else None
         self.plan = grid.plan()
-        self.plan.set_functional(self.func)
+        configure_plan(self.plan, hs, wda, mfa, self.kT)
         self.dr = grid.dr
         self.npts = grid.npts
         self.dev = torch.device('cuda')
+
+    def fex(self, s):
+        """Excess energy sums outside the hard-sphere Phi (times dr): the WDA
+        f_ex and the mean-field energy."""
+        return s[L.S_FEX] + (s[L.S_FMFA] if self.mfa is not None else 0.0)
 
     def omega(self, mu, N, fid, ext, phi, fex):
         """Grand potential from the summed pieces (solver.py:103-119)."""
@@ -78,7 +72,7 @@ def grand_potential(fener, chempot, rho):
     eng.plan.seed(r, eng.V, False, eng.lam3)
     eng.plan.value()
     s = eng.plan.scalars()[0]
-    return eng.omega(chempot, s[L.S_PN], s[L.S_PFID], s[L.S_PEXT], s[L.S_PHI], s[L.S_FEX])
+    return eng.omega(chempot, s[L.S_PN], s[L.S_PFID], s[L.S_PEXT], s[L.S_PHI], eng.fex(s))
 
 
 class Solver:
@@ -212,7 +206,7 @@ class Picard(Solver):
 
     def _density_omega(self, eng, s):
         """Omega of a density just produced (k_zproduce) and derived."""
-        return eng.omega(self.chempot, s[L.S_PN], s[L.S_PFID], s[L.S_PEXT], s[L.S_PHI], s[L.S_FEX])
+        return eng.omega(self.chempot, s[L.S_PN], s[L.S_PFID], s[L.S_PEXT], s[L.S_PHI], eng.fex(s))
 
     def _value_at(self, eng, rho_slot_tensor, res_slot, alpha, clip, out):
         """Omega of rho + alpha r (trial density), value chain only."""
@@ -285,7 +279,7 @@ class Picard(Solver):
         eng.plan.derive()
         s = eng.plan.scalars()[0]
         self.omega0 = self._density_omega(eng, s)
-        self._phi_fex = (s[L.S_PHI], s[L.S_FEX])
+        self._phi_fex = (s[L.S_PHI], eng.fex(s))
         n3max_cur = s[L.S_N3MAX]
         converged = False
         N_new = s[L.S_PN] * eng.dr
@@ -305,7 +299,7 @@ class Picard(Solver):
                     eng.omega(chempot, s[L.S_NCUR], s[L.S_FID], s[L.S_EXT], self._prev_phi(s), self._prev_fex(s)))
                 N_new = s[L.S_PIC_NNEW] * eng.dr
                 iue, dmax = s[L.S_PIC_IUE], s[L.S_PIC_DMAX]
-                self._phi_fex = (s[L.S_PHI], s[L.S_FEX])
+                self._phi_fex = (s[L.S_PHI], eng.fex(s))
             else:
                 eng.plan.finish_anderson(ring[cur], eng.V, False, self._res_ring, 1, 0, None, eng.beta, eng.lam3)
                 eng.plan.synchronize()

@@ -41,6 +41,14 @@
  *  cmm_lj_grid_grad      lennard_jones(derivative=True) (the reference's
  *                        radial derivatives; here with respect to the
  *                        host-type sigma/epsilon)     cmmdft/extpot_calculator.py:156-183
+ *  cmm_mfa_lj_potential  MFAFunctional.generate_potential_lj (on the grid
+ *                        or, folded, on its supercell)  cmmdft/functionals.py:878-910
+ *  cmm_fold              the supercell sampling kpotential[::r0,::r1,::r2]
+ *                        of MFAFunctional.derive, done in real space
+ *                                                      cmmdft/functionals.py:917-923
+ *  cmm_mfa_table         MFAFunctional kpotential (fft x Lanczos, or fft of a
+ *                        loaded potential)             cmmdft/functionals.py:809-812, 910
+ *  cmm_plan_set_mfa      MFAFunctional.derive / value  cmmdft/functionals.py:912-939
  *  cmm_hessian_apply, cmm_adjoint_*
  *                        no reference counterpart: the reference has no
  *                        derivatives of the converged density; these give
@@ -109,7 +117,35 @@ int cmm_plan_get_radix(cmm_plan* plan, int axis, int* nstages, int* radices);
 int cmm_plan_get_info(cmm_plan* plan, long long* info, int n);
 /* device int[nbatch] mask of instances to process (NULL = all) */
 int cmm_plan_set_active(cmm_plan* plan, const int* dev_mask);
+/* sets the hard-sphere / WDA functional; clears the mean-field table */
 int cmm_set_functional(cmm_plan* plan, const cmm_functional* f);
+/* Mean-field (MFA) part: kd is a caller-owned complex half spectrum
+ * double2[nx][ny][nz/2+1] made by cmm_mfa_table (NULL switches the part
+ * off); it must stay valid while set.  The derivative chain then adds
+ * c_MFA = IFFT(K rho^) to c_ex, and the forward chain reduces
+ * (1/2) sum_x rho c_MFA into scalar slot 28 (times dr: the MFA energy).
+ * Call after cmm_set_functional. */
+int cmm_plan_set_mfa(cmm_plan* plan, const double* kd);
+/* LJ pair potential of the mean-field part on the grid of ns[0] x ns[1] x
+ * ns[2] points of the cell rvecs (rows are the cell vectors; points
+ * frac . rvecs with frac_a = i_a / ns_a - 1/2), folded onto the grid of
+ * ns / reps points (reps = {1,1,1}: the plain potential):
+ * U = 4 eps ((sigma/r)^12 - (sigma/r)^6) for r > rmin, limit otherwise;
+ * with use_cutoff, U - U(cutoff) for r <= cutoff and 0 beyond.
+ * out: device double[ns/reps], on stream. */
+int cmm_mfa_lj_potential(const int ns[3], const double rvecs[9], const int reps[3], double sigma, double epsilon,
+                         double rmin, double limit, double cutoff, int use_cutoff, double* out, void* stream);
+/* out[i] = sum over image offsets j of in[i + j n], n = ns / reps (device
+ * arrays, in: ns points, out: ns / reps points) */
+int cmm_fold(const int ns[3], const int reps[3], const double* in, double* out, void* stream);
+/* Mean-field kernel table from uhat = cmm_fft_r2c of the folded potential
+ * (one instance): kd = scale uhat L P_h with L the Lanczos factor (lanczos =
+ * 1; the reference applies it to generated, not to loaded potentials), P_h
+ * the Hermitian part of the reference's phase exp(i pi scalprod) of the
+ * supercell with vectors rvecs_sc (= rvecs with column a times reps[a]) at
+ * the wave vectors reps . m, and scale = dr / npts of the grid. */
+int cmm_mfa_table(cmm_plan* plan, const double* uhat, const int reps[3], const double rvecs_sc[9], int lanczos,
+                  double scale, double* kd);
 /* per-instance fugacity and mixing parameter (host arrays of nbatch) */
 int cmm_set_instances(cmm_plan* plan, const double* fug, const double* alpha);
 

@@ -22,6 +22,11 @@ live in ``tests/test_oracle_*.py``):
                           compressibility, WBII/MFMT -> Carnahan-Starling
                           free energy and excess chemical potential; finite-
                           difference consistency of derive() with value().
+* MFA .................. bulk limit c = 2 a rho with a = compute_vdw_a
+                          (uniform density), a against the radial integral of
+                          the shifted LJ potential, the supercell sampling
+                          against a real-space fold, the phase-displaced
+                          kernel against a direct sum on a tiny grid.
 * WDA / EOS ............ bulk limit (weighted density == density), EOS
                           derivative by finite differences, MBWR coefficient
                           table literal in the reference.
@@ -583,6 +588,66 @@ class ExtPot:
     def value(self, krho, local=False):
         rho = self.grid.ifft(krho)
         return rho * self.potential if local else self.grid.integrate(rho * self.potential)
+
+
+class MFA:
+    """MFAFunctional restated (functionals.py:770-939).  With tail
+    corrections the potential lives on the supercell grid (Grid.supercell,
+    system.py:297-301: cell vectors with column a times reps[a]) and the
+    convolution samples every reps-th supercell wave vector."""
+    name = 'MFA'
+
+    def __init__(self, grid, tailcorrections=False, repetitions=(2, 2, 2)):   # :778-800
+        self.tailcorrections = tailcorrections
+        self.repetitions = list(repetitions)
+        if tailcorrections:
+            self.small_grid = grid
+            r = np.asarray(self.repetitions)
+            self.grid = Grid(grid.rvecs * r, list(np.asarray(grid.npoints) * r))
+        else:
+            self.grid = grid
+        self.potential = None
+        self.kpotential = None
+
+    def generate_potential_lj(self, sigma, epsilon, rmin=None, limit_potential=0, cutoff=None):   # :878-910
+        if rmin is None:
+            rmin = sigma
+        r = self.grid.points[..., 3]
+        pot = np.full(r.shape, limit_potential, dtype=np.float64)
+        mask = r > rmin
+        x = np.zeros_like(r)
+        x[mask] = sigma / r[mask]
+        pot[mask] = 4 * epsilon * (x[mask] ** 12 - x[mask] ** 6)
+        if cutoff is not None:
+            rc_mask = r <= cutoff
+            rc6 = (sigma / cutoff) ** 6
+            shift = 4 * epsilon * (rc6 ** 2 - rc6)
+            pot[rc_mask] -= shift
+            pot[~rc_mask] = 0.0
+        self.potential = pot
+        self.kpotential = self.grid.fft(self.potential) * self.grid.sigma_lanczos
+
+    def load_potential(self, potential):            # :809-812 (array instead of a file)
+        self.potential = np.array(potential, dtype=np.float64)
+        assert self.grid.points.shape[:3] == self.potential.shape
+        self.kpotential = self.grid.fft(self.potential)
+
+    def compute_vdw_a(self):                        # :814-822
+        self.a = 0.5 * self.grid.integrate(self.potential)
+        return self.a
+
+    def derive(self, krho):                         # :912-926
+        if self.tailcorrections:
+            r0, r1, r2 = self.repetitions
+            return self.small_grid.ifft(krho * self.kpotential[::r0, ::r1, ::r2]) * self.grid.volume
+        return self.grid.ifft(krho * self.kpotential) * self.grid.volume
+
+    def value(self, krho, local=False):             # :928-939
+        grid = self.small_grid if self.tailcorrections else self.grid
+        rho = grid.ifft(krho)
+        if local:
+            return 0.5 * rho * self.derive(krho)
+        return 0.5 * grid.integrate(rho * self.derive(krho))
 
 
 # ---------------------------------------------------------------------------

@@ -433,3 +433,78 @@ def test_tensor_phi3_is_tarazona_form():
         d = dict(n2=np.array(n2), nv2=v[None, :], xi=None, nt=[xx, xy, xz, yy, yz, zz, tr2, tr3])
         q = O.HardSphere._q3(d, 'tFMT')
         assert np.isclose(float(np.squeeze(q)), tara, rtol=1e-12, atol=1e-12)
+
+
+def _lj_mfa_setup(shape, edge, tail=False):
+    g = O.Grid(np.eye(3) * edge, shape)
+    m = O.MFA(g, tailcorrections=tail)
+    m.generate_potential_lj(W.GUEST_SIGMA, W.GUEST_EPSILON, cutoff=12 * U.angstrom)
+    return g, m
+
+
+@pytest.mark.parametrize('tail', [False, True])
+def test_mfa_bulk_limit(tail):
+    """Uniform density: c_MFA = 2 a rho_b and F_MFA = a rho_b^2 V with
+    a = compute_vdw_a (k = 0: Lanczos factor and phase are 1)."""
+    g, m = _lj_mfa_setup((12, 10, 14), 15.0 * U.angstrom, tail)
+    rho_b = 3e-3 / U.angstrom ** 3
+    kr = g.fft(np.full(g.npoints, rho_b))
+    a = m.compute_vdw_a()
+    assert np.allclose(m.derive(kr), 2 * a * rho_b, rtol=1e-12)
+    assert np.isclose(m.value(kr), a * rho_b ** 2 * g.volume, rtol=1e-12)
+
+
+def test_mfa_vdw_a_matches_radial_integral():
+    """a = (1/2) sum U dr against (1/2) int 4 pi r^2 U dr of the cut and
+    shifted LJ potential (U = -U(rc) inside sigma, 0 beyond rc), on a grid of
+    h = 0.38 A: within 1 %."""
+    sig, eps, rc = W.GUEST_SIGMA, W.GUEST_EPSILON, 12 * U.angstrom
+    g, m = _lj_mfa_setup((64, 64, 64), 24.5 * U.angstrom)
+    rc6 = (sig / rc) ** 6
+    shift = 4 * eps * (rc6 ** 2 - rc6)
+    well = 16 * np.pi * eps * (sig ** 12 / 9 * (sig ** -9 - rc ** -9) - sig ** 6 / 3 * (sig ** -3 - rc ** -3))
+    a_ref = 0.5 * (well - shift * 4 * np.pi / 3 * rc ** 3)
+    assert abs(m.compute_vdw_a() / a_ref - 1) < 1e-2, (m.compute_vdw_a(), a_ref)
+
+
+def test_mfa_supercell_sampling_is_a_fold():
+    """Every 2nd wave vector of the supercell transform is the transform of
+    the supercell potential folded onto the grid (numpy.fft)."""
+    g, m = _lj_mfa_setup((8, 6, 10), 10.0 * U.angstrom, tail=True)
+    U_sc = m.potential
+    assert U_sc.shape == (16, 12, 20)
+    fold = U_sc.reshape(2, 8, 2, 6, 2, 10).sum(axis=(0, 2, 4))
+    assert np.allclose(np.fft.fftn(U_sc)[::2, ::2, ::2], np.fft.fftn(fold), rtol=0,
+                       atol=1e-12 * np.abs(np.fft.fftn(fold)).max())
+
+
+def test_mfa_convolution_carries_one_phase_factor():
+    """A point potential at grid index s convolved with a point density at
+    the origin (loaded potential: no Lanczos factor), by explicit sums:
+    c(x) = dr Re(prod_a d_a(x_a)) with
+    d_a(x) = (1/N) sum_m exp(-2 pi i m s_a / N) exp(2 pi^2 i fm) exp(2 pi i m x / N):
+    the potential's own position and exactly one residual factor
+    exp(i pi scalprod) of the reference's phased transforms."""
+    shape = (6, 5, 4)
+    g = O.Grid(np.diag([9.0, 8.0, 7.0]), shape)
+    m = O.MFA(g)
+    shifts = (3, 0, 2)
+    pot = np.zeros(shape)
+    pot[shifts] = 1.0
+    rho = np.zeros(shape)
+    rho[0, 0, 0] = 1.0
+    m.load_potential(pot)
+    c = m.derive(g.fft(rho))
+
+    def d(n, shift, x):
+        tot = 0j
+        for k in range(n):
+            fm = k if k <= (n - 1) // 2 else k - n
+            tot += np.exp(-2j * np.pi * k * shift / n) * np.exp(2j * np.pi ** 2 * fm) * np.exp(2j * np.pi * k * x / n)
+        return tot / n
+
+    for x in np.ndindex(*shape):
+        ref = g.dr * (d(6, shifts[0], x[0]) * d(5, shifts[1], x[1]) * d(4, shifts[2], x[2])).real
+        assert abs(c[x] - ref) < 1e-13 * g.dr, (x, c[x], ref)
+    # the displacement is real: the response is not at the potential's offset
+    assert np.argmax(np.abs(c)) != np.ravel_multi_index(shifts, shape)
