@@ -600,43 +600,46 @@ struct PhiOut {
 };
 
 template <typename T>
-__device__ __forceinline__ void phi23(int base, T n3, T ln3, T& p2, T& p3, T& d2, T& d3) {
-  // Same forms as the reference (functionals.py:726-767); divisions by the
-  // common factors q = 1-2n3+n3^2, c = 1-3n3+3n3^2-n3^3 and n3^2 are taken
-  // as multiplications by their reciprocals.
+__device__ __forceinline__ void phi23(int base, T n3, T ln3, T r1, T& p2, T& p3, T& d2, T& d3) {
+  // Same forms as the reference (functionals.py:726-767).  Every
+  // denominator is a product of powers of (1 - n3) and n3, so the divisions
+  // are multiplications by r1 = 1/(1-n3) and (full White-Bear/MFMT branch)
+  // 1/n3: q = (1-n3)^2, c = (1-n3)^3, n3 - n3^2 = n3 (1-n3).
   const T n3_2 = n3 * n3, n3_3 = n3_2 * n3;
   const T q = 1.0 - 2.0 * n3 + n3_2;
   const T c = 1.0 - 3.0 * n3 + 3.0 * n3_2 - n3_3;
-  const T iq = 1.0 / q, ic = 1.0 / c;
+  const T iq = r1 * r1, ic = iq * r1;
   const double i24 = 1.0 / (24.0 * M_PI), i36 = 1.0 / (36.0 * M_PI);
   if (base == 0) {                       // Rosenfeld
-    p2 = 1.0 / (1.0 - n3);
+    p2 = r1;
     d2 = iq;
     p3 = i24 * iq;
     d3 = (1.0 / (12.0 * M_PI)) * ic;
   } else if (base == 1) {                // modified FMT
-    p2 = 1.0 / (1.0 - n3);
+    p2 = r1;
     d2 = iq;
     if (val(n3) <= 1e-8) {
       p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) * i24 * iq;
       d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) * i36 * ic;
     } else {
-      const T in2 = 1.0 / n3_2;
+      const T in1 = 1.0 / n3;
+      const T in2 = in1 * in1;
       p3 = (n3 + q * ln3) * i36 * in2 * iq;
-      d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) * i36 * (in2 / n3) * ic;
+      d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) * i36 * (in2 * in1) * ic;
     }
   } else {                               // White-Bear II
     if (val(n3) <= 1e-8) {
-      p2 = (1.0 + n3_2 / 9.0) / (1.0 - n3);
+      p2 = (1.0 + n3_2 / 9.0) * r1;
       d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) * iq;
       p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) * i24 * iq;
       d3 = (7.0 / 3.0 - n3 / 2.0 + n3_2 / 10.0) * i36 * ic;
     } else {
-      const T in2 = 1.0 / n3_2;
-      p2 = (5.0 * n3 - n3_2 + 2.0 * (1.0 - n3) * ln3) / (3.0 * (n3 - n3_2));
+      const T in1 = 1.0 / n3;
+      const T in2 = in1 * in1;
+      p2 = (5.0 * n3 - n3_2 + 2.0 * (1.0 - n3) * ln3) * ((1.0 / 3.0) * in1 * r1);
       d2 = -2.0 * (n3 - 3.0 * n3_2 + q * ln3) * (1.0 / 3.0) * in2 * iq;
       p3 = -2.0 * (n3 - 3.0 * n3_2 + n3_3 + ln3 * q) * (1.0 / 3.0) * in2 * i24 * iq;
-      d3 = (2.0 * n3 - 5.0 * n3_2 + 6.0 * n3_3 - n3_2 * n3_2 + 2.0 * c * ln3) * i36 * (in2 / n3) * ic;
+      d3 = (2.0 * n3 - 5.0 * n3_2 + 6.0 * n3_3 - n3_2 * n3_2 + 2.0 * c * ln3) * i36 * (in2 * in1) * ic;
     }
   }
 }
@@ -661,8 +664,9 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
   const T ux = vx * i4pR, uy = vy * i4pR, uz = vz * i4pR;   // n_v1
   const T n3 = clip(n3raw, 1e-30, 0.99);
   const T ln3 = log(1.0 - n3);
+  const T r1 = 1.0 / (1.0 - n3);
   T p2, p3, d2, d3;
-  phi23<T>(fp.base, n3, ln3, p2, p3, d2, d3);
+  phi23<T>(fp.base, n3, ln3, r1, p2, p3, d2, d3);
   const T v2 = vx * vx + vy * vy + vz * vz;
   const T dot12 = ux * vx + uy * vy + uz * vz;
   T q3, dq3_n2, fv;
@@ -691,7 +695,7 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
   const T dn0 = -ln3;
   const T dn1 = n2 * p2;
   const T dn2 = n1 * p2 + dq3_n2 * p3;
-  const T dn3 = n0 / (1.0 - n3) + (n1 * n2 - dot12) * d2 + q3 * d3;
+  const T dn3 = n0 * r1 + (n1 * n2 - dot12) * d2 + q3 * d3;
   const double kT = fp.kT;
   o.g0 = kT * (dn0 * i4pR2 + dn1 * i4pR + dn2);
   o.g1 = kT * dn3;
@@ -774,6 +778,48 @@ __device__ __forceinline__ void wda_point(const FuncDev& fp, T rb, T& f, T& df) 
   }
 }
 
+// Pointwise stage of the functional z pass at one point: r holds the NF
+// weighted densities (n3 unclipped) on entry and the NF derivative fields on
+// exit; acc gets Phi, f_ex(rho_bar) and n3 (for the max).
+template <int NF>
+__device__ __forceinline__ void zphi_point(const FuncDev& fp, double (&r)[NF], double (&acc)[3]) {
+  double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
+  double gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (fp.use_hs) {
+    PhiOut<double> o;
+    if constexpr (NF == 12) {
+      o = fmt_point<double, true>(fp, r[0], r[1], r[2], r[3], r[4], &r[6]);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) gt[c] = o.gt[c];
+    } else {
+      o = fmt_point<double>(fp, r[0], r[1], r[2], r[3], r[4]);
+    }
+    acc[0] = o.phi;
+    acc[2] = r[1];
+    g0 = o.g0;
+    g1 = o.g1;
+    gv0 = o.gv[0];
+    gv1 = o.gv[1];
+    gv2 = o.gv[2];
+  }
+  if (fp.use_wda) {
+    double fw, dfw;
+    wda_point<double>(fp, r[5], fw, dfw);
+    acc[1] = fw;
+    g5 = dfw;
+  }
+  r[0] = g0;
+  r[1] = g1;
+  r[2] = gv0;
+  r[3] = gv1;
+  r[4] = gv2;
+  r[5] = g5;
+  if constexpr (NF == 12) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) r[6 + c] = gt[c];
+  }
+}
+
 // z pass of the functional: NF weighted densities (x,y inverse-transformed)
 // -> C2R -> pointwise Phi / WDA derivatives -> R2C -> F[0..NF) in place
 // (NF = 12: the tensor variant's six more fields).
@@ -802,49 +848,16 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int e = (int)fdiv((unsigned)i, fP), pl = i - e * t.P;
     if (p0 + pl >= npen) continue;
-    double& r0 = z_val(sm, t, 0, pl, e);
-    double& r1 = z_val(sm, t, 1, pl, e);
-    double& r2 = z_val(sm, t, 2, pl, e);
-    double& r3 = z_val(sm, t, 3, pl, e);
-    double& r4 = z_val(sm, t, 4, pl, e);
-    double& r5 = z_val(sm, t, 5, pl, e);
-    double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
-    double tn[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (fp.use_hs) {
-      PhiOut<double> o;
-      if constexpr (NF == 12) {
+    double r[NF];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) tn[c] = z_val(sm, t, 6 + c, pl, e);
-        o = fmt_point<double, true>(fp, r0, r1, r2, r3, r4, tn);
+    for (int f = 0; f < NF; ++f) r[f] = z_val(sm, t, f, pl, e);
+    double a[3] = {0.0, 0.0, -INFINITY};
+    zphi_point<NF>(fp, r, a);
+    acc[0] += a[0];
+    acc[1] += a[1];
+    acc[2] = fmax(acc[2], a[2]);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) gt[c] = o.gt[c];
-      } else {
-        o = fmt_point<double>(fp, r0, r1, r2, r3, r4);
-      }
-      acc[0] += o.phi;
-      acc[2] = fmax(acc[2], r1);
-      g0 = o.g0;
-      g1 = o.g1;
-      gv0 = o.gv[0];
-      gv1 = o.gv[1];
-      gv2 = o.gv[2];
-    }
-    if (fp.use_wda) {
-      double fw, dfw;
-      wda_point<double>(fp, r5, fw, dfw);
-      acc[1] += fw;
-      g5 = dfw;
-    }
-    r0 = g0;
-    r1 = g1;
-    r2 = gv0;
-    r3 = gv1;
-    r4 = gv2;
-    r5 = g5;
-    if constexpr (NF == 12) {
-#pragma unroll
-      for (int c = 0; c < 6; ++c) z_val(sm, t, 6 + c, pl, e) = gt[c];
-    }
+    for (int f = 0; f < NF; ++f) z_val(sm, t, f, pl, e) = r[f];
   }
   if (store) {
     __syncthreads();
