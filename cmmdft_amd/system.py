@@ -9,6 +9,8 @@ P = exp(i pi scalprod) (system.py:310-332), computed by the HIP R2C/C2R.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from . import units as U
@@ -59,20 +61,110 @@ class EmptyHost(Host):
         super().__init__(name, cell)
 
 
+def read_chk(fn):
+    """Fields of a yaff checkpoint text file: ``name kind=KIND [shape]``
+    header lines followed by the values of array kinds (intar, fltar, strar,
+    blnar), scalars (int, flt, str, bln) on the header line, ``none``."""
+    conv = {'int': int, 'flt': float, 'str': str, 'bln': lambda v: v.lower() in ('true', '1')}
+    out = {}
+    with open(fn) as fh:
+        tokens = None
+        key = kind = None
+        shape = None
+        for line in fh:
+            words = line.split()
+            if len(words) >= 2 and words[1].startswith('kind='):
+                if tokens is not None:
+                    raise ValueError('%s: field %s is short of values' % (fn, key))
+                key, kind = words[0], words[1][5:]
+                if kind == 'none':
+                    out[key] = None
+                elif kind.endswith('ar'):
+                    shape = tuple(int(v) for v in words[2].split(','))
+                    tokens = []
+                    if int(np.prod(shape)) == 0:
+                        out[key] = np.zeros(shape)
+                        tokens = None
+                else:
+                    out[key] = conv[kind](' '.join(words[2:]) if kind == 'str' else words[2])
+                continue
+            if tokens is None:
+                continue
+            tokens.extend(words)
+            n = int(np.prod(shape))
+            if len(tokens) >= n:
+                base = kind[:-2]
+                if base == 'str':
+                    out[key] = list(tokens[:n]) if len(shape) == 1 else np.array(tokens[:n]).reshape(shape)
+                else:
+                    dtype = {'int': np.int64, 'flt': np.float64, 'bln': bool}[base]
+                    vals = tokens[:n] if base != 'bln' else [v.lower() in ('true', '1') for v in tokens[:n]]
+                    out[key] = np.array(vals, dtype=dtype).reshape(shape)
+                tokens = None
+        if tokens is not None:
+            raise ValueError('%s: field %s is short of values' % (fn, key))
+    return out
+
+
+class ChkSystem:
+    """The parts of a yaff System the hot path reads (positions, cell,
+    force-field atom types, masses), from a checkpoint file."""
+
+    def __init__(self, pos, rvecs, ffatypes, ffatype_ids, masses=None, numbers=None, charges=None):
+        self.pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+        self.cell = Cell(rvecs)
+        self.ffatypes = list(ffatypes)
+        self.ffatype_ids = np.asarray(ffatype_ids, dtype=np.int64)
+        self.masses = None if masses is None else np.asarray(masses, dtype=np.float64)
+        self.numbers = None if numbers is None else np.asarray(numbers)
+        self.charges = None if charges is None else np.asarray(charges, dtype=np.float64)
+        self.natom = len(self.pos)
+
+    @classmethod
+    def from_file(cls, fn):
+        d = read_chk(fn)
+        for k in ('pos', 'rvecs', 'ffatypes', 'ffatype_ids'):
+            if d.get(k) is None:
+                raise ValueError('%s has no %s' % (fn, k))
+        return cls(d['pos'], d['rvecs'], d['ffatypes'], d['ffatype_ids'], d.get('masses'), d.get('numbers'),
+                   d.get('charges'))
+
+
 class NanoporousHost(Host):
-    """Host framework given as a duck-typed molecular system ``mol`` with
-    ``pos``, ``ffatype_ids``, ``ffatypes`` and ``cell.rvecs`` (the attributes
-    the reference reads from a yaff System, system.py:53-76) and a force-field
-    dict {ffatype: (sigma, epsilon)}.  Positions are centred on their mean
+    """Host framework (system.py:53-76): ``chk`` is a yaff checkpoint file
+    (or an object with ``pos``, ``ffatype_ids``, ``ffatypes`` and
+    ``cell.rvecs``), ``par`` a yaff LJ parameter file (or a dict
+    {ffatype: (sigma, epsilon)}).  Positions are centred on their mean
     (system.py:68-70)."""
 
-    def __init__(self, name, mol, ff):
+    def __init__(self, name, chk, par):
         import copy
-        self.mol = copy.copy(mol)
+        from .extpot import read_pars_file
+        if isinstance(chk, (str, os.PathLike)):
+            self.chk = chk
+            mol = ChkSystem.from_file(chk)
+        else:
+            self.chk = None
+            mol = copy.copy(chk)
+        if isinstance(par, (str, os.PathLike)):
+            self.par = par
+            ff = read_pars_file(par)
+        else:
+            self.par = None
+            ff = par
+        self.mol = mol
         pos = np.asarray(mol.pos, dtype=np.float64)
         self.mol.pos = pos - pos.sum(axis=0) / len(pos)
         self.ff = dict(ff)
+        used = {self.mol.ffatypes[int(i)] for i in np.unique(np.asarray(self.mol.ffatype_ids))}
+        missing = sorted(used - set(self.ff))
+        if missing:
+            raise ValueError('no LJ parameters for host atom types %s' % missing)
         super().__init__(name, mol.cell)
+
+    def copy(self):
+        return NanoporousHost(self.name, self.chk if self.chk is not None else self.mol,
+                              self.par if self.par is not None else self.ff)
 
 
 class Guest:

@@ -1,0 +1,96 @@
+"""Reading real framework inputs without yaff: the yaff checkpoint text
+format (system.py:53-76 reads it through yaff.System.from_file) and the LJ
+parameter file (extpot_calculator.py:705-716).  Fixtures under
+tests/golden/inputs are the reference's own input data
+(InputFiles/Host/ZIF8/struct.chk, pars_vdw_uff.txt; InputFiles/Host/ZEO/
+cha.chk, pars_cha.txt).  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+from cmmdft_amd import units as U
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'inputs')
+
+
+def _p(name):
+    return os.path.join(HERE, name)
+
+
+def test_zif8_checkpoint():
+    from cmmdft_amd.system import read_chk
+    d = read_chk(_p('zif8.chk'))
+    assert d['pos'].shape == (276, 3) and d['rvecs'].shape == (3, 3)
+    assert d['ffatypes'] == ['H1', 'H3', 'C2', 'C1', 'C3', 'Zn', 'N']
+    assert d['ffatype_ids'].shape == (276,) and d['ffatype_ids'].max() == 6
+    assert d['bonds'].shape == (312, 2) and d['dipoles'] is None
+    # ZIF-8 cubic cell: Zn12 (C4 H5 N2)24
+    numbers = d['numbers']
+    comp = {z: int((numbers == z).sum()) for z in np.unique(numbers)}
+    assert comp == {1: 120, 6: 96, 7: 48, 30: 12}
+    # masses in electron masses, as written in the file: H 1, C 12.011, N 14.001, Zn 65.39 amu
+    m = d['masses'] / U.amu
+    for z, mz in ((1, 1.0), (6, 12.011), (7, 14.001), (30, 65.39)):
+        assert np.allclose(m[numbers == z], mz, rtol=1e-6), z
+    # cubic, a = 16.596 A, atoms inside the cell's span
+    assert np.allclose(np.diag(d['rvecs']) / U.angstrom, 16.596, atol=1e-3)
+    assert np.abs(d['rvecs'] - np.diag(np.diag(d['rvecs']))).max() < 2e-4
+
+
+def test_cha_checkpoint_is_hexagonal_silica():
+    from cmmdft_amd.system import read_chk
+    d = read_chk(_p('cha.chk'))
+    types = [d['ffatypes'][i] for i in d['ffatype_ids']]
+    assert types.count('Z_Si') * 2 == types.count('Z_O') == 72
+    a, b = d['rvecs'][0], d['rvecs'][1]
+    gamma = np.degrees(np.arccos(a @ b / np.linalg.norm(a) / np.linalg.norm(b)))
+    assert abs(gamma - 120.0) < 0.2
+
+
+def test_pars_file_units():
+    from cmmdft_amd.extpot import read_pars_file
+    ff = read_pars_file(_p('zif8_pars_vdw_uff.txt'))
+    assert set(ff) == {'Zn', 'C1', 'C2', 'C3', 'H1', 'H3', 'N'}
+    assert np.isclose(ff['Zn'][0], 2.462 * U.angstrom) and np.isclose(ff['Zn'][1], 0.124 * U.kcalmol)
+    ff = read_pars_file(_p('cha_pars.txt'))
+    assert np.isclose(ff['Z_O'][0], 3.118 * U.angstrom) and np.isclose(ff['Z_O'][1], 0.06 * U.kcalmol)
+
+
+def test_checkpoint_kinds(tmp_path):
+    """Scalars, none, string/int/float/bool arrays and multi-line arrays."""
+    from cmmdft_amd.system import read_chk
+    fn = tmp_path / 'x.chk'
+    fn.write_text(
+        'title                                     kind=str  hello world\n'
+        'natom                                     kind=int  3\n'
+        'energy                                    kind=flt  -1.5e+00\n'
+        'dipoles                                   kind=none   None\n'
+        'ffatypes                                  kind=strar 2\n'
+        '                     A                      B\n'
+        'ids                                       kind=intar 3\n'
+        '                     0                      1                      0\n'
+        'pos                                       kind=fltar 3,2\n'
+        ' 1.0e+00  2.0e+00  3.0e+00  4.0e+00\n'
+        ' 5.0e+00  6.0e+00\n'
+        'flags                                     kind=blnar 2\n'
+        '  True  False\n')
+    d = read_chk(fn)
+    assert d['title'] == 'hello world' and d['natom'] == 3 and d['energy'] == -1.5 and d['dipoles'] is None
+    assert d['ffatypes'] == ['A', 'B'] and d['ids'].tolist() == [0, 1, 0]
+    assert d['pos'].tolist() == [[1, 2], [3, 4], [5, 6]] and d['flags'].tolist() == [True, False]
+    bad = tmp_path / 'bad.chk'
+    bad.write_text('pos  kind=fltar 2,2\n 1.0 2.0 3.0\n')
+    with pytest.raises(ValueError):
+        read_chk(bad)
+
+
+def test_nanoporous_host_from_files():
+    from cmmdft_amd.system import NanoporousHost
+    h = NanoporousHost('ZIF8', _p('zif8.chk'), _p('zif8_pars_vdw_uff.txt'))
+    assert h.mol.natom == 276 and np.allclose(h.mol.pos.mean(axis=0), 0.0, atol=1e-12)   # centred (system.py:68-70)
+    assert h.chk.endswith('zif8.chk') and np.isclose(h.cell.volume, np.linalg.det(h.cell.rvecs))
+    h2 = h.copy()
+    assert np.array_equal(h2.mol.pos, h.mol.pos) and h2.ff == h.ff
+    with pytest.raises(ValueError):
+        NanoporousHost('ZIF8', _p('zif8.chk'), _p('cha_pars.txt'))     # no parameters for the ZIF-8 types
