@@ -50,6 +50,8 @@ EXPORTS = {
     'cmm_mfa_lj_potential': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
     'cmm_fold': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_field_reduce': ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_double, C.c_double, C.c_void_p,
+                          C.c_void_p, C.c_void_p], C.c_int),
     'cmm_mfa_table': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p], C.c_int),
     'cmm_plan_get_radix': ([C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p], C.c_int),
     'cmm_plan_set_active': ([C.c_void_p, C.c_void_p], C.c_int),
@@ -432,3 +434,24 @@ def fold(field, reps, stream=None):
     check(lib.cmm_fold(_ints(ns), _ints([int(r) for r in reps]), _ptr(field.contiguous()), _ptr(out),
                        C.c_void_p(s.cuda_stream)))
     return out
+
+
+FIELD_OPS = {'sum': 0, 'boltzmann': 1, 'boltzmann_clip': 2, 'min': 3, 'max': 4, 'sumsq': 5, 'fid': 6}
+
+
+def field_reduce(x, op, beta=1.0, lam3=1.0, mask=None, stream=None):
+    """Deterministic device reduction of a field (see cmm_field_reduce);
+    returns a Python float."""
+    torch = _torch()
+    lib = load()
+    x = x.reshape(-1).contiguous()
+    if mask is not None:
+        mask = mask.reshape(-1).contiguous().to(torch.uint8)
+        if mask.numel() != x.numel():
+            raise ValueError('mask must have the shape of the field')
+    work = torch.empty(1024, dtype=torch.float64, device=x.device)
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_field_reduce(_ptr(x), _ptr(mask), int(x.numel()), FIELD_OPS[op], float(beta), float(lam3),
+                               _ptr(work), _ptr(out), C.c_void_p(s.cuda_stream)))
+    return float(out.item())

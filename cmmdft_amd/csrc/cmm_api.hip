@@ -5,6 +5,7 @@
 #include "cmm_lj.hip"
 #include "cmm_adjoint.hip"
 #include "cmm_mfa.hip"
+#include "cmm_calc.hip"
 #include "../../include/cmmdft_hip.h"
 
 #include <algorithm>
@@ -463,6 +464,20 @@ int cmm_mfa_table(cmm_plan* p, const double* uhat, const int* reps, const double
   if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
   (void)hipFree(dph);
   if (e != hipSuccess) return fail(CMM_E_HIP, std::string("k_mfa_table: ") + hipGetErrorString(e));
+  return CMM_OK;
+}
+
+int cmm_field_reduce(const double* x, const unsigned char* mask, long long n, int op, double beta, double lam3,
+                     double* work, double* out, void* stream) {
+  if (!x || !work || !out) return fail(CMM_E_ARG, "null argument");
+  if (n < 1) return fail(CMM_E_ARG, "empty field");
+  if (op < FOP_SUM || op > FOP_FID) return fail(CMM_E_ARG, "unknown reduction");
+  const long long want = (n + RED_NT - 1) / RED_NT;
+  const int nb = (int)std::min<long long>(RED_BLOCKS, want);
+  hipLaunchKernelGGL(k_field_reduce, dim3(nb), dim3(RED_NT), 0, (hipStream_t)stream, x, mask, n, op, beta, lam3, work);
+  LAUNCHCHK("k_field_reduce");
+  hipLaunchKernelGGL(k_field_reduce_final, dim3(1), dim3(64), 0, (hipStream_t)stream, work, nb, op, out);
+  LAUNCHCHK("k_field_reduce_final");
   return CMM_OK;
 }
 

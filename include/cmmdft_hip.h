@@ -49,6 +49,9 @@
  *  cmm_mfa_table         MFAFunctional kpotential (fft x Lanczos, or fft of a
  *                        loaded potential)             cmmdft/functionals.py:809-812, 910
  *  cmm_plan_set_mfa      MFAFunctional.derive / value  cmmdft/functionals.py:912-939
+ *  cmm_field_reduce      Calculator integrals: loading, Henry coefficient,
+ *                        helium void fraction, ideal-gas free energy
+ *                                                      cmmdft/calculator.py:60-88, 152-184, 387-397
  *  cmm_hessian_apply, cmm_adjoint_*
  *                        no reference counterpart: the reference has no
  *                        derivatives of the converged density; these give
@@ -119,6 +122,13 @@ int cmm_plan_get_info(cmm_plan* plan, long long* info, int n);
 int cmm_plan_set_active(cmm_plan* plan, const int* dev_mask);
 /* sets the hard-sphere / WDA functional; clears the mean-field table */
 int cmm_set_functional(cmm_plan* plan, const cmm_functional* f);
+/* Deterministic reduction of a device field x[n] (points with mask[i] = 0
+ * skipped when mask is not NULL) into device out[0]: op 0 sum x, 1 sum
+ * exp(-beta x), 2 sum min(max(exp(-beta x), 0), 1), 3 min x, 4 max x, 5 sum
+ * x^2, 6 sum c (ln(lam3 c) - 1) with c = max(x, 1e-20).  work: device
+ * scratch of at least 1024 doubles. */
+int cmm_field_reduce(const double* x, const unsigned char* mask, long long n, int op, double beta, double lam3,
+                     double* work, double* out, void* stream);
 /* Mean-field (MFA) part: kd is a caller-owned complex half spectrum
  * double2[nx][ny][nz/2+1] made by cmm_mfa_table (NULL switches the part
  * off); it must stay valid while set.  The derivative chain then adds

@@ -146,3 +146,23 @@ def test_radix_candidates_multiply_to_length():
             assert int(np.prod(s)) == n and set(s) <= {2, 3, 4, 5, 8} and len(s) <= lo + 1
     # 128 = 8*8*2 (3 orders), 8*4*4 (3), 8*4*2*2 (12), 4*4*4*2 (4)
     assert len(radix_candidates(128)) == 22
+
+
+def test_adsorption_unit_conversion_and_aif_layout(tmp_path):
+    """Unit factors from masses and volume (tools.py:368-399): 1 molecule per
+    cell of mass M (amu) is 1000/M mol/kg and 100 m/M wt%; AIF text layout
+    (tools.py:401-456)."""
+    from cmmdft_amd import units as U
+    from cmmdft_amd.calculator import convert_units, write_aif
+    m_guest, m_host, vol = 16.04 * U.amu, 2000.0 * U.amu, 4000.0 * U.angstrom ** 3
+    cv = convert_units(m_guest, m_host, vol)
+    assert np.isclose(cv.conversion_factor('au/uc', 'mol/kg'), 1000.0 / 2000.0)
+    assert np.isclose(cv.conversion_factor('au/uc', 'wt%'), 100 * 16.04 / 2000.0)
+    assert np.isclose(cv.conversion_factor('au/uc', 'mg/g'), 1000 * 16.04 / 2000.0)
+    # 'cm3/cm3' as the reference defines it: m_host rho_stp / (m_guest rho_host),
+    # the inverse of the STP volume per cell volume of one molecule per cell
+    stp = 22414.0 / 6.0221415e23 / 4000.0e-24
+    assert np.isclose(cv.conversion_factor('au/uc', 'cm3/cm3'), 1 / stp, rtol=1e-4)
+    fn = tmp_path / 'x.aif'
+    write_aif(fn, 'blk', [('_a', 'x'), ('_b', ['f1', 'f2'])], '_p_', ['u', 'v'], [['1', '2'], ['3', '4']])
+    assert fn.read_text() == "data_blk\n_a x\n_b ['f1', 'f2']\n\nloop_\n_p_u\n_p_v\n1 3 \n2 4 \n"
