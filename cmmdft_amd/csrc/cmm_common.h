@@ -201,6 +201,73 @@ __device__ __forceinline__ void stockham_stage(double2* buf, int batch, const Fa
   __syncthreads();
 }
 
+// The same stage with the length and the stage's Ns fixed at compile time:
+// the index divisions become shifts or multiplications by constants and the
+// twiddle strides immediates.  Same operations in the same order as
+// stockham_stage, so the same results bit for bit.
+template <int R, int SGN, bool EM, int NB, int N, int NS>
+__device__ __forceinline__ void stockham_stage_c(double2* buf, int batch, const FastDiv& fb, int bs, int es,
+                                                 const double2* tws) {
+  constexpr int nq = N / R;
+  constexpr int twstride = N / (NS * R);
+  const int total = batch * nq;
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  double2 v[NB][R];
+  int dst[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = tid + i * nt;
+    dst[i] = -1;
+    if (b < total) {
+      int s, j;
+      if (EM) { j = (int)fdiv((unsigned)b, fb); s = b - j * batch; }
+      else { s = (int)((unsigned)b / (unsigned)nq); j = b - s * nq; }
+      const int jn = (int)((unsigned)j / (unsigned)NS);
+      const int k = j - jn * NS;
+      double2* base = buf + s * bs;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[i][t] = base[(j + t * nq) * es];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+          double2 w = tws[t * k * twstride];
+          if (SGN > 0) w.y = -w.y;
+          v[i][t] = cmul(v[i][t], w);
+        }
+      }
+      dftr<R, SGN>(v[i]);
+      dst[i] = s * bs + (jn * NS * R + k) * es;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (dst[i] >= 0) {
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[dst[i] + t * NS * es] = v[i][t];
+    }
+  }
+  __syncthreads();
+}
+
+template <int... RS>
+__device__ __forceinline__ bool same_radices(const Fft1D& p) {
+  constexpr int rs[] = {RS...};
+  if (p.nst != (int)sizeof...(RS)) return false;
+#pragma unroll
+  for (int i = 0; i < (int)sizeof...(RS); ++i)
+    if (p.rad[i] != rs[i]) return false;
+  return true;
+}
+
+template <int SGN, bool EM, int VPT, int N, int NS, int R, int... RS>
+__device__ __forceinline__ void lds_fft_c(double2* buf, int batch, const FastDiv& fb, int bs, int es,
+                                          const double2* tws) {
+  stockham_stage_c<R, SGN, EM, (VPT + R - 1) / R, N, NS>(buf, batch, fb, bs, es, tws);
+  if constexpr (sizeof...(RS) > 0) lds_fft_c<SGN, EM, VPT, N, NS * R, RS...>(buf, batch, fb, bs, es, tws);
+}
+
 // Full transform of `batch` sequences of length p.n, in place in LDS;
 // element e of sequence s at buf[s*bs + e*es]; tws: the twiddle table in LDS.  EM (element-major): the
 // sequences are the fast index (bs = 1, es >= batch), so consecutive lanes
@@ -209,8 +276,28 @@ __device__ __forceinline__ void stockham_stage(double2* buf, int batch, const Fa
 // batch * n <= VPT * blockDim.x, and that a barrier separates the data's
 // producers from this call.  Ends with a barrier.
 template <int SGN, bool EM, int VPT>
-__device__ void lds_fft(double2* buf, int batch, const FastDiv& fb, int bs, int es, const Fft1D& p,
+__device__ __forceinline__ void lds_fft(double2* buf, int batch, const FastDiv& fb, int bs, int es, const Fft1D& p,
                         const double2* tws) {
+  // compile-time stage sequences: the tuned plans of 128, 96, 64 and 48 and
+  // the default plans of the 5-smooth lengths 48..160
+#define CMM_CODELET(NN, ...)                                                          \
+  if (p.n == NN && same_radices<__VA_ARGS__>(p))                                     \
+    return lds_fft_c<SGN, EM, VPT, NN, 1, __VA_ARGS__>(buf, batch, fb, bs, es, tws);
+  CMM_CODELET(128, 8, 4, 4)
+  CMM_CODELET(128, 8, 8, 2)
+  CMM_CODELET(96, 8, 3, 4)
+  CMM_CODELET(96, 4, 8, 3)
+  CMM_CODELET(96, 3, 8, 4)
+  CMM_CODELET(64, 8, 8)
+  CMM_CODELET(48, 4, 4, 3)
+  CMM_CODELET(48, 3, 8, 2)
+  CMM_CODELET(160, 5, 8, 4)
+  CMM_CODELET(144, 3, 3, 8, 2)
+  CMM_CODELET(120, 5, 3, 8)
+  CMM_CODELET(100, 5, 5, 4)
+  CMM_CODELET(80, 5, 8, 2)
+  CMM_CODELET(72, 3, 3, 8)
+#undef CMM_CODELET
   int Ns = 1;
   for (int s = 0; s < p.nst; ++s) {
     const int r = p.rad[s];
