@@ -184,6 +184,73 @@ def adjoint_leg(shape, reps=3):
             'dN_depsilon': [float(v) for v in g['dN_depsilon']]}
 
 
+def effpot_leg(shape, degree=9):
+    """Orientationally averaged external potential of a rigid CO2 (3 LJ
+    sites) in the bench host (case-2, a = 25.6 A, 276 atoms) on the bench
+    grid at 300 K, with ``degree`` Lebedev directions x ``degree`` angles
+    about the molecular axis (cmm_effpot, one launch).  Median of 3 after a
+    warm-up; the kernel alone, timed with HIP events on the current stream."""
+    import torch
+    from cmmdft_amd import units as U, workloads as W
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    from cmmdft_amd.rotations import combined_rotations
+    host = W.host_case2()
+    mol, ff = W.guest_co2()
+    grid = Grid(host.rvecs, list(shape))
+    gens = get_external_potential_dict(host.ff, ff, host, cutoff=W.CUTOFF)
+    beta = 1 / (U.boltzmann * W.TEMPERATURE)
+    ts = []
+    for k in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        V = generate_effective_potential(mol, grid, beta, gens, degree=degree, as_tensor=True)
+        e1.record()
+        torch.cuda.synchronize()
+        if k > 0:
+            ts.append(e0.elapsed_time(e1))
+    ms = float(np.median(ts))
+    # the interpolated form: 0.15 A derivative tables of the cell per site type
+    # (the reference's interpolate=True recipe), then the interpolated average
+    from cmmdft_amd.extpot import get_external_potential_derivatives, Interpolator
+    h = 0.15 * U.angstrom
+    egrid = Grid(host.rvecs, spacing=h)
+    tt, ti = [], []
+    for k in range(3):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record()
+        igens = {}
+        for atom, (sf, ef) in ff.items():
+            t = get_external_potential_derivatives(egrid, host.ff, sf, ef, host, egrid.spacings, cutoff=W.CUTOFF,
+                                                   as_tensor=True)
+            igens[atom] = Interpolator(t, grid.points[0, 0, 0, :3], [h, h, h]).tricubic
+            igens[atom].table()
+        e1.record()
+        Vi = generate_effective_potential(mol, grid, beta, igens, degree=degree, as_tensor=True)
+        e2.record()
+        torch.cuda.synchronize()
+        if k > 0:
+            tt.append(e0.elapsed_time(e1))
+            ti.append(e1.elapsed_time(e2))
+        del igens, t
+    nrot = len(combined_rotations(degree)[1])
+    npts = int(np.prod(shape))
+    sites = npts * nrot * mol.natom
+    # in-cut-off pairs per site: host atoms in the cut-off sphere (uniform estimate)
+    vol = abs(np.linalg.det(host.rvecs))
+    pairs = sites * host.natom * (4 / 3 * np.pi * W.CUTOFF ** 3) / vol
+    return {'workload': 'rigid CO2 (3 sites) in the case-2 host at %dx%dx%d, 300 K, %d orientations '
+                        '(Lebedev degree %d x %d about the axis), r_c 12 A' % (shape + (nrot, degree, degree)),
+            'ms': ms, 'orientations': nrot, 'site_evaluations_per_s': sites / (ms * 1e-3),
+            'in_cutoff_pairs_per_s_est': pairs / (ms * 1e-3),
+            'fp64_gflops_est': pairs * 30 / (ms * 1e-3) / 1e9,
+            'min_kjmol': float(V.min()) / U.kjmol,
+            'interpolated': {'tables_ms': float(np.median(tt)), 'average_ms': float(np.median(ti)),
+                             'table_grid': list(egrid.npoints),
+                             'site_evaluations_per_s': sites / (float(np.median(ti)) * 1e-3),
+                             'min_kjmol': float(Vi.min()) / U.kjmol}}
+
+
 def uneven_leg(nframe=16, npress=8):
     """The uneven batch: synthetic frameworks k = 0..nframe-1 (orthorhombic
     edges 18-40 A, 0.25 A grids 72..160 per axis, 100-600 atoms of 4 LJ types,
@@ -274,6 +341,8 @@ def main():
     ap.add_argument('--cpu-sample', type=int, default=4, help='oracle iterations timed (0 skips the CPU leg)')
     ap.add_argument('--no-adjoint', action='store_true', help='skip the forward-vs-adjoint timing')
     ap.add_argument('--no-uneven', action='store_true', help='skip the uneven-batch (16 frameworks) leg')
+    ap.add_argument('--no-effpot', action='store_true', help='skip the non-spherical (CO2) effective-potential leg')
+    ap.add_argument('--effpot-only', action='store_true', help='only the CO2 effective-potential leg (profiling)')
     ap.add_argument('--passes-only', action='store_true',
                     help='short run for hardware counters: the functional passes at the bench grid, then a 256 MB '
                          'device copy as the byte-count reference')
@@ -288,6 +357,9 @@ def main():
     shape = (args.grid,) * 3
     if args.passes_only:
         return passes_only(shape)
+    if args.effpot_only:
+        print(json.dumps({'effective_potential': effpot_leg(shape)}))
+        return
     mus = chempots(32)
     host, prog = build_program(shape)
     torch.cuda.synchronize()
@@ -367,6 +439,7 @@ def main():
 
     adjoint = None if args.no_adjoint else adjoint_leg(shape)
     uneven = None if args.no_uneven else uneven_leg()
+    effpot = None if args.no_effpot else effpot_leg(shape)
 
     cpu = None
     if args.cpu_sample > 0:
@@ -402,6 +475,7 @@ def main():
         'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
         'adjoint': adjoint,
         'uneven_batch': uneven,
+        'effective_potential': effpot,
     }
     print(json.dumps(line))
 

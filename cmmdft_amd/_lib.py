@@ -99,6 +99,18 @@ EXPORTS = {
     'cmm_lj_grid_grad': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
                           C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_effpot': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                    C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
+                    C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_lj_derivs': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int,
+                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p],
+                      C.c_int),
+    'cmm_interp_table': ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_interp_points': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_longlong,
+                           C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_effpot_interp': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_int, C.c_void_p,
+                           C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
     'cmm_lj_points': ([C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                        C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
 }
@@ -384,6 +396,84 @@ def lj_points(points_dev, M, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff
     check(lib.cmm_lj_points(_ptr(points_dev), int(M), rv.ctypes.data_as(C.c_void_p), _ptr(atoms_dev), int(natom),
                             float(sigma_f), float(epsilon_f), float(cutoff), int(images), float(limit), _ptr(out),
                             C.c_void_p(s.cuda_stream)))
+
+
+def effpot(npoints, points_dev, M, rvecs, atoms_dev, natom, gtype, gsig, geps, offsets_dev, weights_dev, nrot, dmax,
+           beta, cutoff, images, nanval, out, stream=None):
+    """Orientationally averaged potential (cmm_effpot): on the grid of
+    ``npoints`` when ``points_dev`` is None, else at the M points."""
+    torch = _torch()
+    lib = load()
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    gt = np.ascontiguousarray(np.asarray(gtype, dtype=np.int32))
+    gs = np.ascontiguousarray(np.asarray(gsig, dtype=np.float64))
+    ge = np.ascontiguousarray(np.asarray(geps, dtype=np.float64))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    n = [int(v) for v in npoints] if npoints is not None else [1, 1, 1]
+    check(lib.cmm_effpot(n[0], n[1], n[2], _ptr(points_dev) if points_dev is not None else None, int(M),
+                         rv.ctypes.data_as(C.c_void_p), _ptr(atoms_dev), int(natom), len(gt),
+                         gt.ctypes.data_as(C.c_void_p), len(gs), gs.ctypes.data_as(C.c_void_p),
+                         ge.ctypes.data_as(C.c_void_p), _ptr(offsets_dev), _ptr(weights_dev), int(nrot), float(dmax),
+                         float(beta), float(cutoff), int(images), float(nanval), _ptr(out),
+                         C.c_void_p(s.cuda_stream)))
+
+
+def lj_derivs(npoints, points_dev, M, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff, images, spacings, vmax,
+              out, stream=None):
+    """cmm_lj_derivs: out (8, npts) on the grid of ``npoints`` (points_dev None) or at M points."""
+    torch = _torch()
+    lib = load()
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    h = np.ascontiguousarray(np.asarray(spacings, dtype=np.float64).reshape(3))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    n = [int(v) for v in npoints] if npoints is not None else [1, 1, 1]
+    check(lib.cmm_lj_derivs(n[0], n[1], n[2], _ptr(points_dev) if points_dev is not None else None, int(M),
+                            rv.ctypes.data_as(C.c_void_p), _ptr(atoms_dev), int(natom), float(sigma_f),
+                            float(epsilon_f), float(cutoff), int(images), h.ctypes.data_as(C.c_void_p), float(vmax),
+                            _ptr(out), C.c_void_p(s.cuda_stream)))
+
+
+def interp_table(values_dev, n, estimate, table, stream=None):
+    torch = _torch()
+    lib = load()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_interp_table(_ptr(values_dev), int(n[0]), int(n[1]), int(n[2]), int(estimate), _ptr(table),
+                               C.c_void_p(s.cuda_stream)))
+
+
+def interp_points(table, n, origin, spacing, mode, points_dev, M, out, stream=None):
+    torch = _torch()
+    lib = load()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    o = np.ascontiguousarray(np.asarray(origin, dtype=np.float64).reshape(3))
+    h = np.ascontiguousarray(np.asarray(spacing, dtype=np.float64).reshape(3))
+    check(lib.cmm_interp_points(_ptr(table), _ints([int(v) for v in n]), o.ctypes.data_as(C.c_void_p),
+                                h.ctypes.data_as(C.c_void_p), int(mode), _ptr(points_dev), int(M), _ptr(out),
+                                C.c_void_p(s.cuda_stream)))
+
+
+def effpot_interp(npoints, points_dev, M, rvecs, gtype, tables, tn, torigin, tspacing, tmode, offsets_dev,
+                  weights_dev, nrot, beta, nanval, out, stream=None):
+    """cmm_effpot_interp: ``tables`` a list of device tables (one per guest
+    type) with their sizes ``tn``, origins, spacings and modes."""
+    torch = _torch()
+    lib = load()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    nt = len(tables)
+    ptrs = (C.c_void_p * nt)(*[t.data_ptr() for t in tables])
+    gt = np.ascontiguousarray(np.asarray(gtype, dtype=np.int32))
+    tn_ = np.ascontiguousarray(np.asarray(tn, dtype=np.int32).reshape(nt, 3))
+    to = np.ascontiguousarray(np.asarray(torigin, dtype=np.float64).reshape(nt, 3))
+    th = np.ascontiguousarray(np.asarray(tspacing, dtype=np.float64).reshape(nt, 3))
+    tm = np.ascontiguousarray(np.asarray(tmode, dtype=np.int32).reshape(nt))
+    rv = np.ascontiguousarray(np.asarray(rvecs if rvecs is not None else np.eye(3), dtype=np.float64).reshape(9))
+    n = [int(v) for v in npoints] if npoints is not None else [1, 1, 1]
+    check(lib.cmm_effpot_interp(n[0], n[1], n[2], _ptr(points_dev) if points_dev is not None else None, int(M),
+                                rv.ctypes.data_as(C.c_void_p), len(gt), gt.ctypes.data_as(C.c_void_p), nt,
+                                C.cast(ptrs, C.c_void_p), tn_.ctypes.data_as(C.c_void_p),
+                                to.ctypes.data_as(C.c_void_p), th.ctypes.data_as(C.c_void_p),
+                                tm.ctypes.data_as(C.c_void_p), _ptr(offsets_dev), _ptr(weights_dev), int(nrot),
+                                float(beta), float(nanval), _ptr(out), C.c_void_p(s.cuda_stream)))
 
 
 def lj_grid_grad(npoints, rvecs, atoms_dev, types_dev, natom, ntypes, sigma_f, epsilon_f, cutoff, images, limit, w,

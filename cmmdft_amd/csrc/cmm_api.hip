@@ -3,6 +3,7 @@
 // included here so the launch templates are instantiated next to their use.
 #include "cmm_kernels.hip"
 #include "cmm_lj.hip"
+#include "cmm_effpot.hip"
 #include "cmm_adjoint.hip"
 #include "cmm_mfa.hip"
 #include "cmm_calc.hip"
@@ -1008,6 +1009,168 @@ int cmm_lj_points(const double* points, long long M, const double* rvecs, const 
 
 static dim3 lj_grid_dims(int nx, int ny, int nz) {
   return dim3((nz + LJ_BZ - 1) / LJ_BZ, (ny + LJ_BY - 1) / LJ_BY, (nx + LJ_BX - 1) / LJ_BX);
+}
+
+int cmm_effpot(int nx, int ny, int nz, const double* points, long long M, const double* rvecs, const double* atoms,
+               int natom, int ng, const int* gtype, int nt, const double* gsig, const double* geps,
+               const double* offsets, const double* weights, int nrot, double dmax, double beta, double cutoff,
+               int images, double nanval, double* V, void* stream) {
+  if (!V || !gtype || !gsig || !geps || !offsets || !weights || (natom > 0 && !atoms) || natom < 0)
+    return fail(CMM_E_ARG, "null argument");
+  if (ng < 1 || ng > EP_MAXG) return fail(CMM_E_ARG, "guest atom count must be in [1, 16]");
+  if (nt < 1 || nt > EP_MAXT) return fail(CMM_E_ARG, "guest atom type count must be in [1, 4]");
+  if (nrot < 1) return fail(CMM_E_ARG, "no orientations");
+  if (!(beta > 0.0)) return fail(CMM_E_ARG, "beta must be positive");
+  if (!(dmax >= 0.0)) return fail(CMM_E_ARG, "dmax must be non-negative");
+  EffDev e;
+  std::memset(&e, 0, sizeof(e));
+  const bool grid = points == nullptr;
+  if (!grid && M < 0) return fail(CMM_E_ARG, "negative point count");
+  int rc = lj_setup(grid ? nx : 1, grid ? ny : 1, grid ? nz : 1, rvecs, 0.0, 0.0, cutoff, images, 0.0, e.d);
+  if (rc) return rc;
+  e.nrot = nrot;
+  e.ng = ng;
+  e.nt = nt;
+  for (int i = 0; i < ng; ++i) {
+    if (gtype[i] < 0 || gtype[i] >= nt) return fail(CMM_E_ARG, "guest atom type out of range");
+    e.gtype[i] = gtype[i];
+  }
+  for (int t = 0; t < nt; ++t) {
+    e.gsig[t] = gsig[t];
+    e.geps[t] = geps[t];
+  }
+  e.dmax = dmax;
+  e.beta = beta;
+  e.nanval = nanval;
+  hipStream_t s = (hipStream_t)stream;
+  if (grid) {
+    dim3 g((nz + EP_BZ - 1) / EP_BZ, (ny + EP_BY - 1) / EP_BY, (nx + EP_BX - 1) / EP_BX);
+    if (images) hipLaunchKernelGGL((k_effpot<true, true>), g, dim3(EP_NT), 0, s, atoms, natom, e, offsets, weights,
+                                   nullptr, 0LL, V);
+    else hipLaunchKernelGGL((k_effpot<true, false>), g, dim3(EP_NT), 0, s, atoms, natom, e, offsets, weights,
+                            nullptr, 0LL, V);
+  } else {
+    if (M == 0) return CMM_OK;
+    dim3 g((unsigned)((M + EP_NT - 1) / EP_NT));
+    if (images) hipLaunchKernelGGL((k_effpot<false, true>), g, dim3(EP_NT), 0, s, atoms, natom, e, offsets, weights,
+                                   points, M, V);
+    else hipLaunchKernelGGL((k_effpot<false, false>), g, dim3(EP_NT), 0, s, atoms, natom, e, offsets, weights,
+                            points, M, V);
+  }
+  LAUNCHCHK("k_effpot");
+  return CMM_OK;
+}
+
+int cmm_lj_derivs(int nx, int ny, int nz, const double* points, long long M, const double* rvecs,
+                  const double* atoms, int natom, double sigma_f, double epsilon_f, double cutoff, int images,
+                  const double* spacings, double vmax, double* out, void* stream) {
+  if (!out || !spacings || (natom > 0 && !atoms) || natom < 0) return fail(CMM_E_ARG, "null argument");
+  const bool grid = points == nullptr;
+  if (!grid && M < 0) return fail(CMM_E_ARG, "negative point count");
+  LJDev d;
+  int rc = lj_setup(grid ? nx : 1, grid ? ny : 1, grid ? nz : 1, rvecs, sigma_f, epsilon_f, cutoff, images, 0.0, d);
+  if (rc) return rc;
+  LJDerivArgs da;
+  for (int a = 0; a < 3; ++a) da.h[a] = spacings[a];
+  da.vmax = vmax;
+  hipStream_t s = (hipStream_t)stream;
+  if (grid) {
+    dim3 g((nz + LJ_BZ - 1) / LJ_BZ, (ny + LJ_BY - 1) / LJ_BY, (nx + LJ_BX - 1) / LJ_BX);
+    hipLaunchKernelGGL((k_lj_derivs<true>), g, dim3(LJ_NT), 0, s, atoms, natom, d, da, nullptr, 0LL, out);
+  } else {
+    if (M == 0) return CMM_OK;
+    dim3 g((unsigned)((M + LJD_NT - 1) / LJD_NT));
+    hipLaunchKernelGGL((k_lj_derivs<false>), g, dim3(LJD_NT), 0, s, atoms, natom, d, da, points, M, out);
+  }
+  LAUNCHCHK("k_lj_derivs");
+  return CMM_OK;
+}
+
+int cmm_interp_table(const double* values, int nx, int ny, int nz, int estimate, double* table, void* stream) {
+  if (!values || !table) return fail(CMM_E_ARG, "null argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(CMM_E_ARG, "bad sizes");
+  const long long n = (long long)nx * ny * nz;
+  dim3 g((unsigned)((n + 255) / 256));
+  if (estimate) hipLaunchKernelGGL(k_interp_table<1>, g, dim3(256), 0, (hipStream_t)stream, values, nx, ny, nz, table);
+  else hipLaunchKernelGGL(k_interp_table<0>, g, dim3(256), 0, (hipStream_t)stream, values, nx, ny, nz, table);
+  LAUNCHCHK("k_interp_table");
+  return CMM_OK;
+}
+
+static int interp_tab(const double* table, const int* n, const double* origin, const double* spacing, int mode,
+                      InterpTab& T) {
+  if (!table || !n || !origin || !spacing) return fail(CMM_E_ARG, "null argument");
+  if (mode != 0 && mode != 1) return fail(CMM_E_ARG, "interpolation mode must be 0 (tricubic) or 1 (trilinear)");
+  T.tab = table;
+  for (int a = 0; a < 3; ++a) {
+    if (n[a] < 1) return fail(CMM_E_ARG, "bad table sizes");
+    if (!(spacing[a] > 0.0)) return fail(CMM_E_ARG, "table spacing must be positive");
+    T.n[a] = n[a];
+    T.origin[a] = origin[a];
+    T.h[a] = spacing[a];
+  }
+  T.mode = mode;
+  return CMM_OK;
+}
+
+int cmm_interp_points(const double* table, const int* n, const double* origin, const double* spacing, int mode,
+                      const double* points, long long M, double* out, void* stream) {
+  if (!points || !out || M < 0) return fail(CMM_E_ARG, "null argument");
+  InterpTab T;
+  int rc = interp_tab(table, n, origin, spacing, mode, T);
+  if (rc) return rc;
+  if (M == 0) return CMM_OK;
+  hipLaunchKernelGGL(k_interp_points, dim3((unsigned)((M + 255) / 256)), dim3(256), 0, (hipStream_t)stream, T, points,
+                     M, out);
+  LAUNCHCHK("k_interp_points");
+  return CMM_OK;
+}
+
+int cmm_effpot_interp(int nx, int ny, int nz, const double* points, long long M, const double* rvecs, int ng,
+                      const int* gtype, int nt, const double* const* tables, const int* tn, const double* torigin,
+                      const double* tspacing, const int* tmode, const double* offsets, const double* weights,
+                      int nrot, double beta, double nanval, double* V, void* stream) {
+  if (!V || !gtype || !tables || !tn || !torigin || !tspacing || !tmode || !offsets || !weights)
+    return fail(CMM_E_ARG, "null argument");
+  if (ng < 1 || ng > EP_MAXG) return fail(CMM_E_ARG, "guest atom count must be in [1, 16]");
+  if (nt < 1 || nt > EP_MAXT) return fail(CMM_E_ARG, "guest atom type count must be in [1, 4]");
+  if (nrot < 1) return fail(CMM_E_ARG, "no orientations");
+  if (!(beta > 0.0)) return fail(CMM_E_ARG, "beta must be positive");
+  const bool grid = points == nullptr;
+  if (grid && (!rvecs || nx < 1 || ny < 1 || nz < 1)) return fail(CMM_E_ARG, "bad grid");
+  if (!grid && M < 0) return fail(CMM_E_ARG, "negative point count");
+  EffInterpDev e;
+  std::memset(&e, 0, sizeof(e));
+  for (int t = 0; t < nt; ++t) {
+    int rc = interp_tab(tables[t], tn + 3 * t, torigin + 3 * t, tspacing + 3 * t, tmode[t], e.tab[t]);
+    if (rc) return rc;
+  }
+  e.nrot = nrot;
+  e.ng = ng;
+  for (int i = 0; i < ng; ++i) {
+    if (gtype[i] < 0 || gtype[i] >= nt) return fail(CMM_E_ARG, "guest atom type out of range");
+    e.gtype[i] = gtype[i];
+  }
+  e.beta = beta;
+  e.nanval = nanval;
+  hipStream_t s = (hipStream_t)stream;
+  if (grid) {
+    e.nx = nx;
+    e.ny = ny;
+    e.nz = nz;
+    std::memcpy(e.rv, rvecs, 9 * sizeof(double));
+    e.inv_n[0] = 1.0 / nx;
+    e.inv_n[1] = 1.0 / ny;
+    e.inv_n[2] = 1.0 / nz;
+    dim3 g((nz + EP_BZ - 1) / EP_BZ, (ny + EP_BY - 1) / EP_BY, (nx + EP_BX - 1) / EP_BX);
+    hipLaunchKernelGGL((k_effpot_interp<true>), g, dim3(EP_NT), 0, s, e, offsets, weights, nullptr, 0LL, V);
+  } else {
+    if (M == 0) return CMM_OK;
+    dim3 g((unsigned)((M + EP_NT - 1) / EP_NT));
+    hipLaunchKernelGGL((k_effpot_interp<false>), g, dim3(EP_NT), 0, s, e, offsets, weights, points, M, V);
+  }
+  LAUNCHCHK("k_effpot_interp");
+  return CMM_OK;
 }
 
 int cmm_lj_grad_workspace(int nx, int ny, int nz, int ntypes, long long* ndoubles) {

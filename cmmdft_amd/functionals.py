@@ -21,7 +21,7 @@ import numpy as np
 from . import units as U
 from ._device import torch_mod, as_device, like_input
 from ._lib import CmmFunctional, S_PHI, S_FEX, S_N3MAX, S_FMFA
-from .system import NanoporousHost, SphericalLJGuest
+from .system import NanoporousHost, SphericalLJGuest, NonSphericalGuest
 
 __all__ = ['FreeEnergy', 'Functional', 'HardSphereFunctional', 'WDAVFunctional', 'MFAFunctional',
            'ExternalPotential', 'density_from', 'build_functional', 'device_parts', 'configure_plan']
@@ -428,14 +428,60 @@ class ExternalPotential(Functional):
         return e
 
     def generate_potential(self, temperature=None, rewrite=False, images=False):
-        from .extpot import get_external_potential
-        if not isinstance(self.guest, SphericalLJGuest):
-            raise NotImplementedError('only spherical LJ guests are supported')
-        V = get_external_potential(self.grid, self.host.ff, self.guest.sigma, self.guest.epsilon, self.host.mol,
-                                   cutoff=self.cutoff, images=images, as_tensor=True)
-        self.generated = {'images': bool(images)}
+        """The host's potential on the grid (functionals.py:1030-1066): the LJ
+        grid sum for a spherical guest; for a non-spherical guest
+        (``NonSphericalGuest``, ``DualModelGuest``) the orientational average
+        of ``generate_effective_potential`` at ``temperature`` with
+        ``degree`` Lebedev directions, on the device."""
+        from .extpot import get_external_potential, get_external_potential_dict, generate_effective_potential
+        if isinstance(self.guest, NonSphericalGuest):
+            assert temperature is not None, 'the effective potential of a non-spherical guest needs a temperature'
+            beta = 1 / temperature / U.boltzmann
+            if self.interpolate:
+                gens = self._interpolators(images)
+            else:
+                gens = get_external_potential_dict(self.host.ff, self.guest.ff, self.host.mol, cutoff=self.cutoff,
+                                                   images=images)
+            V = generate_effective_potential(self.guest.mol, self.grid, beta, gens, degree=self.degree,
+                                             as_tensor=True)
+            self.generated = {'images': bool(images), 'temperature': temperature}
+        elif isinstance(self.guest, SphericalLJGuest):
+            V = get_external_potential(self.grid, self.host.ff, self.guest.sigma, self.guest.epsilon, self.host.mol,
+                                       cutoff=self.cutoff, images=images, as_tensor=True)
+            self.generated = {'images': bool(images)}
+        else:
+            raise NotImplementedError('no external potential for guests of type %s' % type(self.guest).__name__)
         self._dev = V.reshape(self.grid.npoints)
         self.potential = self._dev.cpu().numpy()
+
+    def _interpolators(self, images=False):
+        """The reference's interpolate=True set-up (functionals.py:1035-1049):
+        per guest atom type, the LJ field and its derivatives on a grid of
+        the host cell at 0.15 A spacing (npoints = ceil(L / 0.15 A), so the
+        actual spacing is L / npoints), interpolated tricubically with origin
+        the first point of this grid and a spacing of exactly 0.15 A, as the
+        reference does."""
+        from .extpot import get_external_potential_derivatives, Interpolator
+        from .system import Grid
+        h = 0.15 * U.angstrom
+        epot_grid = Grid(self.host.mol.cell, spacing=h)
+        origin = self.grid.points[0, 0, 0, :3]
+        gens = {}
+        for atom, (sigmaff, epsilonff) in self.guest.ff.items():
+            vals = get_external_potential_derivatives(epot_grid, self.host.ff, sigmaff, epsilonff, self.host.mol,
+                                                      epot_grid.spacings, cutoff=self.cutoff, images=images,
+                                                      as_tensor=True)
+            gens[atom] = Interpolator(vals, origin, [h, h, h]).tricubic
+        return gens
+
+    def set_temperature(self, temperature, **kwargs):
+        """A non-spherical guest's potential depends on the temperature
+        (functionals.py:1020-1026): regenerate it when it was generated at
+        another one."""
+        if isinstance(self.guest, NonSphericalGuest) and self.generated and \
+                self.generated.get('temperature') not in (None, temperature):
+            self.generate_potential(temperature, images=self.generated.get('images', False))
+            self.clamp()
 
     def clamp(self):
         """Non-finite and larger-than-limit values -> limit

@@ -23,8 +23,10 @@ class Cell:
     """Minimal stand-in for yaff's Cell: ``rvecs`` rows are the cell vectors."""
 
     def __init__(self, rvecs):
-        self.rvecs = np.asarray(rvecs, dtype=np.float64).reshape(3, 3)
-        self.nvec = 3
+        r = np.asarray(rvecs, dtype=np.float64)
+        # a molecule's checkpoint has no cell vectors (kind=fltar 0,3)
+        self.rvecs = r.reshape(0, 3) if r.size == 0 else r.reshape(3, 3)
+        self.nvec = len(self.rvecs)
 
     @property
     def volume(self):
@@ -123,10 +125,12 @@ class ChkSystem:
     @classmethod
     def from_file(cls, fn):
         d = read_chk(fn)
-        for k in ('pos', 'rvecs', 'ffatypes', 'ffatype_ids'):
+        for k in ('pos', 'ffatypes', 'ffatype_ids'):
             if d.get(k) is None:
                 raise ValueError('%s has no %s' % (fn, k))
-        return cls(d['pos'], d['rvecs'], d['ffatypes'], d['ffatype_ids'], d.get('masses'), d.get('numbers'),
+        # a molecule may have no cell vectors at all
+        rvecs = d['rvecs'] if d.get('rvecs') is not None else np.zeros((0, 3))
+        return cls(d['pos'], rvecs, d['ffatypes'], d['ffatype_ids'], d.get('masses'), d.get('numbers'),
                    d.get('charges'))
 
 
@@ -203,7 +207,7 @@ class SphericalLJGuest(Guest):
     T* = kT / epsilon (system.py:141-158)."""
 
     def __init__(self, name, mass, sigma, epsilon):
-        super().__init__(name, mass)
+        Guest.__init__(self, name, mass)
         self.sigma = sigma
         self.epsilon = epsilon
         self.natom = 1
@@ -215,6 +219,50 @@ class SphericalLJGuest(Guest):
         Tt = U.boltzmann * temperature / self.epsilon
         Rhs = self.sigma * (1 + 0.2977 * Tt) / (1 + 0.33163 * Tt + 0.0010477 * Tt ** 2) / 2
         return Rhs, self.sigma
+
+
+class NonSphericalGuest(Guest):
+    """Rigid multi-site guest (system.py:162-212): atoms, masses and force-field
+    types from a checkpoint file ``chk`` (or a system object), LJ parameters
+    of its atom types from the yaff parameter file ``par`` (or a dict).  The
+    mass is the sum of the atomic masses."""
+
+    def __init__(self, name, chk, par):
+        from .extpot import read_pars_file
+        if isinstance(chk, (str, os.PathLike)):
+            self.chk = chk
+            self.mol = ChkSystem.from_file(chk)
+        else:
+            self.chk = chk
+            self.mol = chk
+        self.par = par
+        self.ff = read_pars_file(par) if isinstance(par, (str, os.PathLike)) else dict(par)
+        self.natom = self.mol.natom
+        if self.mol.masses is None:
+            raise ValueError('Masses not defined in guest chk file, cannot compute total mass for NonSphericalGuest')
+        Guest.__init__(self, name, float(np.sum(self.mol.masses)))
+
+    def copy(self):
+        return type(self)(self.name, self.chk, self.par)
+
+    def _calculate_hardsphere_radius(self, temperature, **kwargs):
+        raise NotImplementedError('Using generic NonSphericalGuest class, cannot compute hardsphere radius, '
+                                  'use DualModelGuest class')
+
+
+class DualModelGuest(SphericalLJGuest, NonSphericalGuest):
+    """Guest with a spherical LJ model (hard-sphere radius, mean field) and a
+    multi-site model (external potential), system.py:214-229."""
+
+    def __init__(self, name, mass, sigma, epsilon, chk, par):
+        SphericalLJGuest.__init__(self, name, mass, sigma, epsilon)
+        NonSphericalGuest.__init__(self, name, chk, par)
+
+    def copy(self):
+        return type(self)(self.name, self.mass, self.sigma, self.epsilon, self.chk, self.par)
+
+    def _calculate_hardsphere_radius(self, temperature, **kwargs):
+        return SphericalLJGuest._calculate_hardsphere_radius(self, temperature, **kwargs)
 
 
 class Grid:

@@ -168,6 +168,19 @@ def host_case5(k):
     return h
 
 
+def guest_co2():
+    """Rigid three-site CO2 (TraPPE: C-O 1.16 A along z, C 2.80 A / 27.0 K,
+    O 3.05 A / 79.0 K, masses 12.011 and 15.999 amu) as a checkpoint-like
+    system and its LJ parameters {ffatype: (sigma, epsilon)}."""
+    from .system import ChkSystem
+    d = 1.16 * U.angstrom
+    pos = np.array([[0.0, 0.0, d], [0.0, 0.0, 0.0], [0.0, 0.0, -d]])
+    mol = ChkSystem(pos, np.zeros((0, 3)), ['O_CD', 'C_CD'], [0, 1, 0],
+                    masses=np.array([15.999, 12.011, 15.999]) * U.amu)
+    ff = {'O_CD': (3.05 * U.angstrom, 79.0 * U.boltzmann), 'C_CD': (2.80 * U.angstrom, 27.0 * U.boltzmann)}
+    return mol, ff
+
+
 def pressures_isotherm(n=32, lo=1e-2, hi=1e2):
     return np.logspace(np.log10(lo), np.log10(hi), n) * U.bar
 

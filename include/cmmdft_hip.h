@@ -38,6 +38,14 @@
  *                                                      cmmdft/solver.py:103-119
  *  cmm_lj_grid/_points   get_external_potential / lennard_jones
  *                                                      cmmdft/extpot_calculator.py:156-222
+ *  cmm_effpot            generate_effective_potential / effective_potential_vectorized
+ *                        (orientational average for non-spherical guests)
+ *                                                      cmmdft/extpot_calculator.py:572-646
+ *  cmm_lj_derivs         get_external_potential_derivatives  cmmdft/extpot_calculator.py:264-346
+ *  cmm_interp_table/_points, cmm_effpot_interp
+ *                        Interpolator (tricubic, trilinear, estimated
+ *                        derivatives) and the interpolated effective potential
+ *                                                      cmmdft/extpot_calculator.py:391-548, 572-646
  *  cmm_lj_grid_grad      lennard_jones(derivative=True) (the reference's
  *                        radial derivatives; here with respect to the
  *                        host-type sigma/epsilon)     cmmdft/extpot_calculator.py:156-183
@@ -261,6 +269,59 @@ int cmm_lj_grid(int nx, int ny, int nz, const double* rvecs, const double* atoms
 /* the same sum at M explicit points (points: M x 3, device), V[M] */
 int cmm_lj_points(const double* points, long long M, const double* rvecs, const double* atoms, int natom,
                   double sigma_f, double epsilon_f, double cutoff, int images, double limit, double* V, void* stream);
+
+/* Orientationally averaged potential of a rigid guest of ng atoms (types
+ * gtype[i] < nt, LJ parameters gsig/geps per type):
+ *   V[p] = -(1/beta) log sum_r weights[r] exp(-beta sum_i V_gtype[i](p + offsets[r][i]))
+ * with V_t the LJ field of the host atoms (x, y, z, sigma, epsilon rows) for
+ * a guest atom of type t, as in cmm_lj_grid; a NaN result becomes nanval.
+ * points == NULL: the nx x ny x nz grid of the cell; else the M points
+ * (M x 3).  offsets: device [nrot][ng][3] (COM + R_r (x_i - COM)); weights:
+ * device [nrot]; dmax >= max |offsets|.  ng <= 16, nt <= 4.
+ * Replaces generate_effective_potential with get_external_potential
+ * generators (effective_potential_vectorized)
+ *                                 cmmdft/extpot_calculator.py:572-646, 186-222 */
+int cmm_effpot(int nx, int ny, int nz, const double* points, long long M, const double* rvecs, const double* atoms,
+               int natom, int ng, const int* gtype, int nt, const double* gsig, const double* geps,
+               const double* offsets, const double* weights, int nrot, double dmax, double beta, double cutoff,
+               int images, double nanval, double* V, void* stream);
+
+/* Field and derivatives of the LJ sum for a guest atom (sigma_f, epsilon_f),
+ * on the nx x ny x nz grid of the cell (points == NULL) or at M points:
+ * out[8][npts] = V, dV/dx h0, dV/dy h1, dV/dz h2, d2V/dxdy h0 h1,
+ * d2V/dxdz h0 h2, d2V/dydz h1 h2, d3V/dxdydz h0 h1 h2 (h = spacings), V and
+ * the first derivatives clipped to +-vmax, the others zeroed where V > vmax.
+ * Replaces get_external_potential_derivatives
+ *                                 cmmdft/extpot_calculator.py:264-346 */
+int cmm_lj_derivs(int nx, int ny, int nz, const double* points, long long M, const double* rvecs,
+                  const double* atoms, int natom, double sigma_f, double epsilon_f, double cutoff, int images,
+                  const double* spacings, double vmax, double* out, void* stream);
+
+/* Interpolation table [nx][ny][nz][8] (channel fastest) from grid values
+ * [8][nx][ny][nz] (estimate = 0) or with the seven derivative channels
+ * estimated from values[0] by periodic central differences (estimate = 1).
+ * Replaces Interpolator.__init__ / estimate_all_derivatives
+ *                                 cmmdft/extpot_calculator.py:391-470 */
+int cmm_interp_table(const double* values, int nx, int ny, int nz, int estimate, double* table, void* stream);
+
+/* Tricubic (mode 0) or trilinear (mode 1) interpolation of a table at M
+ * points: fractional index (x - origin) / spacing per axis, periodic wrap.
+ * n, origin, spacing: host arrays of 3.
+ * Replaces Interpolator.tricubic_interpolation / trilinear_interpolation
+ *                                 cmmdft/extpot_calculator.py:472-548 */
+int cmm_interp_points(const double* table, const int* n, const double* origin, const double* spacing, int mode,
+                      const double* points, long long M, double* out, void* stream);
+
+/* cmm_effpot with the site potentials interpolated from one table per guest
+ * atom type (tables: host array of nt device pointers; tn, torigin,
+ * tspacing: host arrays nt x 3; tmode: host array nt).  rvecs is read for
+ * the grid form only.
+ * Replaces generate_effective_potential with interpolator generators
+ *                                 cmmdft/extpot_calculator.py:572-646 */
+int cmm_effpot_interp(int nx, int ny, int nz, const double* points, long long M, const double* rvecs, int ng,
+                      const int* gtype, int nt, const double* const* tables, const int* tn, const double* torigin,
+                      const double* tspacing, const int* tmode, const double* offsets, const double* weights,
+                      int nrot, double beta, double nanval, double* V, void* stream);
 
 /* Backward of cmm_lj_grid: grad[2t] = sum_i w_i dV_i/dsigma_t,
  * grad[2t+1] = sum_i w_i dV_i/depsilon_t for host atom types t < ntypes

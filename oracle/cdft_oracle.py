@@ -31,6 +31,13 @@ live in ``tests/test_oracle_*.py``):
                           derivative by finite differences, MBWR coefficient
                           table literal in the reference.
 * LJ external potential  single-atom analytic values, brute-force image sum.
+* Orientational average  Lebedev tables = the reference's (fixture) and exact
+                          for spherical monomials up to their degree; rotation
+                          matrices orthonormal with R e_z = the Lebedev point;
+                          one-site guest -> the spherical LJ field; high- and
+                          low-temperature limits (weighted mean, minimum); an
+                          explicit per-orientation loop with the combined
+                          matrices written as R1 diag(c - s, s + c, 1).
 * EL solvers ........... ideal-gas closed form, bulk fixed point
                           mu = kT ln(Lambda^3 rho) + f_ex'(rho), solver-
                           independence of the fixed point (Picard vs Anderson),
@@ -44,11 +51,13 @@ pinned).
 from __future__ import annotations
 
 import math
+import os
 import time
 
 import numpy as np
 import scipy.fft as sfft
 import scipy.optimize as opt
+from scipy.special import logsumexp
 
 from cmmdft_amd import units as U
 
@@ -686,6 +695,244 @@ def external_potential(points, atom_sigma, atom_epsilon, pos, rvecs, sigma_f, ep
         R = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2 + 1e-16)
         V += lennard_jones(R, sm, em, cutoff=cutoff)
     return V
+
+
+# ---------------------------------------------------------------------------
+# Non-spherical guests: orientationally averaged potential
+# (cmmdft/extpot_calculator.py:348-389, 572-646)
+# ---------------------------------------------------------------------------
+
+_LEBEDEV_FIXTURE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'tests', 'golden',
+                                'lebedev_ref.npz')
+
+
+def lebedev_table(degree):                          # AngGrid.py:150-169, 256-287
+    """The reference's Lebedev points and weights x 4 pi (its tables, kept as a
+    test fixture); degrees 3, 5, 7, 9, 11."""
+    with np.load(_LEBEDEV_FIXTURE, allow_pickle=False) as z:
+        return z['points_%d' % degree], z['weights_%d' % degree] * 4 * np.pi
+
+
+def rotation_matrix(degree, dimension):             # extpot_calculator.py:348-377
+    if dimension == 2:
+        theta = np.linspace(0, 2 * np.pi, degree, endpoint=False)
+        c, s = np.cos(theta), np.sin(theta)
+        rot_2 = np.array([[c, -s, np.zeros_like(c)], [s, c, np.zeros_like(c)],
+                          [np.zeros_like(c), np.zeros_like(c), np.ones_like(c)]])
+        return rot_2.transpose(2, 0, 1), 1 / (degree * 4 * np.pi)
+    xyz, w = lebedev_table(degree)
+    phi1 = np.arctan2(np.sqrt(xyz[:, 1] ** 2 + xyz[:, 0] ** 2), xyz[:, 2])
+    phi2 = np.arctan2(xyz[:, 1], xyz[:, 0])
+    c1, s1 = np.cos(phi1), np.sin(phi1)
+    c2, s2 = np.cos(phi2), np.sin(phi2)
+    zeros = np.zeros(len(phi1))
+    rot = np.array([[c1 * c2, -s2, s1 * c2], [c1 * s2, c2, s1 * s2], [-s1, zeros, c1]])
+    return rot.transpose(2, 0, 1), w
+
+
+def orientations(degree):                           # extpot_calculator.py:627-631
+    R1, weights1 = rotation_matrix(degree, 3)
+    R2, weights2 = rotation_matrix(degree, 2)
+    combined_rot = np.einsum('aij,bjk->abij', R1, R2).reshape(-1, 3, 3)
+    expanded_weights = np.repeat(weights1 * weights2, len(R2))
+    return combined_rot, expanded_weights
+
+
+def effective_potential(points, guest_pos, guest_masses, guest_sigma, guest_epsilon, host_sigma, host_epsilon,
+                        host_pos, rvecs, beta, degree=11, cutoff=12 * U.angstrom, limit_potential=1e4 * U.kjmol,
+                        float32=True):
+    """generate_effective_potential (:615-646) with get_external_potential
+    generators and effective_potential_vectorized (:572-613) at the (..., 3)
+    ``points``; per guest atom i its LJ parameters guest_sigma[i],
+    guest_epsilon[i] (the generator of its type).  ``float32=True`` keeps the
+    reference's float32 per-atom array (:595) and hence its float32 sum and
+    -beta*pot (a Python-float beta does not promote a float32 array); False
+    holds them in float64."""
+    gens = [lambda c, a=a: external_potential(c, host_sigma, host_epsilon, host_pos, rvecs, guest_sigma[a],
+                                              guest_epsilon[a], cutoff=cutoff) for a in range(len(guest_pos))]
+    return effective_potential_gen(points, guest_pos, guest_masses, gens, beta, degree, limit_potential, float32)
+
+
+def effective_potential_gen(points, guest_pos, guest_masses, gens, beta, degree=11, limit_potential=1e4 * U.kjmol,
+                            float32=True):
+    """The same average with one generator per guest atom (a function of
+    (M, 3) positions, e.g. an Interpolator method)."""
+    pts = np.asarray(points, dtype=np.float64)
+    shape = pts.shape[:-1]
+    position_shifts = pts.reshape(-1, 3)
+    rotations, weights = orientations(degree)
+    m = position_shifts.shape[0]
+    natom = len(guest_pos)
+    nrot = rotations.shape[0]
+    masses = np.asarray(guest_masses, dtype=np.float64).reshape(natom, 1)
+    total_mass = np.sum(masses)
+    neutral_pos = np.asarray(guest_pos)[None, :, :] + position_shifts[:, None, :]
+    COMs = np.sum(neutral_pos * masses[None, :, :], axis=1) / total_mass
+    rel_pos = neutral_pos - COMs[:, None, :]
+    rotated = np.einsum('rij,mnj->mnri', rotations, rel_pos) + COMs[:, None, None, :]
+    pot = np.zeros((m, natom, nrot), dtype=np.float32 if float32 else np.float64)
+    for a in range(natom):
+        coords = rotated[:, a, :, :].reshape(m * nrot, 3)
+        vals = gens[a](coords)
+        pot[:, a, :] = vals.reshape(m, nrot)
+    pot = np.sum(pot, axis=1)
+    log_sum = logsumexp(-float(beta) * pot, b=weights, axis=1)
+    result = -log_sum / beta
+    result = np.where(np.isnan(result), limit_potential, result)
+    return result.reshape(shape)
+
+
+def lj_derivatives(points, atom_sigma, atom_epsilon, pos, rvecs, sigma_f, epsilon_f, spacings,
+                   cutoff=12 * U.angstrom):                   # extpot_calculator.py:264-346
+    """(8, M): V and its derivatives in unit-cube form, with the clamps."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    n = len(pts)
+    Vext, dVdx, dVdy, dVdz = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    dVdxy, dVdxz, dVdyz, dVdxyz = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    X, Y, Z = pts.T
+    dx, dy, dz = spacings
+    L = np.linalg.norm(np.asarray(rvecs), axis=1)
+    for i in range(len(pos)):
+        sm = 0.5 * (atom_sigma[i] + sigma_f)
+        em = np.sqrt(atom_epsilon[i] * epsilon_f)
+        rx = X - pos[i, 0]
+        ry = Y - pos[i, 1]
+        rz = Z - pos[i, 2]
+        rx -= L[0] * (rx / L[0]).round()
+        ry -= L[1] * (ry / L[1]).round()
+        rz -= L[2] * (rz / L[2]).round()
+        R = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2 + 1e-16)
+        inside = R < cutoff
+        V, dV, ddV, dddV = (np.zeros_like(R) for _ in range(4))
+        r6 = (sm / R[inside]) ** 6
+        r12 = r6 * r6
+        rc6 = (sm / cutoff) ** 6
+        V[inside] = 4 * em * (r12 - r6) - 4 * em * (rc6 ** 2 - rc6)
+        dV[inside] = 24 * em * (r6 - 2 * r12) / R[inside] ** 2
+        ddV[inside] = 96 * em * (7 * r12 - 2 * r6) / R[inside] ** 4
+        dddV[inside] = 384 * em * (5 * r6 - 28 * r12) / R[inside] ** 6
+        Vext += V
+        dVdx += dV * rx
+        dVdy += dV * ry
+        dVdz += dV * rz
+        dVdxy += ddV * rx * ry
+        dVdxz += ddV * rx * rz
+        dVdyz += ddV * ry * rz
+        dVdxyz += dddV * rx * ry * rz
+    max_value = 1e+6 * U.kjmol
+    V_mask = Vext > max_value
+    Vext = np.clip(Vext, -max_value, max_value)
+    dVdx = np.clip(dVdx, -max_value, max_value)
+    dVdy = np.clip(dVdy, -max_value, max_value)
+    dVdz = np.clip(dVdz, -max_value, max_value)
+    dVdxy[V_mask] = 0.0
+    dVdxz[V_mask] = 0.0
+    dVdyz[V_mask] = 0.0
+    dVdxyz[V_mask] = 0.0
+    dVdx *= dx
+    dVdy *= dy
+    dVdz *= dz
+    dVdxy *= dx * dy
+    dVdxz *= dx * dz
+    dVdyz *= dy * dz
+    dVdxyz *= dx * dy * dz
+    return np.array([Vext, dVdx, dVdy, dVdz, dVdxy, dVdxz, dVdyz, dVdxyz])
+
+
+_CORNERS = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0), (0, 0, 1), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
+_DERIVS = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1)]
+
+
+def tricubic_coefficients():
+    """The 64 x 64 matrix of extpot_calculator.py:20-147: a = C X maps the 8
+    channels at the 8 corners (column corner + 8 channel) to the coefficients
+    a[i + 4j + 16k] of x^i y^j z^k.  Built here as the inverse of the
+    constraint matrix (value and derivatives of each monomial at each
+    corner); it is integer-valued and its bytes are checked against the
+    reference's table by checksum."""
+    def dmono(p, d, x):
+        if d > p:
+            return 0.0
+        c = 1
+        for q in range(d):
+            c *= p - q
+        return c * x ** (p - d)
+    B = np.zeros((64, 64))
+    for ci, c in enumerate(_CORNERS):
+        for di, d in enumerate(_DERIVS):
+            for i in range(4):
+                for j in range(4):
+                    for k in range(4):
+                        B[ci + 8 * di, i + 4 * j + 16 * k] = dmono(i, d[0], c[0]) * dmono(j, d[1], c[1]) * \
+                            dmono(k, d[2], c[2])
+    return np.round(np.linalg.inv(B))
+
+
+class Interpolator:                                 # extpot_calculator.py:391-548
+    def __init__(self, grid_values, grid_origin, grid_spacing):
+        self.grid_values = np.asarray(grid_values, dtype=np.float64)
+        self.origin = np.array(grid_origin)
+        self.spacing = np.array(grid_spacing)
+        self.coeff = tricubic_coefficients()
+
+    def _frac(self, positions):
+        s = (positions - self.origin) / self.spacing
+        ix = np.floor(s).astype(int)
+        return ix, s - ix
+
+    def _corners(self, ix, Nx, Ny, Nz):
+        ix0, iy0, iz0 = ix[:, 0] % Nx, ix[:, 1] % Ny, ix[:, 2] % Nz
+        return [((ix0 + dx) % Nx, (iy0 + dy) % Ny, (iz0 + dz) % Nz) for dx, dy, dz in _CORNERS]
+
+    @staticmethod
+    def estimate_all_derivatives(V):
+        Vx = (np.roll(V, -1, axis=0) - np.roll(V, 1, axis=0)) / (2)
+        Vy = (np.roll(V, -1, axis=1) - np.roll(V, 1, axis=1)) / (2)
+        Vz = (np.roll(V, -1, axis=2) - np.roll(V, 1, axis=2)) / (2)
+        Vxy = (np.roll(Vx, -1, axis=1) - np.roll(Vx, 1, axis=1)) / (2)
+        Vxz = (np.roll(Vx, -1, axis=2) - np.roll(Vx, 1, axis=2)) / (2)
+        Vyz = (np.roll(Vy, -1, axis=2) - np.roll(Vy, 1, axis=2)) / (2)
+        Vxyz = (np.roll(Vxy, -1, axis=2) - np.roll(Vxy, 1, axis=2)) / (2)
+        return np.stack([V, Vx, Vy, Vz, Vxy, Vxz, Vyz, Vxyz], axis=0)
+
+    def trilinear(self, positions):
+        positions = np.atleast_2d(positions)
+        values = self.grid_values[0] if self.grid_values.ndim == 4 else self.grid_values
+        Nx, Ny, Nz = values.shape
+        ix, rd = self._frac(positions)
+        rx, ry, rz = rd[:, 0], rd[:, 1], rd[:, 2]
+        V = np.zeros((positions.shape[0], 8))
+        for c, (xi, yi, zi) in enumerate(self._corners(ix, Nx, Ny, Nz)):
+            V[:, c] = values[xi, yi, zi]
+        return np.sum(V * np.array([(1 - rx) * (1 - ry) * (1 - rz), rx * (1 - ry) * (1 - rz),
+                                    (1 - rx) * ry * (1 - rz), rx * ry * (1 - rz), (1 - rx) * (1 - ry) * rz,
+                                    rx * (1 - ry) * rz, (1 - rx) * ry * rz, rx * ry * rz]).T, axis=1)
+
+    def tricubic(self, positions, estimate_derivatives=False):
+        positions = np.atleast_2d(positions)
+        if self.grid_values.ndim == 3:
+            values = self.estimate_all_derivatives(self.grid_values)
+        elif estimate_derivatives:
+            values = self.estimate_all_derivatives(self.grid_values[0])
+        else:
+            values = self.grid_values
+        Nx, Ny, Nz = values.shape[1:]
+        N = positions.shape[0]
+        ix, rd = self._frac(positions)
+        rx, ry, rz = rd[:, 0], rd[:, 1], rd[:, 2]
+        X = np.zeros((N, 64))
+        for c, (xi, yi, zi) in enumerate(self._corners(ix, Nx, Ny, Nz)):
+            for d in range(8):
+                X[:, c + d * 8] = values[d, xi, yi, zi]
+        result = np.zeros(N)
+        a = X @ self.coeff.T
+        for i in range(4):
+            ui = rx ** i
+            for j in range(4):
+                vj = ry ** j
+                for k in range(4):
+                    result += a[:, i + 4 * j + 16 * k] * ui * vj * rz ** k
+        return result
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,308 @@
+"""Orientationally averaged external potential of non-spherical guests on the
+device (cmm_effpot) against the oracle's restatement of the reference's
+generate_effective_potential / effective_potential_vectorized
+(cmmdft/extpot_calculator.py:572-646), and a converged density with a
+DualModelGuest against the oracle's solve."""
+import os
+
+import numpy as np
+import pytest
+
+from cmmdft_amd import units as U, workloads as W
+from oracle import cdft_oracle as O
+from helpers import T, rho_rel_err
+
+pytestmark = pytest.mark.gpu
+
+INPUTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'inputs')
+BETA = 1 / (U.boltzmann * T)
+LIMIT = 1e4 * U.kjmol
+
+
+def _guest(name):
+    from cmmdft_amd.system import ChkSystem
+    from cmmdft_amd.extpot import read_pars_file
+    chk_fn = os.path.join(INPUTS, '%s_AA.chk' % name)
+    par_fn = os.path.join(INPUTS, '%s_pars_vdw_AA_trappe.txt' % name)
+    chk = ChkSystem.from_file(chk_fn)
+    ff = read_pars_file(par_fn)
+    gs = np.array([ff[chk.ffatypes[t]][0] for t in chk.ffatype_ids])
+    ge = np.array([ff[chk.ffatypes[t]][1] for t in chk.ffatype_ids])
+    return chk_fn, par_fn, chk, gs, ge
+
+
+def _oracle(host, points, chk, gs, ge, degree, float32=False):
+    sig, eps = host.atom_params()
+    return O.effective_potential(points, chk.pos, chk.masses, gs, ge, sig, eps, host.pos, host.rvecs, BETA,
+                                 degree=degree, cutoff=W.CUTOFF, float32=float32)
+
+
+def _clamped(V):
+    V = np.where(np.isfinite(V), V, LIMIT)
+    return np.minimum(V, LIMIT)
+
+
+def _rel_err(V, Vr):
+    # over the points below the clamp limit: max |dV| / max |V| and the
+    # pointwise relative error where |V| > 1e-6 of the maximum
+    V, Vr = _clamped(V), _clamped(Vr)
+    assert np.array_equal(V >= LIMIT, Vr >= LIMIT)
+    m = Vr < LIMIT
+    scale = np.abs(Vr[m]).max()
+    big = m & (np.abs(Vr) > 1e-6 * scale)
+    return np.abs(V[m] - Vr[m]).max() / scale, (np.abs(V - Vr)[big] / np.abs(Vr[big])).max()
+
+
+@pytest.mark.parametrize('name,degree,shape', [('co2', 3, (16, 12, 10)), ('co2', 9, (16, 12, 10)),
+                                               ('c2h6', 5, (12, 12, 12))])
+def test_effective_potential_grid_matches_oracle(name, degree, shape):
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest(name)
+    og = O.Grid(host.rvecs, shape)
+    Vr = _oracle(host, og.points[..., :3], chk, gs, ge, degree)
+    gens = get_external_potential_dict(host.ff, par_fn, host, cutoff=W.CUTOFF)
+    V = generate_effective_potential(chk_fn, Grid(host.rvecs, shape), BETA, gens, degree=degree)
+    assert V.shape == shape
+    a, r = _rel_err(V, Vr)
+    print('effpot %s degree %d %s: max|dV|/max|V| %.2e, pointwise %.2e' % (name, degree, shape, a, r))
+    assert a <= 1e-10 and r <= 1e-10, (a, r)
+
+
+def test_effective_potential_point_list_equals_grid():
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest('c2h6')
+    shape = (10, 14, 9)
+    grid = Grid(host.rvecs, shape)
+    gens = get_external_potential_dict(host.ff, par_fn, host, cutoff=W.CUTOFF)
+    Vg = generate_effective_potential(chk_fn, grid, BETA, gens, degree=5)
+    pts = np.ascontiguousarray(grid.points[..., :3])
+    Vp = generate_effective_potential(chk_fn, pts, BETA, gens, degree=5)
+    Vl = generate_effective_potential(chk_fn, pts.reshape(-1, 3), BETA, gens, degree=5).reshape(shape)
+    a, r = _rel_err(Vp, Vg)
+    assert a <= 1e-12 and r <= 1e-12, (a, r)
+    a, r = _rel_err(Vl, Vg)
+    assert a <= 1e-12 and r <= 1e-12, (a, r)
+
+
+def test_effective_potential_vs_reference_float32():
+    # the reference holds the per-atom values in float32: agreement to its precision
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest('co2')
+    shape = (12, 12, 12)
+    og = O.Grid(host.rvecs, shape)
+    Vr = _clamped(_oracle(host, og.points[..., :3], chk, gs, ge, 5, float32=True))
+    gens = get_external_potential_dict(host.ff, par_fn, host, cutoff=W.CUTOFF)
+    V = _clamped(generate_effective_potential(chk_fn, Grid(host.rvecs, shape), BETA, gens, degree=5))
+    assert np.all(np.abs(V - Vr) <= 4e-7 * np.maximum(np.abs(Vr), 1 / BETA))
+
+
+def test_one_site_guest_equals_lj_grid():
+    from cmmdft_amd.system import ChkSystem, Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential, get_external_potential
+    from cmmdft_amd.rotations import combined_rotations
+    host = W.host_case2()
+    shape = (32, 32, 32)
+    guest = ChkSystem(np.zeros((1, 3)), np.zeros((0, 3)), ['ME'], [0], masses=[16.04 * U.amu])
+    gens = get_external_potential_dict(host.ff, {'ME': (W.GUEST_SIGMA, W.GUEST_EPSILON)}, host, cutoff=W.CUTOFF)
+    grid = Grid(host.rvecs, shape)
+    V = generate_effective_potential(guest, grid, BETA, gens, degree=9)
+    Vlj = get_external_potential(grid, host.ff, W.GUEST_SIGMA, W.GUEST_EPSILON, host, cutoff=W.CUTOFF)
+    wsum = combined_rotations(9)[1].sum()
+    a, r = _rel_err(V.ravel(), Vlj - np.log(wsum) / BETA)
+    # the pair terms are summed in another form (1/r^2 by reciprocal, not
+    # sigma/sqrt): rounding only, at the external-potential tolerance
+    assert a <= 1e-12 and r <= 1e-10, (a, r)
+
+
+def test_images_equal_minimum_image_when_cell_is_large():
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    host = W.host_case2()      # L/2 = 12.8 A > r_c: one image per atom within the cut-off
+    chk_fn, par_fn, chk, gs, ge = _guest('co2')
+    grid = Grid(host.rvecs, (20, 20, 20))
+    V0 = generate_effective_potential(chk_fn, grid, BETA, get_external_potential_dict(host.ff, par_fn, host,
+                                                                                       cutoff=W.CUTOFF), degree=5)
+    V1 = generate_effective_potential(chk_fn, grid, BETA, get_external_potential_dict(host.ff, par_fn, host,
+                                                                                       cutoff=W.CUTOFF, images=True),
+                                      degree=5)
+    a, r = _rel_err(V1, V0)
+    # image positions a + nL instead of the wrapped difference: rounding only
+    assert a <= 1e-12 and r <= 1e-10, (a, r)
+
+
+def test_argument_errors():
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest('co2')
+    gens = get_external_potential_dict(host.ff, par_fn, host, cutoff=W.CUTOFF)
+    with pytest.raises(NotImplementedError):
+        generate_effective_potential(chk_fn, np.zeros((4, 3)), BETA, gens, degree=13)
+    with pytest.raises(KeyError):
+        generate_effective_potential(chk_fn, np.zeros((4, 3)), BETA, {'C_CD': gens['C_CD']}, degree=5)
+    with pytest.raises(TypeError):
+        generate_effective_potential(chk_fn, np.zeros((4, 3)), BETA, {k: (lambda x: x) for k in gens}, degree=5)
+
+
+def test_dual_model_guest_solve_matches_oracle():
+    """WBII + MFA-WDA + the CO2 effective potential (degree 5) on the case-1
+    host: the converged density of the device static Picard solver against
+    the oracle's on the oracle's float64 effective potential."""
+    from cmmdft_amd.program import Program
+    from cmmdft_amd.system import NanoporousHost, DualModelGuest
+    from cmmdft_amd.eos import MFAEOS
+    from cmmdft_amd.solver import Picard
+    from cmmdft_amd.solver import grand_potential
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest('co2')
+    shape = (24, 24, 24)
+    sigma, eps = 3.73 * U.angstrom, 195.0 * U.boltzmann      # the spherical model's LJ parameters
+    degree = 5
+    prog = Program(write_files=False)
+    guest = DualModelGuest('CO2', 44.0 * U.amu, sigma, eps, chk_fn, par_fn)
+    mass = guest.mass
+    prog.set_system(NanoporousHost('host', host, host.ff), guest)
+    prog.set_grid(npoints=list(shape))
+    prog.init_free_energy(T)
+    prog.fener.add_hard_sphere('WBII')
+    prog.fener.add_wdav(MFAEOS(mass, sigma, eps))
+    prog.fener.add_external_potential(temperature=T, cutoff=W.CUTOFF, degree=degree)
+    # oracle side
+    og = O.Grid(host.rvecs, shape)
+    Vr = _oracle(host, og.points[..., :3], chk, gs, ge, degree).reshape(shape)
+    ext = O.ExtPot(og, Vr)
+    a, r = _rel_err(prog.fener.part_dict['ExtPot'].potential, ext.potential)
+    assert a <= 1e-10 and r <= 1e-10, (a, r)
+    R = O.bh_radius(sigma, eps, T)
+    lam = O.thermal_wavelength(mass, T)
+    parts = [O.HardSphere(og, R, 'WBII', T), O.WDAV(og, R, O.MFAEOS(mass, sigma, eps), T), ext]
+    mu = O.MBWREOS(mass, sigma, eps).calculate_mu(T, 1.0 * U.bar)
+    rho0 = O.initial_density(og, mu, BETA, lam, V=ext.potential)
+    s = O.Picard(og, parts, BETA, lam, nsteps=5000, alpha_mix=0.05, method='static', criterion='RIUE',
+                 threshold=1e-13)
+    Nref, rho_ref, conv = s.solve(mu, rho0)
+    assert conv
+    omega_ref = s.omega(rho_ref, og.fft(rho_ref))
+    # the same solver family as the oracle's (static Picard zeroes densities
+    # below 1e-10 A^-3, so its fixed point differs from Anderson's there)
+    prog.set_solver(Picard(prog, nsteps=5000, alpha_mix=0.05, method='static', criterion='RIUE', threshold=1e-13))
+    N, rho, conv = prog.solve(mu)
+    assert conv
+    rel, mism = rho_rel_err(rho, rho_ref)
+    assert rel <= 1e-8 and mism == 0, (rel, mism)
+    assert abs(N - Nref) <= 1e-8 * abs(Nref)
+    om = grand_potential(prog.fener, mu, rho)
+    assert abs(om - omega_ref) <= 1e-8 * abs(omega_ref)
+
+
+# --- interpolated site potentials ---------------------------------------------
+
+def test_lj_derivatives_grid_matches_oracle():
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_derivatives
+    host = W.host_case1()
+    shape = (20, 18, 16)
+    grid = Grid(host.rvecs, shape)
+    og = O.Grid(host.rvecs, shape)
+    sig, eps = host.atom_params()
+    sf, ef = 3.05 * U.angstrom, 79.0 * U.boltzmann
+    ref = O.lj_derivatives(og.points[..., :3], sig, eps, host.pos, host.rvecs, sf, ef, og.spacings, cutoff=W.CUTOFF)
+    got = get_external_potential_derivatives(grid, host.ff, sf, ef, host, grid.spacings, cutoff=W.CUTOFF)
+    assert got.shape == (8,) + shape
+    got = got.reshape(8, -1)
+    pts = og.points[..., :3].reshape(-1, 3)
+    got_p = get_external_potential_derivatives(pts[::7], host.ff, sf, ef, host, grid.spacings, cutoff=W.CUTOFF)
+    for q in range(8):
+        scale = np.abs(ref[q]).max()
+        assert np.abs(got[q] - ref[q]).max() <= 1e-10 * scale, q
+        assert np.abs(got_p[q] - ref[q][::7]).max() <= 1e-10 * scale, q
+
+
+@pytest.mark.parametrize('method', ['tricubic', 'trilinear', 'tricubic_estimated', 'values3d'])
+def test_interpolator_matches_oracle(method):
+    from cmmdft_amd.extpot import Interpolator
+    rng = np.random.default_rng(12)
+    vals = rng.normal(size=(8, 9, 10, 11))
+    if method == 'values3d':
+        vals = vals[0]
+    h = np.array([0.3, 0.25, 0.2])
+    origin = np.array([-1.2, 0.4, 0.7])
+    pos = rng.uniform(-6, 6, (5000, 3))
+    oi = O.Interpolator(vals, origin, h)
+    ref = {'tricubic': lambda: oi.tricubic(pos), 'trilinear': lambda: oi.trilinear(pos),
+           'tricubic_estimated': lambda: oi.tricubic(pos, estimate_derivatives=True),
+           'values3d': lambda: oi.tricubic(pos)}[method]()
+    it = Interpolator(vals, origin, h)
+    got = getattr(it, 'tricubic' if method == 'values3d' else method)(pos)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(vals).max() * 8
+
+
+def test_interpolated_effective_potential_matches_oracle():
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import (get_external_potential_derivatives, Interpolator, generate_effective_potential)
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest('co2')
+    from cmmdft_amd.extpot import read_pars_file
+    gff = read_pars_file(par_fn)
+    # the site tables on a coarse 0.5 A grid of the cell (the oracle builds the same tables)
+    tshape = (48, 48, 48)
+    tgrid = Grid(host.rvecs, tshape)
+    og_t = O.Grid(host.rvecs, tshape)
+    sig, eps = host.atom_params()
+    shape = (12, 10, 8)
+    og = O.Grid(host.rvecs, shape)
+    origin = og.points[0, 0, 0, :3]
+    h = np.array(tgrid.spacings)
+    gens, ogens = {}, {}
+    for atom, (sf, ef) in gff.items():
+        t = get_external_potential_derivatives(tgrid, host.ff, sf, ef, host, tgrid.spacings, cutoff=W.CUTOFF)
+        gens[atom] = Interpolator(t, origin, h).tricubic
+        ogens[atom] = O.Interpolator(t, origin, h).tricubic
+    V = generate_effective_potential(chk_fn, Grid(host.rvecs, shape), BETA, gens, degree=5)
+    site = [ogens[chk.ffatypes[t]] for t in chk.ffatype_ids]
+    Vr = O.effective_potential_gen(og.points[..., :3], chk.pos, chk.masses, site, BETA, degree=5, float32=False)
+    a, r = _rel_err(V, Vr)
+    assert a <= 1e-10 and r <= 1e-10, (a, r)
+    del og_t
+
+
+def test_external_potential_interpolate_path():
+    """ExternalPotential(interpolate=True) follows the reference's recipe
+    (0.15 A derivative grids of the host cell, origin the first grid point,
+    spacing 0.15 A), and on the 24 A cell -- where ceil(L / 0.15 A) gives
+    exactly 0.15 A -- it approximates the direct average."""
+    from cmmdft_amd.program import Program
+    from cmmdft_amd.system import NanoporousHost, DualModelGuest, Grid
+    from cmmdft_amd.extpot import (get_external_potential_derivatives, Interpolator, generate_effective_potential,
+                                   read_pars_file)
+    host = W.host_case1()
+    chk_fn, par_fn, chk, gs, ge = _guest('co2')
+    shape = (24, 24, 24)
+    prog = Program(write_files=False)
+    prog.set_system(NanoporousHost('host', host, host.ff),
+                    DualModelGuest('CO2', 44.0 * U.amu, 3.73 * U.angstrom, 195.0 * U.boltzmann, chk_fn, par_fn))
+    prog.set_grid(npoints=list(shape))
+    prog.init_free_energy(T)
+    ep = prog.fener.add_external_potential(temperature=T, cutoff=W.CUTOFF, degree=5, interpolate=True)
+    Vi = ep.potential
+    hmol = prog.system.host.mol
+    egrid = Grid(hmol.cell, spacing=0.15 * U.angstrom)
+    assert egrid.npoints == [160, 160, 160]
+    gens = {}
+    for atom, (sf, ef) in read_pars_file(par_fn).items():
+        t = get_external_potential_derivatives(egrid, host.ff, sf, ef, hmol, egrid.spacings, cutoff=W.CUTOFF)
+        gens[atom] = Interpolator(t, prog.grid.points[0, 0, 0, :3], [0.15 * U.angstrom] * 3).tricubic
+    Vm = _clamped(generate_effective_potential(chk_fn, prog.grid, BETA, gens, degree=5))
+    assert np.array_equal(Vi, Vm)
+    # against the direct average (interpolation error only)
+    from cmmdft_amd.extpot import get_external_potential_dict
+    Vd = _clamped(generate_effective_potential(chk_fn, prog.grid, BETA,
+                                               get_external_potential_dict(host.ff, par_fn, host, cutoff=W.CUTOFF),
+                                               degree=5))
+    m = Vd < 0
+    assert np.abs(Vi[m] - Vd[m]).max() <= 1e-3 * np.abs(Vd[m]).max()
