@@ -111,6 +111,8 @@ EXPORTS = {
     'cmm_effpot_interp': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_int, C.c_void_p,
                            C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                            C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_min_distance': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
+                         C.c_int),
     'cmm_lj_points': ([C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
                        C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
 }
@@ -474,6 +476,15 @@ def effpot_interp(npoints, points_dev, M, rvecs, gtype, tables, tn, torigin, tsp
                                 to.ctypes.data_as(C.c_void_p), th.ctypes.data_as(C.c_void_p),
                                 tm.ctypes.data_as(C.c_void_p), _ptr(offsets_dev), _ptr(weights_dev), int(nrot),
                                 float(beta), float(nanval), _ptr(out), C.c_void_p(s.cuda_stream)))
+
+
+def min_distance(npoints, rvecs, pos_dev, natom, out, stream=None):
+    torch = _torch()
+    lib = load()
+    rv = np.ascontiguousarray(np.asarray(rvecs, dtype=np.float64).reshape(9))
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_min_distance(int(npoints[0]), int(npoints[1]), int(npoints[2]), rv.ctypes.data_as(C.c_void_p),
+                               _ptr(pos_dev), int(natom), _ptr(out), C.c_void_p(s.cuda_stream)))
 
 
 def lj_grid_grad(npoints, rvecs, atoms_dev, types_dev, natom, ntypes, sigma_f, epsilon_f, cutoff, images, limit, w,

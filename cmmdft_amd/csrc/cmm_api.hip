@@ -1173,6 +1173,43 @@ int cmm_effpot_interp(int nx, int ny, int nz, const double* points, long long M,
   return CMM_OK;
 }
 
+int cmm_min_distance(int nx, int ny, int nz, const double* rvecs, const double* pos, int natom, double* out,
+                     void* stream) {
+  if (!rvecs || !out || (natom > 0 && !pos) || natom < 1) return fail(CMM_E_ARG, "null argument or no atoms");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(CMM_E_ARG, "bad sizes");
+  MinDistDev d;
+  d.nx = nx;
+  d.ny = ny;
+  d.nz = nz;
+  std::memcpy(d.rv, rvecs, 9 * sizeof(double));
+  // G = R^-T: rows of G are the reciprocal vectors (G R^T = I)
+  const double* r = rvecs;
+  const double det = r[0] * (r[4] * r[8] - r[5] * r[7]) - r[1] * (r[3] * r[8] - r[5] * r[6]) +
+                     r[2] * (r[3] * r[7] - r[4] * r[6]);
+  if (!(std::fabs(det) > 0.0)) return fail(CMM_E_ARG, "singular cell");
+  // inverse of R (row-major), then transpose
+  double inv[9];
+  inv[0] = (r[4] * r[8] - r[5] * r[7]) / det;
+  inv[1] = (r[2] * r[7] - r[1] * r[8]) / det;
+  inv[2] = (r[1] * r[5] - r[2] * r[4]) / det;
+  inv[3] = (r[5] * r[6] - r[3] * r[8]) / det;
+  inv[4] = (r[0] * r[8] - r[2] * r[6]) / det;
+  inv[5] = (r[2] * r[3] - r[0] * r[5]) / det;
+  inv[6] = (r[3] * r[7] - r[4] * r[6]) / det;
+  inv[7] = (r[1] * r[6] - r[0] * r[7]) / det;
+  inv[8] = (r[0] * r[4] - r[1] * r[3]) / det;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) d.gv[3 * a + b] = inv[3 * b + a];
+  d.inv_n[0] = 1.0 / nx;
+  d.inv_n[1] = 1.0 / ny;
+  d.inv_n[2] = 1.0 / nz;
+  const long long n = (long long)nx * ny * nz;
+  hipLaunchKernelGGL(k_min_dist, dim3((unsigned)((n + MD_NT - 1) / MD_NT)), dim3(MD_NT), 0, (hipStream_t)stream, pos,
+                     natom, d, out);
+  LAUNCHCHK("k_min_dist");
+  return CMM_OK;
+}
+
 int cmm_lj_grad_workspace(int nx, int ny, int nz, int ntypes, long long* ndoubles) {
   if (!ndoubles || nx < 1 || ny < 1 || nz < 1 || ntypes < 1) return fail(CMM_E_ARG, "bad arguments");
   const dim3 g = lj_grid_dims(nx, ny, nz);

@@ -65,3 +65,66 @@ __global__ void __launch_bounds__(RED_NT) k_field_reduce_final(const double* __r
 }
 
 }  // namespace cmm
+
+namespace cmm {
+
+// Distance from every grid point to the nearest host atom
+// (Program.calc_distance, cmmdft/program.py:145-177): the minimum image of
+// the difference vector taken in fractional coordinates, f = G d,
+// f -= rint(f), d = f R (yaff's Cell.mic with G the reciprocal vectors as
+// rows), then the minimum over the atoms.  Atoms stream through LDS in
+// chunks; one thread per point.
+constexpr int MD_NT = 256;
+
+struct MinDistDev {
+  int nx, ny, nz;
+  double rv[9];   // cell vectors as rows
+  double gv[9];   // reciprocal vectors as rows (G R^T = I)
+  double inv_n[3];
+};
+
+__global__ void __launch_bounds__(MD_NT) k_min_dist(const double* __restrict__ pos, int natom, MinDistDev d,
+                                                   double* __restrict__ out) {
+  __shared__ double sx[MD_NT], sy[MD_NT], sz[MD_NT];
+  const long long n = (long long)d.nx * d.ny * d.nz;
+  const long long idx = (long long)blockIdx.x * MD_NT + threadIdx.x;
+  const long long ii = idx < n ? idx : n - 1;
+  const int iz = (int)(ii % d.nz);
+  const int iy = (int)((ii / d.nz) % d.ny);
+  const int ix = (int)(ii / ((long long)d.ny * d.nz));
+  const double f0 = ix * d.inv_n[0] + (-0.5), f1 = iy * d.inv_n[1] + (-0.5), f2 = iz * d.inv_n[2] + (-0.5);
+  double p[3];
+#pragma unroll
+  for (int b = 0; b < 3; ++b) p[b] = d.rv[b] * f0 + d.rv[3 + b] * f1 + d.rv[6 + b] * f2;
+  double best = INFINITY;
+  for (int c0 = 0; c0 < natom; c0 += MD_NT) {
+    const int j = c0 + threadIdx.x;
+    if (j < natom) {
+      sx[threadIdx.x] = pos[3 * j];
+      sy[threadIdx.x] = pos[3 * j + 1];
+      sz[threadIdx.x] = pos[3 * j + 2];
+    }
+    __syncthreads();
+    const int cnt = min(MD_NT, natom - c0);
+    for (int e = 0; e < cnt; ++e) {
+      const double v[3] = {p[0] - sx[e], p[1] - sy[e], p[2] - sz[e]};
+      double f[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        f[a] = d.gv[3 * a] * v[0] + d.gv[3 * a + 1] * v[1] + d.gv[3 * a + 2] * v[2];
+        f[a] -= rint(f[a]);
+      }
+      double r2 = 0.0;
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const double w = f[0] * d.rv[b] + f[1] * d.rv[3 + b] + f[2] * d.rv[6 + b];
+        r2 += w * w;
+      }
+      best = fmin(best, r2);
+    }
+    __syncthreads();
+  }
+  if (idx < n) out[idx] = sqrt(best);
+}
+
+}  // namespace cmm

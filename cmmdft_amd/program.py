@@ -45,8 +45,70 @@ class Program:
         self.grid = None
         self.fener = None
 
+    def copy(self):
+        """A Program sharing this one's system, grid, free energy and solver
+        (program.py:72-96)."""
+        new = Program(prefix=self.name_dict['prefix'], hostname=self.name_dict['hostname'],
+                      guestname=self.name_dict['guestname'], ff_suffix=self.name_dict['ff_suffix'],
+                      funct_suffix=self.name_dict['funct_suffix'], grid_suffix=self.name_dict['grid_suffix'],
+                      suffix=self.name_dict['suffix'], overwrite=self.overwrite, write_files=self.write_files)
+        new.workdir = self.workdir
+        new.rho_fn = self.rho_fn
+        new.pars_fn = getattr(self, 'pars_fn', None)
+        new.system = self.system
+        new.grid = self.grid
+        new.fener = self.fener
+        if hasattr(self, 'solver'):
+            new.solver = self.solver
+        return new
+
     def set_system(self, host, guest):
         self.system = System(host, guest)
+
+    def calc_distance(self, rewrite=False):
+        """Distance from every grid point to the nearest host atom
+        (program.py:145-177; minimum image in fractional coordinates, as
+        yaff's Cell.mic), on the device.  Kept as ``self.dis``; with file
+        output on it is cached as <prefix>/<hostname>/<grid_suffix>/distances.npy."""
+        dist_file = Path(self.name_dict['prefix']) / self.name_dict['hostname'] / self.name_dict['grid_suffix'] / \
+            'distances.npy'
+        if self.write_files:
+            dist_file.parent.mkdir(parents=True, exist_ok=True)
+            if rewrite and dist_file.is_file():
+                dist_file.unlink()
+            if dist_file.is_file():
+                self.dis = np.load(dist_file)
+                return
+        torch = torch_mod()
+        from . import _lib
+        pos = as_device(np.asarray(self.system.host.mol.pos, dtype=np.float64).reshape(-1, 3))
+        out = torch.empty(tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
+        _lib.min_distance(self.grid.npoints, np.asarray(self.grid.cell.rvecs, dtype=np.float64), pos, pos.shape[0],
+                          out)
+        self.dis = out.cpu().numpy()
+        if self.write_files:
+            np.save(dist_file, self.dis)
+
+    def calc_regions(self, energy_cutoff=0.55, range_cutoff=3.4 * U.angstrom, mof_cutoff=5):
+        """Three region masks of the framework (program.py:179-217): near the
+        host and/or energetically favoured sites, the framework itself
+        (V > mof_cutoff kT), and the empty pore space.  Returns
+        (mask_site, mask_mof, mask_empty)."""
+        self.calc_distance()
+        range_mask = self.dis < range_cutoff
+        ep = self._extpot()
+        if ep is None:
+            raise ValueError('The regions of a nanoporous material can only be calculated if an external potential '
+                             'is defined')
+        epot_data = np.asarray(ep.potential)
+        crit = np.amin(epot_data) - energy_cutoff * np.amin(epot_data)
+        energy_mask = epot_data < crit
+        self.r_mask = range_mask
+        self.e_mask = energy_mask
+        self.mask_mof = epot_data > mof_cutoff * U.boltzmann * self.fener.temperature
+        self.mask_site = (energy_mask + range_mask) * (~self.mask_mof)
+        self.mask_empty = (~energy_mask) * (~range_mask) * (~self.mask_mof)
+        return self.mask_site, self.mask_mof, self.mask_empty
 
     def set_grid(self, npoints=None, spacing=0.5 * U.angstrom):
         assert self.system is not None, "Host and guest must first be set using 'set_system'"

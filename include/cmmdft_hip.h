@@ -57,6 +57,7 @@
  *  cmm_mfa_table         MFAFunctional kpotential (fft x Lanczos, or fft of a
  *                        loaded potential)             cmmdft/functionals.py:809-812, 910
  *  cmm_plan_set_mfa      MFAFunctional.derive / value  cmmdft/functionals.py:912-939
+ *  cmm_min_distance      Program.calc_distance          cmmdft/program.py:145-177
  *  cmm_field_reduce      Calculator integrals: loading, Henry coefficient,
  *                        helium void fraction, ideal-gas free energy
  *                                                      cmmdft/calculator.py:60-88, 152-184, 387-397
@@ -322,6 +323,13 @@ int cmm_effpot_interp(int nx, int ny, int nz, const double* points, long long M,
                       const int* gtype, int nt, const double* const* tables, const int* tn, const double* torigin,
                       const double* tspacing, const int* tmode, const double* offsets, const double* weights,
                       int nrot, double beta, double nanval, double* V, void* stream);
+
+/* Distance from each point of the nx x ny x nz grid of the cell to the
+ * nearest of the natom host atoms (device pos[natom][3]), minimum image in
+ * fractional coordinates.  Replaces Program.calc_distance
+ *                                 cmmdft/program.py:145-177 */
+int cmm_min_distance(int nx, int ny, int nz, const double* rvecs, const double* pos, int natom, double* out,
+                     void* stream);
 
 /* Backward of cmm_lj_grid: grad[2t] = sum_i w_i dV_i/dsigma_t,
  * grad[2t+1] = sum_i w_i dV_i/depsilon_t for host atom types t < ntypes

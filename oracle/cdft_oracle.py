@@ -935,6 +935,34 @@ class Interpolator:                                 # extpot_calculator.py:391-5
         return result
 
 
+def min_distance(points, pos, rvecs):              # program.py:145-177
+    """Distance of each point to the nearest atom, the difference vector
+    wrapped by yaff's Cell.mic (fractional coordinates rounded to the
+    nearest integer)."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    R = np.asarray(rvecs, dtype=np.float64)
+    G = np.linalg.inv(R).T                     # reciprocal vectors as rows
+    best = np.full(len(pts), np.inf)
+    for a in np.asarray(pos, dtype=np.float64).reshape(-1, 3):
+        d = pts - a
+        f = d @ G.T
+        f -= np.round(f)
+        d = f @ R
+        best = np.minimum(best, np.sqrt(np.sum(d * d, axis=1)))
+    return best
+
+
+def regions(dis, epot, temperature, energy_cutoff=0.55, range_cutoff=3.4 * U.angstrom, mof_cutoff=5):
+    """Program.calc_regions (program.py:179-217)."""
+    range_mask = dis < range_cutoff
+    crit = np.amin(epot) - energy_cutoff * np.amin(epot)
+    energy_mask = epot < crit
+    mask_mof = epot > mof_cutoff * U.boltzmann * temperature
+    mask_site = (energy_mask + range_mask) * (~mask_mof)
+    mask_empty = (~energy_mask) * (~range_mask) * (~mask_mof)
+    return mask_site, mask_mof, mask_empty
+
+
 # ---------------------------------------------------------------------------
 # Euler-Lagrange solvers (cmmdft/solver.py:18-604)
 # ---------------------------------------------------------------------------

@@ -82,3 +82,27 @@ def test_real_framework_solution(name, shape, version, p_bar):
     assert abs(N - Nref) <= 1e-8 * Nref
     om = grand_potential(prog.fener, mu, rho)
     assert abs(om - omega_ref) <= 1e-8 * abs(omega_ref), (om, omega_ref)
+
+
+@pytest.mark.parametrize('name,shape', [('ZIF8', (30, 32, 28)), ('CHA', (28, 28, 30))])
+def test_distance_and_regions(name, shape):
+    """Program.calc_distance (device nearest-atom distance, fractional minimum
+    image) and calc_regions against the oracle (program.py:145-217)."""
+    from cmmdft_amd.program import Program
+    from cmmdft_amd.system import SphericalLJGuest
+    h = _host(name)
+    prog = Program(write_files=False)
+    prog.set_system(h, SphericalLJGuest('CH4', W.GUEST_MASS, W.GUEST_SIGMA, W.GUEST_EPSILON))
+    prog.set_grid(npoints=list(shape))
+    prog.init_free_energy(T)
+    prog.fener.add_external_potential(cutoff=W.CUTOFF)
+    site, mof, empty = prog.calc_regions()
+    og = O.Grid(np.asarray(h.cell.rvecs), shape)
+    dref = O.min_distance(og.points[..., :3], h.mol.pos, np.asarray(h.cell.rvecs)).reshape(shape)
+    assert np.abs(prog.dis - dref).max() <= 1e-12 * dref.max()
+    ep = prog.fener.part_dict['ExtPot'].potential
+    rs, rm, re = O.regions(dref, ep, T)
+    assert np.array_equal(site, rs) and np.array_equal(mof, rm) and np.array_equal(empty, re)
+    assert site.any() and mof.any() and empty.any()
+    c = prog.copy()
+    assert c.grid is prog.grid and c.fener is prog.fener and c.system is prog.system

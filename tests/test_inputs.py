@@ -94,3 +94,29 @@ def test_nanoporous_host_from_files():
     assert np.array_equal(h2.mol.pos, h.mol.pos) and h2.ff == h.ff
     with pytest.raises(ValueError):
         NanoporousHost('ZIF8', _p('zif8.chk'), _p('cha_pars.txt'))     # no parameters for the ZIF-8 types
+
+
+def test_min_distance_oracle_is_nearest_image():
+    # Program.calc_distance (program.py:145-177): for an orthorhombic cell the
+    # fractional minimum image is the nearest periodic image
+    from oracle import cdft_oracle as O
+    rng = np.random.default_rng(3)
+    R = np.diag([9.0, 11.0, 13.0])
+    pos = (rng.random((7, 3)) - 0.5) @ R
+    pts = (rng.random((200, 3)) - 0.5) @ R
+    got = O.min_distance(pts, pos, R)
+    shifts = np.array([[i, j, k] for i in (-1, 0, 1) for j in (-1, 0, 1) for k in (-1, 0, 1)]) @ R
+    brute = np.min(np.linalg.norm(pts[:, None, None, :] - pos[None, :, None, :] - shifts[None, None, :, :],
+                                  axis=-1), axis=(1, 2))
+    assert np.abs(got - brute).max() <= 1e-12
+    # skewed cell: never below the nearest image, equal to it close to an atom
+    Rs = np.array([[10.0, 0.0, 0.0], [3.0, 9.0, 0.0], [1.0, 2.0, 11.0]])
+    pos = (rng.random((5, 3)) - 0.5) @ Rs
+    pts = np.concatenate([(rng.random((100, 3)) - 0.5) @ Rs, pos + 0.3 * rng.standard_normal((5, 3))])
+    got = O.min_distance(pts, pos, Rs)
+    shifts = np.array([[i, j, k] for i in range(-2, 3) for j in range(-2, 3) for k in range(-2, 3)]) @ Rs
+    brute = np.min(np.linalg.norm(pts[:, None, None, :] - pos[None, :, None, :] - shifts[None, None, :, :],
+                                  axis=-1), axis=(1, 2))
+    assert np.all(got >= brute - 1e-12)
+    near = brute < 1.5
+    assert np.abs(got[near] - brute[near]).max() <= 1e-12
