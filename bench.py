@@ -65,17 +65,20 @@ def pass_bytes(nx, ny, nz):
     return {0: 12 * nh, 1: 7 * nh, 2: 7 * nh, 3: 12 * nh}
 
 
-def measured_traffic(kernel, n):
-    """HBM bytes per launch of ``kernel`` at n^3 from the committed
-    FETCH_SIZE/WRITE_SIZE summary (profiles/functional_passes_128cube.md),
-    or None when no measurement for that grid is recorded."""
-    fn = os.path.join(ROOT, 'profiles', 'functional_passes_128cube.md')
-    if not os.path.exists(fn):
-        return None
-    for line in open(fn):
-        w = line.split()
-        if len(w) == 4 and w[0] == 'traffic_bytes_per_launch' and w[1] == kernel.replace(' ', '_') and int(w[2]) == n:
-            return int(w[3])
+def measured_traffic(kernel, n, nbatch=1):
+    """HBM bytes per launch of ``kernel`` at n^3 (a launch covering ``nbatch``
+    instances) from the committed FETCH_SIZE/WRITE_SIZE summaries
+    (profiles/functional_passes_128cube*.md), or None when no measurement
+    for that grid and batch is recorded."""
+    for name in ('functional_passes_128cube.md', 'functional_passes_128cube_batch32.md'):
+        fn = os.path.join(ROOT, 'profiles', name)
+        if not os.path.exists(fn):
+            continue
+        for line in open(fn):
+            w = line.split()
+            if len(w) in (4, 5) and w[0] == 'traffic_bytes_per_launch' and w[1] == kernel.replace(' ', '_') \
+                    and int(w[2]) == n and (int(w[3][1:]) if len(w) == 5 else 1) == nbatch:
+                return int(w[-1])
     return None
 
 
@@ -97,17 +100,19 @@ def time_passes(plan, reps=20):
     return out
 
 
-def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample):
-    """The oracle (NumPy/SciPy restatement of the reference path, scipy.fft
-    with one worker) timed for ``nsample`` Anderson iterations at the same
-    grid and potential; extrapolated to points/s with the GPU's mean
-    iteration count per point."""
+def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample, cores=1):
+    """The oracle (NumPy/SciPy restatement of the reference path; scipy.fft
+    with ``cores`` workers, BLAS/OpenMP pools limited to as many) timed for
+    ``nsample`` Anderson iterations at the same grid and potential;
+    extrapolated to points/s with the GPU's mean iteration count per
+    point."""
+    import scipy.fft as sfft
     from threadpoolctl import threadpool_limits
     from cmmdft_amd import units as U, workloads as W
     from oracle import cdft_oracle as O
     host = W.host_case2()
     T = W.TEMPERATURE
-    with threadpool_limits(limits=1):
+    with threadpool_limits(limits=cores), sfft.set_workers(cores):
         og = O.Grid(host.rvecs, shape)
         R = O.bh_radius(W.GUEST_SIGMA, W.GUEST_EPSILON, T)
         lam = O.thermal_wavelength(W.GUEST_MASS, T)
@@ -127,10 +132,10 @@ def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample):
         s.solve(mu, rho0)
     per_iter = float(np.median(np.diff(stamps))) if len(stamps) > 1 else float('nan')
     pts = 1.0 / (per_iter * gpu_iters_per_point)
-    return {'value': pts, 'unit': 'points/s', 'cores': 1, 'kind': 'port',
-            'sample': ('oracle Anderson (SLSQP mixing, scipy.fft 1 worker) at %dx%dx%d, %d iterations timed, median '
-                       '%.3f s/iteration, extrapolated with the GPU mean of %.1f iterations/point'
-                       % (shape + (nsample, per_iter, gpu_iters_per_point))),
+    return {'value': pts, 'unit': 'points/s', 'cores': cores, 'kind': 'port',
+            'sample': ('oracle Anderson (SLSQP mixing, scipy.fft %d worker%s) at %dx%dx%d, %d iterations timed, '
+                       'median %.3f s/iteration, extrapolated with the GPU mean of %.1f iterations/point'
+                       % ((cores, '' if cores == 1 else 's') + shape + (nsample, per_iter, gpu_iters_per_point))),
             'seconds_per_iteration': per_iter}
 
 
@@ -297,10 +302,12 @@ def uneven_leg(nframe=16, npress=8):
                                        for g in grads][:4]}
 
 
-def passes_only(shape, reps=5):
+def passes_only(shape, reps=5, nbatch=1):
     """Functional chain of one realistic density (ideal-gas start at 10 bar),
-    each pass launched ``reps`` times, then a plain 256 MB device-to-device
-    copy (the reference for scaling FETCH_SIZE / WRITE_SIZE counters)."""
+    on ``nbatch`` instances of it (one plan, every kernel launch covering
+    all of them, as in the batched isotherm), each pass launched ``reps``
+    times, then a plain 256 MB device-to-device copy (the reference for
+    scaling FETCH_SIZE / WRITE_SIZE counters)."""
     import torch
     from cmmdft_amd import workloads as W
     from cmmdft_amd.eos import ModifiedBenedictWebbRubinEOS
@@ -312,21 +319,29 @@ def passes_only(shape, reps=5):
     from cmmdft_amd.solver import _Engine
     eng = _Engine(prog.grid, prog.fener)
     rho = prog._set_initial_density(chempot=mu, temperature=W.TEMPERATURE).reshape(-1).contiguous()
-    eng.plan.seed(rho, eng.V, False, eng.lam3)
-    eng.plan.derive()
+    plan = eng.plan
+    if nbatch > 1:
+        from cmmdft_amd._lib import Plan
+        from cmmdft_amd.functionals import configure_plan
+        plan = Plan(prog.grid.npoints, prog.grid.cell.rvecs, nbatch=nbatch)
+        configure_plan(plan, eng.hs, eng.wda, eng.mfa, eng.kT)
+        rho = rho.repeat(nbatch).contiguous()
+    plan.seed(rho, eng.V, False, eng.lam3)
+    plan.derive()
     torch.cuda.synchronize()
     for which in PASS_NAMES:
         for _ in range(reps):
-            eng.plan.launch_pass(which)
+            plan.launch_pass(which)
     torch.cuda.synchronize()
     a = torch.empty(256 * 2 ** 20 // 8, dtype=torch.float64, device='cuda').fill_(1.0)
     b = torch.empty_like(a)
     for _ in range(reps):
         b.copy_(a)
     torch.cuda.synchronize()
-    t = time_passes(eng.plan)
-    print(json.dumps({'passes_only': True, 'grid': list(shape), 'pass_ms': {PASS_NAMES[k]: v for k, v in t.items()},
-                      'pass_bytes': {PASS_NAMES[k]: v for k, v in pass_bytes(*shape).items()},
+    t = time_passes(plan)
+    print(json.dumps({'passes_only': True, 'grid': list(shape), 'nbatch': nbatch,
+                      'pass_ms': {PASS_NAMES[k]: v for k, v in t.items()},
+                      'pass_bytes': {PASS_NAMES[k]: nbatch * v for k, v in pass_bytes(*shape).items()},
                       'copy_bytes': a.numel() * 8}))
 
 
@@ -343,6 +358,7 @@ def main():
     ap.add_argument('--no-uneven', action='store_true', help='skip the uneven-batch (16 frameworks) leg')
     ap.add_argument('--no-effpot', action='store_true', help='skip the non-spherical (CO2) effective-potential leg')
     ap.add_argument('--effpot-only', action='store_true', help='only the CO2 effective-potential leg (profiling)')
+    ap.add_argument('--batch', type=int, default=1, help='instances per launch in --passes-only')
     ap.add_argument('--passes-only', action='store_true',
                     help='short run for hardware counters: the functional passes at the bench grid, then a 256 MB '
                          'device copy as the byte-count reference')
@@ -356,7 +372,7 @@ def main():
     from cmmdft_amd.solver import Anderson
     shape = (args.grid,) * 3
     if args.passes_only:
-        return passes_only(shape)
+        return passes_only(shape, nbatch=args.batch)
     if args.effpot_only:
         print(json.dumps({'effective_potential': effpot_leg(shape)}))
         return
@@ -430,9 +446,18 @@ def main():
     plan = prog.grid.plan()
     tp = time_passes(plan)
     per_iter_weight = {0: 1, 1: 1, 2: 1, 3: 2}
-    dom = max(tp, key=lambda w: tp[w] * per_iter_weight[w])
     pb = pass_bytes(*shape)
-    achieved = pb[dom] / (tp[dom] * 1e-3) / 1e9
+    # the same passes as the batched isotherm launches them: one launch per
+    # pass covering all nb instances of the batch plan (every instance
+    # active); the roofline is taken from these when the batch ran
+    nbr, tpr = 1, tp
+    if batched_pps is not None:
+        bplan = batch._groups[0]['plan']
+        bplan.set_active(None)
+        nbr = bplan.nbatch
+        tpr = time_passes(bplan)
+    dom = max(tpr, key=lambda w: tpr[w] * per_iter_weight[w])
+    achieved = nbr * pb[dom] / (tpr[dom] * 1e-3) / 1e9
     npts = int(np.prod(shape))
     b_iter = npts * (16 * 14 + 24) + npts * 8 * (3 * 5 + 2)
     iter_gbps = b_iter * int(sum(int(v) for v in (biters if biters else iters))) / dt / 1e9
@@ -445,6 +470,11 @@ def main():
     if args.cpu_sample > 0:
         V = prog.fener.part_dict['ExtPot'].potential
         cpu = cpu_baseline(V, shape, mus[len(mus) // 2], iters_per_point, args.cpu_sample)
+        # the same on the host's cores (at most 16: one GPU's share of the box)
+        ncores = min(16, len(os.sched_getaffinity(0)))
+        if ncores > 1:
+            cpu['multi_core'] = cpu_baseline(V, shape, mus[len(mus) // 2], iters_per_point, args.cpu_sample,
+                                             cores=ncores)
 
     line = {
         'metric': METRIC, 'value': points_per_s, 'unit': 'points/s', 'n_gpus': 1, 'steps': args.steps,
@@ -457,10 +487,10 @@ def main():
                    'mode': 'batched' if batched_pps is not None else 'plain'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': PEAK_HBM_GBPS, 'unit': 'GB/s',
                      'frac': achieved / PEAK_HBM_GBPS,
-                     'traffic': (measured_traffic(PASS_NAMES[dom], shape[0]) if shape[0] == shape[1] == shape[2]
+                     'traffic': (measured_traffic(PASS_NAMES[dom], shape[0], nbr) if shape[0] == shape[1] == shape[2]
                                  else None),
-                     'kernel': PASS_NAMES[dom],
-                     'avg_launch_ms': tp[dom], 'algorithmic_bytes_per_launch': pb[dom]},
+                     'kernel': PASS_NAMES[dom], 'instances_per_launch': nbr,
+                     'avg_launch_ms': tpr[dom], 'algorithmic_bytes_per_launch': nbr * pb[dom]},
         'cpu_baseline': cpu,
         'plain_points_per_s': plain_pps,
         'batched_points_per_s': batched_pps,
@@ -473,6 +503,9 @@ def main():
         'all_converged': bool(all(convs) and all(bconvs)),
         'vext_ms': vext_ms,
         'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
+        'pass_ms_batched': ({PASS_NAMES[k]: v for k, v in tpr.items()} if nbr > 1 else None),
+        'pass_GBps_batched': ({PASS_NAMES[k]: nbr * pb[k] / (v * 1e-3) / 1e9 for k, v in tpr.items()}
+                              if nbr > 1 else None),
         'adjoint': adjoint,
         'uneven_batch': uneven,
         'effective_potential': effpot,
