@@ -238,15 +238,21 @@ def effpot_leg(shape, degree=9):
             tt.append(e0.elapsed_time(e1))
             ti.append(e1.elapsed_time(e2))
         del igens, t
-    nrot = len(combined_rotations(degree)[1])
+    from cmmdft_amd.extpot import _effpot_setup, _merge_orientations
+    rot, wts = combined_rotations(degree)
+    nrot = len(wts)
+    # orientations the kernel evaluates: those with distinct site offsets
+    # (a linear guest's rotations about its axis coincide)
+    ndist = len(_merge_orientations(_effpot_setup(mol, gens, rot)[4], wts)[1])
     npts = int(np.prod(shape))
-    sites = npts * nrot * mol.natom
+    sites = npts * ndist * mol.natom
     # in-cut-off pairs per site: host atoms in the cut-off sphere (uniform estimate)
     vol = abs(np.linalg.det(host.rvecs))
     pairs = sites * host.natom * (4 / 3 * np.pi * W.CUTOFF ** 3) / vol
     return {'workload': 'rigid CO2 (3 sites) in the case-2 host at %dx%dx%d, 300 K, %d orientations '
                         '(Lebedev degree %d x %d about the axis), r_c 12 A' % (shape + (nrot, degree, degree)),
-            'ms': ms, 'orientations': nrot, 'site_evaluations_per_s': sites / (ms * 1e-3),
+            'ms': ms, 'orientations': nrot, 'distinct_orientations': ndist,
+            'site_evaluations_per_s': sites / (ms * 1e-3),
             'in_cutoff_pairs_per_s_est': pairs / (ms * 1e-3),
             'fp64_gflops_est': pairs * 30 / (ms * 1e-3) / 1e9,
             'min_kjmol': float(V.min()) / U.kjmol,

@@ -155,6 +155,26 @@ def _guest_system(guest):
     return getattr(guest, 'mol', guest)
 
 
+def _merge_orientations(off, weights, tol=1e-12):
+    """Orientations whose site offsets coincide (to ``tol`` times the largest
+    offset, in bohr) give the same U_r: keep one of each with the summed
+    weight, in first-occurrence order.  For a linear guest along its z axis
+    every rotation about z is such a copy, so the 'degree' angles collapse
+    to one; sum_r w_r exp(-beta U_r) is unchanged up to rounding."""
+    off = np.asarray(off, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64).reshape(-1)
+    scale = tol * max(1.0, float(np.abs(off).max()))
+    key = np.round(off.reshape(len(off), -1) / scale).astype(np.int64)
+    _, first, inv = np.unique(key, axis=0, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    order = np.argsort(first, kind='stable')             # groups in first-occurrence order
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    wsum = np.zeros(len(order))
+    np.add.at(wsum, rank[inv], w)
+    return np.ascontiguousarray(off[np.sort(first)]), wsum
+
+
 def _effpot_setup(guest, epot_generator_dict, rotations):
     """Device inputs of cmm_effpot: host table, guest types and parameters,
     the (nrot, ng, 3) offsets COM + R_r (x_i - COM) and their largest norm."""
@@ -210,12 +230,13 @@ def effective_potential_vectorized(guest, position_shifts, epot_generator_dict, 
     torch = torch_mod()
     guest = _guest_system(guest)
     g0, gtype, gsig, geps, off, dmax = _effpot_setup(guest, epot_generator_dict, rotations)
+    off, w = _merge_orientations(off, weights)
     was_tensor = isinstance(position_shifts, torch.Tensor)
     P = as_device(position_shifts).reshape(-1, 3).contiguous()
     M = P.shape[0]
     out = torch.empty(M, dtype=torch.float64, device='cuda')
-    wdev = as_device(np.asarray(weights, dtype=np.float64))
-    _launch_effpot(None, P, M, g0, gtype, gsig, geps, as_device(off), wdev, len(weights), dmax, beta,
+    wdev = as_device(w)
+    _launch_effpot(None, P, M, g0, gtype, gsig, geps, as_device(off), wdev, len(w), dmax, beta,
                    limit_potential, out)
     return out if was_tensor else out.cpu().numpy()
 
@@ -249,7 +270,8 @@ def generate_effective_potential(guest_chk_fn, points, beta, epot_generator_dict
     guest = _guest_system(guest_chk_fn)
     rot, w = combined_rotations(degree)
     g0, gtype, gsig, geps, off, dmax = _effpot_setup(guest, epot_generator_dict, rot)
-    wdev = as_device(np.asarray(w, dtype=np.float64))
+    off, w = _merge_orientations(off, w)
+    wdev = as_device(w)
     offdev = as_device(off)
     if hasattr(points, 'npoints'):
         grid = points

@@ -362,3 +362,25 @@ def test_lj_derivatives_are_derivatives():
         fd = (ch(lower, e[axis]) - ch(lower, -e[axis])) / (2 * d) * h[axis]
         scale = np.abs(base[q][ok]).max()
         assert np.abs(fd[ok] - base[q][ok]).max() <= 1e-5 * scale, q
+
+
+def test_coinciding_orientations_are_merged():
+    # a linear guest along z: the rotations about z give the same site
+    # offsets, so each Lebedev direction is evaluated once with the summed weight
+    from cmmdft_amd.extpot import _merge_orientations
+    for name, merged in (('co2', True), ('c2h6', False)):
+        chk, _, _ = _guest(name)
+        for degree in (5, 9):
+            rot, w = O.orientations(degree)
+            com = (chk.pos * chk.masses[:, None]).sum(0) / chk.masses.sum()
+            off = np.einsum('rij,nj->rni', rot, chk.pos - com) + com
+            o2, w2 = _merge_orientations(off, w)
+            nleb = len(O.lebedev_table(degree)[1])
+            assert len(w2) == (nleb if merged else len(w))
+            assert abs(w2.sum() - w.sum()) <= 1e-15
+            # every original offset set is within rounding of a kept one
+            d = np.abs(off[:, None] - o2[None]).max(axis=(2, 3)).min(axis=1)
+            assert d.max() <= 1e-12 * np.abs(off).max()
+            if merged:
+                # the kept weights are the per-direction sums
+                assert np.allclose(w2, w.reshape(nleb, degree).sum(axis=1), rtol=1e-15, atol=0)
