@@ -391,11 +391,12 @@ class Anderson(Picard):
             # residual of the current density into the ring (solver.py:550-552)
             rslot = nsaved % m
             eng.plan.finish_anderson(ring[cur], eng.V, False, rring, m, rslot, None, eng.beta, eng.lam3)
-            # the pure method always takes the Anderson step, so its residual
-            # checks wait for the one host read after the step (the residual
-            # pass's slots survive the mixing and derive passes)
-            pure = 'hybrid' not in self.Anderson_method.lower()
-            if not pure:
+            # the pure method always takes the Anderson step, and the hybrid
+            # one does once it has taken one (And_true stays set); then the
+            # residual checks are deferred to the one host read after the step
+            # (the residual pass's slots survive the mixing and derive passes)
+            deferred = 'hybrid' not in self.Anderson_method.lower() or self.And_true
+            if not deferred:
                 sA = eng.plan.scalars()[0]
                 if sA[L.S_AND_NONFIN] > 0:
                     # G of the current density is not finite: the reference falls
@@ -407,13 +408,13 @@ class Anderson(Picard):
                 free_rho.append(old_rho)
             hist.append((cur, rslot))
             nsaved += 1
-            if not pure:
+            if not deferred:
                 res_norm = np.sqrt(max(sA[L.S_GRAM + rslot], 0.0))
                 N_cur = sA[L.S_NCUR] * eng.dr
                 it_eps = res_norm / N_cur if N_cur != 0 else np.inf
                 if self.curr_step < 3:
                     self.it_eps0 = it_eps
-            cond = pure or ((it_eps <= self.it_eps0 * self.delta) and self.curr_step > 4) or self.And_true
+            cond = deferred or ((it_eps <= self.it_eps0 * self.delta) and self.curr_step > 4) or self.And_true
             nxt = free_rho.pop(0)
             used_anderson = False
             if cond:
@@ -427,7 +428,7 @@ class Anderson(Picard):
                 eng.plan.anderson_mix(ring, order_rho, rring, order_r, ring[cur], ring[nxt], eng.V, False, eng.lam3)
                 eng.plan.derive()
                 s = eng.plan.scalars()[0]
-                if pure:
+                if deferred:
                     if s[L.S_AND_NONFIN] > 0:
                         raise FloatingPointError('non-finite Boltzmann factor')
                     res_norm = np.sqrt(max(s[L.S_GRAM + rslot], 0.0))
