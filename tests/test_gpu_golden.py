@@ -86,3 +86,48 @@ def test_converged_solution_against_fixture(method):
     om = grand_potential(prog.fener, mu, rho)
     oref = float(FIX['omega_conv'])
     assert abs(om - oref) <= 1e-8 * abs(oref)
+
+
+EFF = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'oracle_effpot.npz'))
+INPUTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'inputs')
+
+
+@pytest.mark.parametrize('name', ['co2', 'c2h6'])
+def test_effective_potential_against_fixture(name):
+    # tests/golden/oracle_effpot.npz: the oracle's float64 average (1e-10)
+    # and the reference's float32 form (4e-7 relative, or 4e-7 kT)
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
+    host = W.host_case1()
+    shape = tuple(int(n) for n in EFF['shape_%s' % name])
+    gens = get_external_potential_dict(host.ff, os.path.join(INPUTS, '%s_pars_vdw_AA_trappe.txt' % name), host,
+                                       cutoff=W.CUTOFF)
+    V = generate_effective_potential(os.path.join(INPUTS, '%s_AA.chk' % name), Grid(EFF['rvecs'], shape), BETA, gens,
+                                     degree=int(EFF['degree_%s' % name]))
+    lim = 1e4 * U.kjmol
+
+    def clamp(x):
+        return np.minimum(np.where(np.isfinite(x), x, lim), lim)
+    V, r64, r32 = clamp(V), clamp(EFF['effpot_%s_f64' % name]), clamp(EFF['effpot_%s_f32' % name])
+    m = r64 < lim
+    assert np.array_equal(V >= lim, r64 >= lim)
+    assert np.abs(V[m] - r64[m]).max() <= 1e-10 * np.abs(r64[m]).max()
+    assert np.all(np.abs(V - r32) <= 4e-7 * np.maximum(np.abs(r32), 1 / BETA))
+
+
+def test_lj_derivative_table_and_interpolation_against_fixture():
+    from cmmdft_amd.system import Grid
+    from cmmdft_amd.extpot import get_external_potential_derivatives, Interpolator
+    host = W.host_case1()
+    tshape = tuple(int(n) for n in EFF['table_shape'])
+    g = Grid(EFF['rvecs'], tshape)
+    tab = get_external_potential_derivatives(g, host.ff, 3.05 * U.angstrom, 79.0 * U.boltzmann, host, g.spacings,
+                                             cutoff=W.CUTOFF)
+    for q in range(8):
+        scale = np.abs(EFF['table'][q]).max()
+        assert np.abs(tab[q] - EFF['table'][q]).max() <= 1e-10 * scale, q
+    it = Interpolator(EFF['interp_table'], EFF['interp_origin'], EFF['interp_spacing'])
+    pts = EFF['interp_points']
+    scale = np.abs(EFF['interp_table']).max()
+    for meth in ('tricubic', 'trilinear', 'tricubic_estimated'):
+        assert np.abs(getattr(it, meth)(pts) - EFF[meth]).max() <= 1e-12 * 8 * scale, meth
