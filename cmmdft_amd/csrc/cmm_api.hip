@@ -533,7 +533,7 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
 // nx * tb columns with all fields in LDS ((nl + 1) nx tb complex values for
 // nl fields, plus the static twiddle table, at most 64 KB together)
 static int xpass_tb(cmm_plan* p, int nl) {
-  const int lim = (65536 - CMM_MAX_N * 16) / (16 * (nl + 1) * p->nx);
+  const int lim = (65536 - CMM_RED_SH * 8 - 16 * p->nx) / (16 * nl * p->nx);
   return std::max(1, std::min(p->tbx, lim));
 }
 
@@ -549,11 +549,21 @@ static unsigned xpass_threads(int T, int nl) {
   return (unsigned)std::min(NTX, ((need + 63) / 64) * 64);
 }
 
+// Full-size x-pass blocks (NTX threads, the 128^3 tile of 4 columns) hold
+// ~53 KB of LDS, so three fit on a CU when the kernel is held to 6 waves per
+// SIMD (80 VGPRs).  That spills ~25 VGPRs to scratch and is still faster
+// (measured at 128^3: k_xweights -16 %, k_xback -11 % for one instance,
+// -10 % / -13 % for 32); smaller blocks already fit five per CU and keep
+// the unconstrained build.
+constexpr int XOCC_FULL = 6;
+
 template <int NF, bool MFA>
 static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   const int T = p->nx * tb;
+  const unsigned nt = xpass_threads(T, NF);
+  auto kern = nt == (unsigned)NTX ? k_xweights<NF, MFA, XOCC_FULL> : k_xweights<NF, MFA, 1>;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL((k_xweights<NF, MFA>), grid, dim3(xpass_threads(T, NF)), (size_t)(NF + 1) * T * sizeof(double2),
+  hipLaunchKernelGGL(kern, grid, dim3(nt), (size_t)(NF * T + p->nx) * sizeof(double2),
                      p->stream,
                      p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
                      make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)p->nzh), per,
@@ -581,8 +591,10 @@ template <int NF, bool MFA>
 static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   constexpr int NL = NF + (MFA ? 1 : 0);
   const int T = p->nx * tb;
+  const unsigned nt = xpass_threads(T, NL);
+  auto kern = nt == (unsigned)NTX ? k_xback<NF, MFA, XOCC_FULL> : k_xback<NF, MFA, 1>;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL((k_xback<NF, MFA>), grid, dim3(xpass_threads(T, NL)), (size_t)(NL + 1) * T * sizeof(double2),
+  hipLaunchKernelGGL(kern, grid, dim3(nt), (size_t)(NL * T + p->nx) * sizeof(double2),
                      p->stream, p->F,
                      (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb, make_fastdiv((unsigned)tb),
                      make_fastdiv((unsigned)(NL * tb)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh), per,

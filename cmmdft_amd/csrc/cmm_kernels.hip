@@ -331,6 +331,11 @@ __device__ __forceinline__ KW kweights(const GridDev& g, const FuncDev& fp, int 
   const double wd = kn * fp.D;
   if (wd <= 1e-8) {
     r.wD = sl;
+  } else if (fp.D == 2.0 * fp.R && !r.k0) {
+    // WDA diameter D = 2R (the hard-sphere radius): wd = 2w exactly, and the
+    // double-angle identities reuse sin w, cos w
+    const double s2 = 2.0 * s * c, c2 = (c - s) * (c + s);
+    r.wD = j_ball(wd, s2, c2) * sl;
   } else {
     sincos(wd, &s, &c);
     r.wD = j_ball(wd, s, c) * sl;
@@ -395,13 +400,13 @@ constexpr int NTX = 512;
 // |rho^(k)|^2 K(k) over the full spectrum (K Hermitian, so the sum is real):
 // on the half spectrum the planes kz = 0 and kz = Nyquist count once, the
 // others twice.  Block partials are indexed by tile (ntiles of them).
-template <int NF, bool MFA>
-__global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
+template <int NF, bool MFA, int OCC>
+__global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC))) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
                                                  long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
                                                  FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
                                                  const int* __restrict__ active, RedDev red) {
-  extern __shared__ double2 sm[];
-  __shared__ double2 s_tw[CMM_MAX_N];
+  extern __shared__ double2 sm[];    // [NF T] fields, then the nx twiddles
+  double2* s_tw = sm + NF * g.nx * TB;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
@@ -409,8 +414,8 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
   const int S6 = NF * TB;
-  double2* A = sm;        // [nx][TB]
-  double2* B = sm + T;    // [nx][NF TB]
+  double2* B = sm;                    // [nx][NF TB]
+  double2* A = sm + (NF - 1) * TB;    // rho^ in the slot of field NF-1: [x * S6 + b]
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
@@ -428,20 +433,22 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
       const int i = i0 + u * (int)blockDim.x;
       if (i < T) {
         const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-        A[i] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+        A[x * S6 + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
       }
     }
   }
   __syncthreads();
-  lds_fft<-1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
+  lds_fft<-1, true, VPT>(A, TB, fTB, 1, S6, g.fx, s_tw);
   double emfa = 0.0;
+  // each element's weighted copies overwrite only that element's own slots
+  // (its rho^ among them), and one thread reads and writes them
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
     const KW w = kweights(g, fp, x, wy, wz);
-    const double2 a = A[i];
+    const double2 a = A[x * S6 + b];
     if (MFA && b < tbv) {
       const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
       emfa += half * (a.x * a.x + a.y * a.y) * fp.kmfa[(long long)x * nb + bb].x;
@@ -473,14 +480,14 @@ __global__ void __launch_bounds__(NTX) k_xweights(const double2* __restrict__ rz
 // tile of every field before it writes field 0's tile).
 // MFA: the density's spectrum (rz, z and y transformed) is loaded as field NF
 // and transformed with the others; K rho^ joins the weighted sum.
-template <int NF, bool MFA>
-__global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
+template <int NF, bool MFA, int OCC>
+__global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC))) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
                                               FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fT, FastDiv fnzh,
                                               int per, long long ntiles, const int* __restrict__ active,
                                               const double2* __restrict__ rz, long long rzs) {
   constexpr int NL = NF + (MFA ? 1 : 0);    // fields held in LDS
-  extern __shared__ double2 sm[];
-  __shared__ double2 s_tw[CMM_MAX_N];
+  extern __shared__ double2 sm[];    // [NL T] fields, then the nx twiddles
+  double2* s_tw = sm + NL * g.nx * TB;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
@@ -488,8 +495,7 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
   const int S6 = NL * TB;
-  double2* B = sm;             // [nx][NL TB]
-  double2* A = sm + NL * T;    // [nx][TB]
+  double2* B = sm;    // [nx][NL TB]; the weighted sum goes to field 0's slot
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
@@ -527,14 +533,14 @@ __global__ void __launch_bounds__(NTX) k_xback(double2* __restrict__ F, long lon
 #pragma unroll
     for (int f = 0; f < NF; ++f) acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, f, B[x * S6 + f * TB + b], 1), g.inv_npts));
     if (MFA) acc = cadd(acc, cmul(B[x * S6 + NF * TB + b], fp.kmfa[(long long)x * nb + bb]));
-    A[i] = acc;
+    B[x * S6 + b] = acc;    // this element's slots are read above by this thread only
   }
   __syncthreads();
-  lds_fft<1, true, VPT>(A, TB, fTB, 1, TB, g.fx, s_tw);
+  lds_fft<1, true, VPT>(B, TB, fTB, 1, S6, g.fx, s_tw);
   double2* dst = F + inst * Fs + b0;
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-    if (b < tbv) dst[(long long)x * nb + b] = A[i];
+    if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + b];
   }
 }
 

@@ -45,14 +45,21 @@ def test_real_framework_external_potential(name, shape):
     assert (np.abs(V - Vr)[big] / np.abs(Vr[big])).max() <= 1e-10
 
 
-@pytest.mark.parametrize('name,shape,version,p_bar', [('ZIF8', (24, 24, 24), 'WBII', 10.0),
-                                                      ('CHA', (24, 24, 25), 'atWBII', 0.1)])
-def test_real_framework_solution(name, shape, version, p_bar):
+@pytest.mark.parametrize('name,shape,version,p_bar,tol_dilute', [('ZIF8', (24, 24, 24), 'WBII', 10.0, 1e-8),
+                                                                 ('CHA', (24, 24, 25), 'atWBII', 0.1, 3e-8)])
+def test_real_framework_solution(name, shape, version, p_bar, tol_dilute):
     """Methane, hard spheres + mean-field WDA + LJ: the product program built
     from the input files against the oracle's Anderson solve (the fixed
     point; chabazite's deep wells, -24 kJ/mol, make the reference's static
     Picard crawl, and at 10 bar its ideal-gas start overfills the cages so
-    that the reference's first steps are non-finite: 0.1 bar there)."""
+    that the reference's first steps are non-finite: 0.1 bar there).
+
+    The RIUE criterion bounds the integrated change, so in chabazite's most
+    dilute corners (rho ~ 4e-8 of its maximum) the converged density is fixed
+    only to ~1e-8 relative: the oracle started from rho0 (1 + 1e-12) lands
+    1.15e-8 from its own unperturbed solution there, and 1.2e-9 where rho
+    exceeds 1e-4 of the maximum.  Hence 3e-8 over every rho > 1e-10 A^-3 and
+    1e-8 over the dense part."""
     from cmmdft_amd.program import Program
     from cmmdft_amd.system import SphericalLJGuest
     from cmmdft_amd.eos import MFAEOS
@@ -78,7 +85,9 @@ def test_real_framework_solution(name, shape, version, p_bar):
     N, rho, conv = prog.solve(mu)
     assert conv
     rel, mism = rho_rel_err(rho, rho_ref)
-    assert rel <= 1e-8, rel
+    assert rel <= tol_dilute, rel
+    dense = rho_ref > 1e-4 * rho_ref.max()
+    assert np.max(np.abs(rho[dense] - rho_ref[dense]) / rho_ref[dense]) <= 1e-8
     assert abs(N - Nref) <= 1e-8 * Nref
     om = grand_potential(prog.fener, mu, rho)
     assert abs(om - omega_ref) <= 1e-8 * abs(omega_ref), (om, omega_ref)
