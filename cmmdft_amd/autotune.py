@@ -147,7 +147,11 @@ def _tile_candidates(plan):
     nx, ny, nz = plan.npoints
     budget = 8 * 256
     tbx = [t for t in range(1, 33) if t * nx <= 512 and (t & (t - 1) == 0 or t == 512 // nx)]
-    tby = [t for t in (1, 2, 4, 8, 16, 32) if t * ny <= budget and t * ny * 16 <= 65536]
+    # y tiles: powers of two, and the widths that split the nz/2+1 columns into
+    # k equal tiles (65 = 5 x 13 at 128^3: no near-empty last tile)
+    nzh = nz // 2 + 1
+    even = [-(-nzh // k) for k in range(1, 9)]
+    tby = sorted({t for t in [1, 2, 4, 8, 16, 32] + even if t * ny <= budget and t * ny * 16 <= 65536})
     pz = [p for p in (2, 4, 8, 16, 32) if (p // 2) * nz <= budget and nz * ((p // 2) | 1) * 16 <= 65536]
     pphi = [p for p in (2, 4, 6, 8, 10, 12, 16) if 3 * p * nz <= budget and nz * ((3 * p) | 1) * 16 <= 65536]
     return tbx, tby, pz, pphi

@@ -530,8 +530,8 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
 }
 
 // x passes of the functional chain (see k_xweights / k_xback): tiles of
-// nx * tb columns with all fields in LDS ((nl + 1) nx tb complex values for
-// nl fields, plus the static twiddle table, at most 64 KB together)
+// nx * tb columns with all fields in LDS (nl nx tb complex values for nl
+// fields, the nx twiddles and the reduction scratch, at most 64 KB together)
 static int xpass_tb(cmm_plan* p, int nl) {
   const int lim = (65536 - CMM_RED_SH * 8 - 16 * p->nx) / (16 * nl * p->nx);
   return std::max(1, std::min(p->tbx, lim));

@@ -34,20 +34,21 @@ __device__ __forceinline__ double& z_val(double2* lds, const ZTile& t, int f, in
 // Loaders issue the global loads of LU elements per thread before any of
 // their LDS stores, with addresses clamped into range and out-of-range values
 // replaced afterwards: no branch separates the loads, so they are all in
-// flight together.
+// flight together.  Offsets inside one field of one instance are 32-bit
+// (pencils nx ny <= 65536, nz/2+1 <= 129 elements each).
 constexpr int LU = 4;
 
 __device__ __forceinline__ void z_load_real(double2* lds, const double* src, long long p0, const ZTile& t, long long npen,
                                             int n, int f) {
   const int tot = t.P * n;
+  const int ip0 = (int)p0, ilast = (int)npen - 1;
   for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
     double v[LU];
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = min(i0 + u * (int)blockDim.x, tot - 1);
       const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
-      const long long p = min(p0 + pl, npen - 1);
-      v[u] = src[p * n + e];
+      v[u] = src[min(ip0 + pl, ilast) * n + e];
     }
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
@@ -64,6 +65,7 @@ __device__ __forceinline__ void z_load_real(double2* lds, const double* src, lon
 __device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, long long p0, const ZTile& t, long long npen,
                                              int n, int nzh, int f) {
   const int tot = t.B * nzh;
+  const int ip0 = (int)p0, inpen = (int)npen;
 #pragma unroll 4
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
@@ -71,9 +73,9 @@ __device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, l
     const double2 zm = lds[(k == 0 ? 0 : n - k) * t.ES + f * t.B + q];
     const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
     const double2 Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
-    const long long pa = p0 + 2 * q, pb = pa + 1;
-    if (pa < npen) dst[pa * nzh + k] = A;
-    if (pb < npen) dst[pb * nzh + k] = Bv;
+    const int pa = ip0 + 2 * q, o = pa * nzh + k;
+    if (pa < inpen) dst[o] = A;
+    if (pa + 1 < inpen) dst[o + nzh] = Bv;
   }
 }
 
@@ -84,24 +86,25 @@ __device__ __forceinline__ void z_store_half(const double2* lds, double2* dst, l
 __device__ __forceinline__ void z_load_half(double2* lds, const double2* src, long long p0, const ZTile& t, long long npen,
                                             int n, int nzh, int f) {
   const int tot = t.B * nzh;
+  const int ip0 = (int)p0, ilast = (int)npen - 1;
   for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
     double2 A[LU], B[LU];
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = min(i0 + u * (int)blockDim.x, tot - 1);
       const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
-      const long long pa = p0 + 2 * q;
-      A[u] = src[min(pa, npen - 1) * nzh + k];
-      B[u] = src[min(pa + 1, npen - 1) * nzh + k];
+      const int pa = ip0 + 2 * q;
+      A[u] = src[min(pa, ilast) * nzh + k];
+      B[u] = src[min(pa + 1, ilast) * nzh + k];
     }
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       if (i < tot) {
         const int q = (int)fdiv((unsigned)i, t.fnzh), k = i - q * nzh;
-        const long long pa = p0 + 2 * q;
-        double2 a = (pa < npen) ? A[u] : make_double2(0.0, 0.0);
-        double2 b = (pa + 1 < npen) ? B[u] : make_double2(0.0, 0.0);
+        const int pa = ip0 + 2 * q;
+        double2 a = (pa <= ilast) ? A[u] : make_double2(0.0, 0.0);
+        double2 b = (pa + 1 <= ilast) ? B[u] : make_double2(0.0, 0.0);
         const bool selfconj = (k == 0) || (2 * k == n);
         if (selfconj) { a.y = 0.0; b.y = 0.0; }
         lds[k * t.ES + f * t.B + q] = make_double2(a.x - b.y, a.y + b.x);
@@ -116,6 +119,7 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
                                                   const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn) {
   const int per = t.B * nzh;
   const int tot = nf * per;
+  const int ip0 = (int)p0, ilast = (int)npen - 1;
   for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
     double2 A[LU], B[LU];
 #pragma unroll
@@ -124,10 +128,10 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
       const int f = (int)fdiv((unsigned)i, fBn);
       const int r = i - f * per;
       const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
-      const long long pa = p0 + 2 * q;
+      const int pa = ip0 + 2 * q;
       const double2* sf = src + f * fs;
-      A[u] = sf[min(pa, npen - 1) * nzh + k];
-      B[u] = sf[min(pa + 1, npen - 1) * nzh + k];
+      A[u] = sf[min(pa, ilast) * nzh + k];
+      B[u] = sf[min(pa + 1, ilast) * nzh + k];
     }
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
@@ -136,9 +140,9 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
         const int f = (int)fdiv((unsigned)i, fBn);
         const int r = i - f * per;
         const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
-        const long long pa = p0 + 2 * q;
-        double2 a = (pa < npen) ? A[u] : make_double2(0.0, 0.0);
-        double2 b = (pa + 1 < npen) ? B[u] : make_double2(0.0, 0.0);
+        const int pa = ip0 + 2 * q;
+        double2 a = (pa <= ilast) ? A[u] : make_double2(0.0, 0.0);
+        double2 b = (pa + 1 <= ilast) ? B[u] : make_double2(0.0, 0.0);
         const bool selfconj = (k == 0) || (2 * k == n);
         if (selfconj) { a.y = 0.0; b.y = 0.0; }
         lds[k * t.ES + f * t.B + q] = make_double2(a.x - b.y, a.y + b.x);
@@ -161,10 +165,10 @@ __device__ __forceinline__ void z_store_half_multi(const double2* lds, double2* 
     const double2 zm = lds[(k == 0 ? 0 : n - k) * t.ES + f * t.B + q];
     const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
     const double2 Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
-    const long long pa = p0 + 2 * q, pb = pa + 1;
+    const int pa = (int)p0 + 2 * q, o = pa * nzh + k;
     double2* df = dst + f * fs;
-    if (pa < npen) df[pa * nzh + k] = A;
-    if (pb < npen) df[pb * nzh + k] = Bv;
+    if (pa < (int)npen) df[o] = A;
+    if (pa + 1 < (int)npen) df[o + nzh] = Bv;
   }
 }
 
