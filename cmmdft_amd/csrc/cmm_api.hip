@@ -549,22 +549,29 @@ static unsigned xpass_threads(int T, int nl) {
   return (unsigned)std::min(NTX, ((need + 63) / 64) * 64);
 }
 
-// Full-size x-pass blocks (NTX threads, the 128^3 tile of 4 columns) hold
-// ~53 KB of LDS, so three fit on a CU when the kernel is held to 6 waves per
-// SIMD (80 VGPRs).  That spills ~25 VGPRs to scratch and is still faster
-// (measured at 128^3: k_xweights -16 %, k_xback -11 % for one instance,
-// -10 % / -13 % for 32); smaller blocks already fit five per CU and keep
+// Full-size x-pass blocks (NTX threads, the 128^3 tile of 4 columns of six
+// fields) hold ~53 KB of LDS, so three fit on a CU when the kernel is held to
+// 6 waves per SIMD (80 VGPRs).  That spills ~25 VGPRs to scratch and is still
+// faster (measured at 128^3: k_xweights -16 %, k_xback -11 % for one
+// instance, -10 % / -13 % for 32).  Blocks whose LDS admits only two per CU
+// gain nothing from it, and smaller blocks already fit five per CU: both keep
 // the unconstrained build.
 constexpr int XOCC_FULL = 6;
+constexpr size_t CU_LDS = 160 * 1024;   // LDS of one gfx950 CU
+
+static bool xpass_full_occ(unsigned nt, size_t lds_bytes) {
+  return nt == (unsigned)NTX && 3 * lds_bytes <= CU_LDS;
+}
 
 template <int NF, bool MFA>
 static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NF);
-  auto kern = nt == (unsigned)NTX ? k_xweights<NF, MFA, XOCC_FULL> : k_xweights<NF, MFA, 1>;
+  const size_t lds = (size_t)(NF * T + p->nx) * sizeof(double2);
+  auto kern = xpass_full_occ(nt, lds + (MFA ? CMM_RED_SH * sizeof(double) : 0)) ? k_xweights<NF, MFA, XOCC_FULL>
+                                                                                 : k_xweights<NF, MFA, 1>;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL(kern, grid, dim3(nt), (size_t)(NF * T + p->nx) * sizeof(double2),
-                     p->stream,
+  hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream,
                      p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
                      make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)p->nzh), per,
                      ntiles, p->active, red_of(p));
@@ -592,10 +599,10 @@ static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   constexpr int NL = NF + (MFA ? 1 : 0);
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NL);
-  auto kern = nt == (unsigned)NTX ? k_xback<NF, MFA, XOCC_FULL> : k_xback<NF, MFA, 1>;
+  const size_t lds = (size_t)(NL * T + p->nx) * sizeof(double2);
+  auto kern = xpass_full_occ(nt, lds) ? k_xback<NF, MFA, XOCC_FULL> : k_xback<NF, MFA, 1>;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
-  hipLaunchKernelGGL(kern, grid, dim3(nt), (size_t)(NL * T + p->nx) * sizeof(double2),
-                     p->stream, p->F,
+  hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream, p->F,
                      (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb, make_fastdiv((unsigned)tb),
                      make_fastdiv((unsigned)(NL * tb)), make_fastdiv((unsigned)T), make_fastdiv((unsigned)p->nzh), per,
                      ntiles, p->active, p->rz, p->nhalf);
