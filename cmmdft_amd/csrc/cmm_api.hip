@@ -532,6 +532,9 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
 // x passes of the functional chain (see k_xweights / k_xback): tiles of
 // nx * tb columns with all fields in LDS (nl nx tb complex values for nl
 // fields, the nx twiddles and the reduction scratch, at most 64 KB together)
+// fields an x pass weights and transforms (XFields: four of the six)
+static int xfields(const cmm_plan* p) { return p->nf == 6 ? XFields<6>::nc : p->nf; }
+
 static int xpass_tb(cmm_plan* p, int nl) {
   const int lim = (65536 - CMM_RED_SH * 8 - 16 * p->nx) / (16 * nl * p->nx);
   return std::max(1, std::min(p->tbx, lim));
@@ -565,21 +568,22 @@ static bool xpass_full_occ(unsigned nt, size_t lds_bytes) {
 
 template <int NF, bool MFA>
 static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
+  constexpr int NC = XFields<NF>::nc;
   const int T = p->nx * tb;
-  const unsigned nt = xpass_threads(T, NF);
-  const size_t lds = (size_t)(NF * T + p->nx) * sizeof(double2);
+  const unsigned nt = xpass_threads(T, NC);
+  const size_t lds = (size_t)(NC * T + p->nx) * sizeof(double2);
   auto kern = xpass_full_occ(nt, lds + (MFA ? CMM_RED_SH * sizeof(double) : 0)) ? k_xweights<NF, MFA, XOCC_FULL>
                                                                                  : k_xweights<NF, MFA, 1>;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
   hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream,
                      p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
-                     make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NF * tb)), make_fastdiv((unsigned)p->nzh), per,
+                     make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NC * tb)), make_fastdiv((unsigned)p->nzh), per,
                      ntiles, p->active, red_of(p));
 }
 
 static int launch_xweights(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
-  const int tb = xpass_tb(p, p->nf);
+  const int tb = xpass_tb(p, xfields(p));
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
   const bool mfa = p->fp.kmfa != nullptr;
@@ -596,7 +600,7 @@ static int launch_xweights(cmm_plan* p) {
 
 template <int NF, bool MFA>
 static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
-  constexpr int NL = NF + (MFA ? 1 : 0);
+  constexpr int NL = XFields<NF>::nc + (MFA ? 1 : 0);
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NL);
   const size_t lds = (size_t)(NL * T + p->nx) * sizeof(double2);
@@ -611,7 +615,7 @@ static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
 static int launch_xback(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
   const bool mfa = p->fp.kmfa != nullptr;
-  const int tb = xpass_tb(p, p->nf + (mfa ? 1 : 0));
+  const int tb = xpass_tb(p, xfields(p) + (mfa ? 1 : 0));
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
   if (p->nf == 12) {

@@ -396,7 +396,26 @@ __device__ __forceinline__ double2 apply_w(const GridDev& g, const KW& w, int ix
 // ---------------------------------------------------------------------------
 constexpr int NTX = 512;
 
-// forward chain: rho (z,y transformed) -> x FFT -> NF weighted copies ->
+// Six-field functionals (NF = 6) weight and transform four fields only: on a
+// column of the x pass ky and kz are constant, so the y and z components of
+// n_v2, i ky w3 rho^ and i kz w3 rho^ (zero on their axis' Nyquist plane),
+// are i ky' and i kz' times the n3 column before and after the x transform,
+// and in the back-convolution i ky' G_y + i kz' G_z joins G_3 before it.
+// Slots 0..3 hold the fields n2, n3, nv2_x, rho_bar (weights 0, 1, 2, 5).
+template <int NF>
+struct XFields {
+  static constexpr bool red = (NF == 6);
+  static constexpr int nc = red ? 4 : NF;   // fields weighted and transformed
+  __device__ static constexpr int field(int c) { return (red && c == 3) ? 5 : c; }
+};
+
+// ky and kz of a column, zero on their axis' Nyquist plane
+__device__ __forceinline__ void column_k(const GridDev& g, int wy, int wz, double& kyp, double& kzp) {
+  kyp = (wy == g.ky.nyq) ? 0.0 : g.ky.k[wy];
+  kzp = (wz == g.kz.nyq) ? 0.0 : g.kz.k[wz];
+}
+
+// forward chain: rho (z,y transformed) -> x FFT -> weighted copies ->
 // inverse x FFT of all of them -> F[0..NF).  The 1/N of the inverse 3-D
 // transform is folded into the weights.  NF = 6 (fields n2, n3, nv2 x/y/z,
 // rho_bar) or 12 (plus the six tensor components).
@@ -405,21 +424,23 @@ constexpr int NTX = 512;
 // on the half spectrum the planes kz = 0 and kz = Nyquist count once, the
 // others twice.  Block partials are indexed by tile (ntiles of them).
 template <int NF, bool MFA, int OCC>
-__global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC))) k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F,
-                                                 long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB,
-                                                 FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
-                                                 const int* __restrict__ active, RedDev red) {
-  extern __shared__ double2 sm[];    // [NF T] fields, then the nx twiddles
-  double2* s_tw = sm + NF * g.nx * TB;
+__global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
+k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F, long long Fs, long long Ffs,
+           GridDev g, FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
+           const int* __restrict__ active, RedDev red) {
+  using XF = XFields<NF>;
+  constexpr int NC = XF::nc;
+  extern __shared__ double2 sm[];    // [NC T] fields, then the nx twiddles
+  double2* s_tw = sm + NC * g.nx * TB;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
   const int T = g.nx * TB;
-  const int S6 = NF * TB;
-  double2* B = sm;                    // [nx][NF TB]
-  double2* A = sm + (NF - 1) * TB;    // rho^ in the slot of field NF-1: [x * S6 + b]
+  const int S6 = NC * TB;
+  double2* B = sm;                    // [nx][NC TB]
+  double2* A = sm + (NC - 1) * TB;    // rho^ in the slot of the last field: [x * S6 + b]
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
@@ -458,7 +479,8 @@ __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
       emfa += half * (a.x * a.x + a.y * a.y) * fp.kmfa[(long long)x * nb + bb].x;
     }
 #pragma unroll
-    for (int f = 0; f < NF; ++f) B[x * S6 + f * TB + b] = cscale(apply_w(g, w, x, wy, wz, f, a, -1), g.inv_npts);
+    for (int c = 0; c < NC; ++c)
+      B[x * S6 + c * TB + b] = cscale(apply_w(g, w, x, wy, wz, XF::field(c), a, -1), g.inv_npts);
   }
   if (MFA) {
     __shared__ double sh[CMM_RED_SH];
@@ -469,12 +491,35 @@ __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
   }
   __syncthreads();
   lds_fft<1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    double2* dst = F + inst * Fs + f * Ffs + b0;
+  if constexpr (XF::red) {
+    // one element's six fields per thread; consecutive threads write
+    // consecutive columns of a row of each field
+    double2* dst = F + inst * Fs + b0;
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
       const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
-      if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + f * TB + b];
+      if (b >= tbv) continue;
+      const unsigned bb = (unsigned)(b0 + b);
+      const int wy = (int)fdiv(bb, fnzh);
+      const int wz = (int)bb - wy * g.nzh;
+      double kyp, kzp;
+      column_k(g, wy, wz, kyp, kzp);
+      const long long o = (long long)x * nb + b;
+      const double2 n3 = B[x * S6 + TB + b];
+      dst[o] = B[x * S6 + b];
+      dst[Ffs + o] = n3;
+      dst[2 * Ffs + o] = B[x * S6 + 2 * TB + b];
+      dst[3 * Ffs + o] = cmul_i(cscale(n3, kyp), -1.0);
+      dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
+      dst[5 * Ffs + o] = B[x * S6 + 3 * TB + b];
+    }
+  } else {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double2* dst = F + inst * Fs + f * Ffs + b0;
+      for (int i = threadIdx.x; i < T; i += blockDim.x) {
+        const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+        if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + f * TB + b];
+      }
     }
   }
 }
@@ -482,14 +527,16 @@ __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
 // back-convolution: the NF derivative fields -> x FFT of all of them ->
 // weighted sum -> inverse x FFT -> F[0] (in place: the block reads its whole
 // tile of every field before it writes field 0's tile).
-// MFA: the density's spectrum (rz, z and y transformed) is loaded as field NF
-// and transformed with the others; K rho^ joins the weighted sum.
+// MFA: the density's spectrum (rz, z and y transformed) is loaded as one more
+// field and transformed with the others; K rho^ joins the weighted sum.
 template <int NF, bool MFA, int OCC>
-__global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC))) k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
-                                              FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fT, FastDiv fnzh,
-                                              int per, long long ntiles, const int* __restrict__ active,
-                                              const double2* __restrict__ rz, long long rzs) {
-  constexpr int NL = NF + (MFA ? 1 : 0);    // fields held in LDS
+__global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
+k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB, FastDiv fTB,
+        FastDiv fS6, FastDiv fT, FastDiv fnzh, int per, long long ntiles, const int* __restrict__ active,
+        const double2* __restrict__ rz, long long rzs) {
+  using XF = XFields<NF>;
+  constexpr int NC = XF::nc;
+  constexpr int NL = NC + (MFA ? 1 : 0);    // fields held in LDS
   extern __shared__ double2 sm[];    // [NL T] fields, then the nx twiddles
   double2* s_tw = sm + NL * g.nx * TB;
   load_twiddles(s_tw, g.fx);
@@ -505,23 +552,49 @@ __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = F + inst * Fs + b0;
   const double2* rsrc = MFA ? rz + inst * rzs + b0 : nullptr;
-  for (int i0 = threadIdx.x; i0 < NL * T; i0 += LU * blockDim.x) {
-    double2 v[LU];
-#pragma unroll
-    for (int u = 0; u < LU; ++u) {
-      const int i = min(i0 + u * (int)blockDim.x, NL * T - 1);
-      const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
-      const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
+  if constexpr (XF::red) {
+    // one element's six fields (and rho^) per thread, loads in flight
+    // together; G_3 + i ky' G_y + i kz' G_z formed before the transform
+    for (int i = threadIdx.x; i < T; i += blockDim.x) {
+      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
       const long long o = (long long)x * nb + min(b, tbv - 1);
-      v[u] = (MFA && f == NF) ? rsrc[o] : src[f * Ffs + o];
+      const double2 v0 = src[o], v1 = src[Ffs + o], v2 = src[2 * Ffs + o], v3 = src[3 * Ffs + o],
+                    v4 = src[4 * Ffs + o], v5 = src[5 * Ffs + o];
+      const double2 r = MFA ? rsrc[o] : make_double2(0.0, 0.0);
+      const unsigned bb = (unsigned)min(b0 + b, nb - 1);
+      const int wy = (int)fdiv(bb, fnzh);
+      const int wz = (int)bb - wy * g.nzh;
+      double kyp, kzp;
+      column_k(g, wy, wz, kyp, kzp);
+      const double2 h = cadd(v1, cmul_i(make_double2(kyp * v3.x + kzp * v4.x, kyp * v3.y + kzp * v4.y), 1.0));
+      // columns past the tile's end hold zeros (selected per component: a
+      // select of whole double2 values is lowered through scratch memory)
+      const bool in = b < tbv;
+      B[x * S6 + b] = make_double2(in ? v0.x : 0.0, in ? v0.y : 0.0);
+      B[x * S6 + TB + b] = make_double2(in ? h.x : 0.0, in ? h.y : 0.0);
+      B[x * S6 + 2 * TB + b] = make_double2(in ? v2.x : 0.0, in ? v2.y : 0.0);
+      B[x * S6 + 3 * TB + b] = make_double2(in ? v5.x : 0.0, in ? v5.y : 0.0);
+      if (MFA) B[x * S6 + 4 * TB + b] = make_double2(in ? r.x : 0.0, in ? r.y : 0.0);
     }
+  } else {
+    for (int i0 = threadIdx.x; i0 < NL * T; i0 += LU * blockDim.x) {
+      double2 v[LU];
 #pragma unroll
-    for (int u = 0; u < LU; ++u) {
-      const int i = i0 + u * (int)blockDim.x;
-      if (i < NL * T) {
+      for (int u = 0; u < LU; ++u) {
+        const int i = min(i0 + u * (int)blockDim.x, NL * T - 1);
         const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
         const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
-        B[x * S6 + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+        const long long o = (long long)x * nb + min(b, tbv - 1);
+        v[u] = (MFA && f == NF) ? rsrc[o] : src[f * Ffs + o];
+      }
+#pragma unroll
+      for (int u = 0; u < LU; ++u) {
+        const int i = i0 + u * (int)blockDim.x;
+        if (i < NL * T) {
+          const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
+          const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
+          B[x * S6 + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+        }
       }
     }
   }
@@ -535,8 +608,9 @@ __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
     const KW w = kweights(g, fp, x, wy, wz);
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, f, B[x * S6 + f * TB + b], 1), g.inv_npts));
-    if (MFA) acc = cadd(acc, cmul(B[x * S6 + NF * TB + b], fp.kmfa[(long long)x * nb + bb]));
+    for (int c = 0; c < NC; ++c)
+      acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, XF::field(c), B[x * S6 + c * TB + b], 1), g.inv_npts));
+    if (MFA) acc = cadd(acc, cmul(B[x * S6 + NC * TB + b], fp.kmfa[(long long)x * nb + bb]));
     B[x * S6 + b] = acc;    // this element's slots are read above by this thread only
   }
   __syncthreads();
