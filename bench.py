@@ -55,14 +55,16 @@ def chempots(n=32):
     return np.array([eos.calculate_mu(W.TEMPERATURE, p) for p in W.pressures_isotherm(n)])
 
 
-PASS_NAMES = {0: 'k_zphi', 1: 'k_xweights', 2: 'k_xback', 3: 'k_ypass(6 fields)'}
+PASS_NAMES = {0: 'k_zphi', 1: 'k_xweights', 2: 'k_xback', 3: 'k_ypass(5 fields)'}
 
 
 def pass_bytes(nx, ny, nz):
     """Algorithmic HBM bytes per launch of each functional pass (one
-    instance): every complex half-spectrum element read or written once."""
+    instance) of the six-field functional: every complex half-spectrum
+    element read or written once; five fields travel between the passes
+    (the z component of n_v2 is i kz times n3 and is formed in the z pass)."""
     nh = nx * ny * (nz // 2 + 1) * 16
-    return {0: 12 * nh, 1: 7 * nh, 2: 7 * nh, 3: 12 * nh}
+    return {0: 10 * nh, 1: 6 * nh, 2: 6 * nh, 3: 10 * nh}
 
 
 def measured_traffic(kernel, n, nbatch=1):
@@ -445,7 +447,7 @@ def main():
     points_per_s = value
     iters_per_point = float(np.mean(iters))
 
-    # dominant pass: per-launch time x launches per iteration (2 y passes of 6 fields),
+    # dominant pass: per-launch time x launches per iteration (2 y passes of 5 fields),
     # timed on the single-instance plan (solve one point first if only the batch ran)
     if plain_pps is None:
         solver.solve(mus[0], prog._set_initial_density(chempot=mus[0], temperature=W.TEMPERATURE))

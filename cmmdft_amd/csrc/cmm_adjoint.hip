@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
-  z_load_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
+  z_load_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft<1>(sm, t, g.fz, s_tw);
   const double* nb = nfld + (long long)inst * NF * npts;
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   }
   __syncthreads();
   z_fft<-1>(sm, t, g.fz, s_tw);
-  z_store_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
+  z_store_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
 }
 
 // CG direction and its transform:

@@ -494,8 +494,9 @@ int cmm_set_instances(cmm_plan* p, const double* fug, const double* alpha) {
 // --------------------------------------------------------------------------
 // launch helpers
 // --------------------------------------------------------------------------
-// y passes over nf fields (field stride nhalf, instance stride is)
-static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sgn) {
+// y passes over nf fields (field stride nhalf, instance stride is); with
+// fskip >= 0, over nf fields of the nf + 1 that skip field fskip
+static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sgn, int fskip = -1) {
   const int ntz = (p->nzh + p->tby - 1) / p->tby;
   const long long ntiles = (long long)p->nx * ntz;
   const int per = (int)((ntiles + 7) / 8);
@@ -504,16 +505,26 @@ static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sg
   const FastDiv fT = make_fastdiv((unsigned)p->tby);
   if (sgn < 0)
     hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per,
-                       p->active);
+                       p->active, fskip);
   else
     hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per,
-                       p->active);
+                       p->active, fskip);
   LAUNCHCHK("k_ypass");
   return CMM_OK;
 }
 
 static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
   return launch_ypass_is(p, F, (nf == 1 && F == p->rz) ? p->nhalf : (long long)p->fcap * p->nhalf, nf, sgn);
+}
+
+// y passes of the functional chain's fields.  Six-field functional: five of
+// them -- the z component of n_v2 (field 4) is i kz times n3 on both sides of
+// the y and x passes and is not carried (k_xweights, z_load_half_multi),
+// unless `full` asks for it (callers that read the weighted densities).
+static int launch_ypass_chain(cmm_plan* p, int sgn, bool full = false) {
+  const long long is = (long long)p->fcap * p->nhalf;
+  if (p->nf == 6 && !full) return launch_ypass_is(p, p->F, is, 5, sgn, 4);
+  return launch_ypass_is(p, p->F, is, p->nf, sgn);
 }
 
 static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
@@ -567,7 +578,7 @@ static bool xpass_full_occ(unsigned nt, size_t lds_bytes) {
 }
 
 template <int NF, bool MFA>
-static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
+static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, bool full) {
   constexpr int NC = XFields<NF>::nc;
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NC);
@@ -578,21 +589,21 @@ static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream,
                      p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
                      make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NC * tb)), make_fastdiv((unsigned)p->nzh), per,
-                     ntiles, p->active, red_of(p));
+                     ntiles, p->active, red_of(p), full ? 1 : 0);
 }
 
-static int launch_xweights(cmm_plan* p) {
+static int launch_xweights(cmm_plan* p, bool full = false) {
   const long long nb = (long long)p->ny * p->nzh;
   const int tb = xpass_tb(p, xfields(p));
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
   const bool mfa = p->fp.kmfa != nullptr;
   if (p->nf == 12) {
-    if (mfa) launch_xweights_nf<12, true>(p, tb, ntiles, per);
-    else launch_xweights_nf<12, false>(p, tb, ntiles, per);
+    if (mfa) launch_xweights_nf<12, true>(p, tb, ntiles, per, full);
+    else launch_xweights_nf<12, false>(p, tb, ntiles, per, full);
   } else {
-    if (mfa) launch_xweights_nf<6, true>(p, tb, ntiles, per);
-    else launch_xweights_nf<6, false>(p, tb, ntiles, per);
+    if (mfa) launch_xweights_nf<6, true>(p, tb, ntiles, per, full);
+    else launch_xweights_nf<6, false>(p, tb, ntiles, per, full);
   }
   LAUNCHCHK("k_xweights");
   return mfa ? launch_reduce(p, ntiles, 1, 0u, S_FMFA) : CMM_OK;
@@ -696,13 +707,13 @@ int cmm_seed(cmm_plan* p, const double* rho, const double* vext, int vb, double 
   return launch_produce<MODE_SEED>(p, el, "k_zproduce(seed)");
 }
 
-static int forward_chain(cmm_plan* p) {
+static int forward_chain(cmm_plan* p, bool full = false) {
   if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
   int rc = launch_ypass(p, p->rz, 1, -1);
   if (rc) return rc;
-  rc = launch_xweights(p);
+  rc = launch_xweights(p, full);
   if (rc) return rc;
-  return launch_ypass(p, p->F, p->nf, 1);
+  return launch_ypass_chain(p, 1, full);
 }
 
 static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
@@ -721,7 +732,7 @@ static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
 // back-convolution of the nf derivative fields in F: inverse y, the x pass
 // with the weighted sum, inverse y of the sum (left in field 0 of F)
 static int back_chain(cmm_plan* p) {
-  int rc = launch_ypass(p, p->F, p->nf, -1);
+  int rc = launch_ypass_chain(p, -1);
   if (rc) return rc;
   rc = launch_xback(p);
   if (rc) return rc;
@@ -767,7 +778,7 @@ int cmm_launch_pass(cmm_plan* p, int which) {
     case 0: return launch_zphi(p, 1, false);
     case 1: return launch_xweights(p);
     case 2: return launch_xback(p);
-    case 3: return launch_ypass(p, p->F, p->nf, -1);
+    case 3: return launch_ypass_chain(p, -1);
     case 4: return launch_ypass(p, p->rz, 1, -1);
     default: return fail(CMM_E_ARG, "unknown pass");
   }
@@ -775,7 +786,7 @@ int cmm_launch_pass(cmm_plan* p, int which) {
 
 int cmm_weighted_densities(cmm_plan* p, double* out) {
   if (!p || !out) return fail(CMM_E_ARG, "null argument");
-  int rc = forward_chain(p);
+  int rc = forward_chain(p, true);
   if (rc) return rc;
   dim3 grid((unsigned)zblocks(p, p->pz), p->nf, p->nb);
   hipLaunchKernelGGL(k_zc2r, grid, dim3(NT), zlds(p, p->pz, 1), p->stream, p->F, (long long)p->fcap * p->nhalf,

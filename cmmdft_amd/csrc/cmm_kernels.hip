@@ -114,9 +114,17 @@ __device__ __forceinline__ void z_load_half(double2* lds, const double2* src, lo
   }
 }
 
-// nf half-spectrum fields (field stride fs) into the tile in one loop
+// kz of a z-spectrum index (table kzk, Nyquist index kzn), zero on the Nyquist plane
+__device__ __forceinline__ double kz_of(const double* kzk, int kzn, int k) { return (k == kzn) ? 0.0 : kzk[k]; }
+
+// nf half-spectrum fields (field stride fs) into the tile in one loop.
+// VZ (six-field functional): field 4, the z component of n_v2, is not stored;
+// it is -i kz times field 1's (n3 = w3 rho^) spectrum, zero on the Nyquist
+// plane, formed here from field 1 per pencil.
+template <bool VZ = false>
 __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* src, long long fs, int nf, long long p0,
-                                                  const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn) {
+                                                  const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn,
+                                                  const double* kzk = nullptr, int kzn = -1) {
   const int per = t.B * nzh;
   const int tot = nf * per;
   const int ip0 = (int)p0, ilast = (int)npen - 1;
@@ -129,7 +137,7 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
       const int r = i - f * per;
       const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
       const int pa = ip0 + 2 * q;
-      const double2* sf = src + f * fs;
+      const double2* sf = src + ((VZ && f == 4) ? 1 : f) * fs;
       A[u] = sf[min(pa, ilast) * nzh + k];
       B[u] = sf[min(pa + 1, ilast) * nzh + k];
     }
@@ -143,6 +151,11 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
         const int pa = ip0 + 2 * q;
         double2 a = (pa <= ilast) ? A[u] : make_double2(0.0, 0.0);
         double2 b = (pa + 1 <= ilast) ? B[u] : make_double2(0.0, 0.0);
+        if (VZ && f == 4) {
+          const double kk = kz_of(kzk, kzn, k);
+          a = cmul_i(cscale(a, kk), -1.0);
+          b = cmul_i(cscale(b, kk), -1.0);
+        }
         const bool selfconj = (k == 0) || (2 * k == n);
         if (selfconj) { a.y = 0.0; b.y = 0.0; }
         lds[k * t.ES + f * t.B + q] = make_double2(a.x - b.y, a.y + b.x);
@@ -152,19 +165,39 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
   }
 }
 
+// the two half spectra of pair q of field f at index k of a transformed tile
+__device__ __forceinline__ void z_unpack(const double2* lds, const ZTile& t, int n, int f, int q, int k, double2& A,
+                                         double2& Bv) {
+  const double2 zk = lds[k * t.ES + f * t.B + q];
+  const double2 zm = lds[(k == 0 ? 0 : n - k) * t.ES + f * t.B + q];
+  A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+  Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+}
+
+// VZ (six-field functional): the derivative field of the z component of n_v2
+// is not stored; its back-convolution term i kz w3 G_vz joins field 1's
+// (w3 G_n3) as G_n3 + i kz G_vz, formed here per pencil.
+template <bool VZ = false>
 __device__ __forceinline__ void z_store_half_multi(const double2* lds, double2* dst, long long fs, int nf, long long p0,
-                                                   const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn) {
+                                                   const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn,
+                                                   const double* kzk = nullptr, int kzn = -1) {
   const int per = t.B * nzh;
-  const int tot = nf * per;
+  const int tot = (VZ ? nf - 1 : nf) * per;
 #pragma unroll 4
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-    const int f = (int)fdiv((unsigned)i, fBn);
-    const int r = i - f * per;
+    const int fs0 = (int)fdiv((unsigned)i, fBn);
+    const int r = i - fs0 * per;
+    const int f = (VZ && fs0 >= 4) ? fs0 + 1 : fs0;
     const int q = (int)fdiv((unsigned)r, t.fnzh), k = r - q * nzh;
-    const double2 zk = lds[k * t.ES + f * t.B + q];
-    const double2 zm = lds[(k == 0 ? 0 : n - k) * t.ES + f * t.B + q];
-    const double2 A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-    const double2 Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+    double2 A, Bv;
+    z_unpack(lds, t, n, f, q, k, A, Bv);
+    if (VZ && f == 1) {
+      double2 A4, B4;
+      z_unpack(lds, t, n, 4, q, k, A4, B4);
+      const double kk = kz_of(kzk, kzn, k);
+      A = cadd(A, cmul_i(cscale(A4, kk), 1.0));
+      Bv = cadd(Bv, cmul_i(cscale(B4, kk), 1.0));
+    }
     const int pa = (int)p0 + 2 * q, o = pa * nzh + k;
     double2* df = dst + f * fs;
     if (pa < (int)npen) df[o] = A;
@@ -231,9 +264,12 @@ __device__ __forceinline__ long long xcd_tile(int per) {
 
 // y pass: one block per tile of TB consecutive kz columns of one x plane (all
 // y) of one field; tiles t = x * ntz + kz-tile.
+// fskip >= 0: that field is not stored (the z component of n_v2 of a
+// six-field functional) and the grid's y index skips it.
 template <int SGN>
 __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
-                                             FastDiv fTB, int ntz, int per, const int* __restrict__ active) {
+                                             FastDiv fTB, int ntz, int per, const int* __restrict__ active,
+                                             int fskip) {
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fy);
@@ -244,7 +280,8 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   const int x = (int)(t / ntz);
   const int k0 = (int)(t - (long long)x * ntz) * TB;
   const int tbv = min(TB, g.nzh - k0);
-  double2* base = F + inst * is + blockIdx.y * fs + (long long)x * g.ny * g.nzh + k0;
+  const int fld = (fskip >= 0 && (int)blockIdx.y >= fskip) ? (int)blockIdx.y + 1 : (int)blockIdx.y;
+  double2* base = F + inst * is + fld * fs + (long long)x * g.ny * g.nzh + k0;
   const int tot = g.ny * TB;
   for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
     double2 v[LU];
@@ -400,7 +437,10 @@ constexpr int NTX = 512;
 // column of the x pass ky and kz are constant, so the y and z components of
 // n_v2, i ky w3 rho^ and i kz w3 rho^ (zero on their axis' Nyquist plane),
 // are i ky' and i kz' times the n3 column before and after the x transform,
-// and in the back-convolution i ky' G_y + i kz' G_z joins G_3 before it.
+// and in the back-convolution i ky' G_vy joins G_n3 before it (the z pass has
+// folded i kz' G_vz into G_n3 already: kz is constant along y and x too, so
+// the z component is not stored between the passes at all, see
+// z_load_half_multi; vz = 1 stores it for callers that read the fields).
 // Slots 0..3 hold the fields n2, n3, nv2_x, rho_bar (weights 0, 1, 2, 5).
 template <int NF>
 struct XFields {
@@ -427,7 +467,7 @@ template <int NF, bool MFA, int OCC>
 __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F, long long Fs, long long Ffs,
            GridDev g, FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
-           const int* __restrict__ active, RedDev red) {
+           const int* __restrict__ active, RedDev red, int vz) {
   using XF = XFields<NF>;
   constexpr int NC = XF::nc;
   extern __shared__ double2 sm[];    // [NC T] fields, then the nx twiddles
@@ -509,7 +549,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       dst[Ffs + o] = n3;
       dst[2 * Ffs + o] = B[x * S6 + 2 * TB + b];
       dst[3 * Ffs + o] = cmul_i(cscale(n3, kyp), -1.0);
-      dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
+      if (vz) dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
       dst[5 * Ffs + o] = B[x * S6 + 3 * TB + b];
     }
   } else {
@@ -553,20 +593,21 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
   const double2* src = F + inst * Fs + b0;
   const double2* rsrc = MFA ? rz + inst * rzs + b0 : nullptr;
   if constexpr (XF::red) {
-    // one element's six fields (and rho^) per thread, loads in flight
-    // together; G_3 + i ky' G_y + i kz' G_z formed before the transform
+    // one element's five fields (and rho^) per thread, loads in flight
+    // together; G_n3 + i ky' G_vy formed before the transform (the z pass
+    // has already folded i kz' G_vz into G_n3)
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
       const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
       const long long o = (long long)x * nb + min(b, tbv - 1);
       const double2 v0 = src[o], v1 = src[Ffs + o], v2 = src[2 * Ffs + o], v3 = src[3 * Ffs + o],
-                    v4 = src[4 * Ffs + o], v5 = src[5 * Ffs + o];
+                    v5 = src[5 * Ffs + o];
       const double2 r = MFA ? rsrc[o] : make_double2(0.0, 0.0);
       const unsigned bb = (unsigned)min(b0 + b, nb - 1);
       const int wy = (int)fdiv(bb, fnzh);
       const int wz = (int)bb - wy * g.nzh;
       double kyp, kzp;
       column_k(g, wy, wz, kyp, kzp);
-      const double2 h = cadd(v1, cmul_i(make_double2(kyp * v3.x + kzp * v4.x, kyp * v3.y + kzp * v4.y), 1.0));
+      const double2 h = cadd(v1, cmul_i(cscale(v3, kyp), 1.0));
       // columns past the tile's end hold zeros (selected per component: a
       // select of whole double2 values is lowered through scratch memory)
       const bool in = b < tbv;
@@ -921,7 +962,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
-  z_load_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
+  z_load_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft<1>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
@@ -946,7 +987,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   if (store) {
     __syncthreads();
     z_fft<-1>(sm, t, g.fz, s_tw);
-    z_store_half_multi(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn);
+    z_store_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
