@@ -61,11 +61,9 @@ class EquationOfState:
 
     def _bracket_solve(self, target, f_grid, fun, grid):
         sols = []
-        lo = np.minimum(f_grid[:-1], f_grid[1:])
         inside = np.nonzero((f_grid[:-1] <= target) & (target <= f_grid[1:]))[0]
         for j in inside:
             sols.append(brentq(fun, grid[j], grid[j + 1]))
-        del lo
         return sorted(sols)
 
     def solve_densities_from_pressures(self, pressures, n_rough_gridpoints=10000):

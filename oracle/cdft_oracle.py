@@ -28,8 +28,13 @@ live in ``tests/test_oracle_*.py``):
                           against a real-space fold, the phase-displaced
                           kernel against a direct sum on a tiny grid.
 * WDA / EOS ............ bulk limit (weighted density == density), EOS
-                          derivative by finite differences, MBWR coefficient
-                          table literal in the reference.
+                          derivative by finite differences.  MBWR: the 32
+                          coefficients equal the reference's literals
+                          (fixture with checksum); the equation's critical
+                          point is the published T*c = 1.313, rho*c = 0.310,
+                          P*c = 0.13; its low-density limit is the LJ second
+                          virial coefficient by quadrature; a planted sign
+                          flip or swap of any coefficient fails these.
 * LJ external potential  single-atom analytic values, brute-force image sum.
 * Orientational average  Lebedev tables = the reference's (fixture) and exact
                           for spherical monomials up to their degree; rotation
@@ -695,6 +700,66 @@ def external_potential(points, atom_sigma, atom_epsilon, pos, rvecs, sigma_f, ep
         R = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2 + 1e-16)
         V += lennard_jones(R, sm, em, cutoff=cutoff)
     return V
+
+
+def lattice_images(rvecs, cutoff, extent):
+    """Integer lattice translations n (rows) that can bring an atom within
+    ``cutoff`` of a point when atoms and points both lie within ``extent``
+    of the origin: |n . rvecs| <= 2 extent + cutoff is necessary, so every
+    n with |n_a| <= (2 extent + cutoff) / d_a (d_a the spacing of the lattice
+    planes normal to reciprocal axis a) is enumerated."""
+    R = np.asarray(rvecs, dtype=np.float64)
+    vol = abs(np.linalg.det(R))
+    d = [vol / np.linalg.norm(np.cross(R[(a + 1) % 3], R[(a + 2) % 3])) for a in range(3)]
+    K = [int(np.ceil((2 * extent + cutoff) / d[a])) for a in range(3)]
+    rng = [np.arange(-k, k + 1) for k in K]
+    return np.array(np.meshgrid(*rng, indexing='ij')).reshape(3, -1).T
+
+
+def external_potential_images(points, atom_sigma, atom_epsilon, pos, rvecs, sigma_f, epsilon_f,
+                              cutoff=12 * U.angstrom):
+    """LJ sum over every periodic image of every host atom within the cut-off,
+    a brute-force restatement of what the reference's minimum-image sum
+    (extpot_calculator.py:186-222) approximates and its periodic-shift
+    version (extpot_calculator.py:224-225, unimplemented there) was meant to
+    give: V(p) = sum_i sum_n LJ(|p - x_i - n . rvecs|) over R < cutoff with
+    R = sqrt(r^2 + 1e-16), Lorentz-Berthelot mixing and the cut-off shift as
+    in ``lennard_jones``.  ``points`` is an (nx, ny, nz, 3) grid of an
+    orthorhombic cell (each image is then evaluated on the index block of
+    the grid inside its cut-off cube only) or any (M, 3) array (all points
+    for every image)."""
+    pts = np.asarray(points, dtype=np.float64)
+    R = np.asarray(rvecs, dtype=np.float64)
+    pos = np.asarray(pos, dtype=np.float64)
+    extent = max(np.max(np.linalg.norm(pts.reshape(-1, 3), axis=1)), np.max(np.linalg.norm(pos, axis=1)))
+    imgs = lattice_images(R, cutoff, extent) @ R
+    grid = pts.ndim == 4 and np.allclose(R, np.diag(np.diag(R)))
+    if grid:
+        ax = [pts[:, 0, 0, 0], pts[0, :, 0, 1], pts[0, 0, :, 2]]
+        V = np.zeros(pts.shape[:3])
+    else:
+        P = pts.reshape(-1, 3)
+        V = np.zeros(len(P))
+    for i in range(len(pos)):
+        sm = 0.5 * (atom_sigma[i] + sigma_f)
+        em = np.sqrt(atom_epsilon[i] * epsilon_f)
+        for t in imgs:
+            q = pos[i] + t
+            if grid:
+                lo = [np.searchsorted(ax[a], q[a] - cutoff, side='left') for a in range(3)]
+                hi = [np.searchsorted(ax[a], q[a] + cutoff, side='right') for a in range(3)]
+                if any(h <= l for l, h in zip(lo, hi)):
+                    continue
+                dx = (ax[0][lo[0]:hi[0]] - q[0])[:, None, None]
+                dy = (ax[1][lo[1]:hi[1]] - q[1])[None, :, None]
+                dz = (ax[2][lo[2]:hi[2]] - q[2])[None, None, :]
+                r = np.sqrt(dx ** 2 + dy ** 2 + dz ** 2 + 1e-16)
+                V[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += lennard_jones(r, sm, em, cutoff=cutoff)
+            else:
+                d = P - q
+                r = np.sqrt(np.sum(d * d, axis=1) + 1e-16)
+                V += lennard_jones(r, sm, em, cutoff=cutoff)
+    return V.reshape(-1)
 
 
 # ---------------------------------------------------------------------------

@@ -508,3 +508,49 @@ def test_mfa_convolution_carries_one_phase_factor():
         assert abs(c[x] - ref) < 1e-13 * g.dr, (x, c[x], ref)
     # the displacement is real: the response is not at the potential's offset
     assert np.argmax(np.abs(c)) != np.ravel_multi_index(shifts, shape)
+
+
+def _image_host(L_ang, natom, seed):
+    rng = np.random.default_rng(seed)
+    L = np.asarray(L_ang) * U.angstrom
+    pos = (rng.random((natom, 3)) - 0.5) * L
+    sig = rng.uniform(3.0, 3.5, natom) * U.angstrom
+    eps = rng.uniform(20.0, 60.0, natom) * U.boltzmann
+    return np.diag(L), pos, sig, eps
+
+
+def test_image_sum_equals_minimum_image_when_edges_exceed_twice_the_cutoff():
+    """Every cell edge >= 2 r_c: at most one image of an atom is inside the
+    cut-off sphere of a point, the minimum one."""
+    rv, pos, sig, eps = _image_host([25.6, 26.0, 24.5], 12, 7)
+    g = O.Grid(rv, (10, 12, 9))
+    Vi = O.external_potential_images(g.points[..., :3], sig, eps, pos, rv, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    Vm = O.external_potential(g.points[..., :3], sig, eps, pos, rv, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    assert np.abs(Vi - Vm).max() <= 1e-13 * np.abs(Vm).max()
+
+
+@pytest.mark.parametrize('edges', [[9.0, 10.5, 11.2], [13.0, 17.5, 30.0]])
+def test_image_sum_equals_supercell_minimum_image(edges):
+    """Edges shorter than 2 r_c: the sum over all images equals the
+    minimum-image sum over the atoms replicated into a k x k x k supercell
+    whose half edges exceed r_c (each image within the cut-off is then the
+    minimum image of exactly one replica).  Grid-block and point-list forms
+    agree."""
+    rv, pos, sig, eps = _image_host(edges, 5, 11)
+    g = O.Grid(rv, (9, 10, 11))
+    pts = g.points[..., :3]
+    Vi = O.external_potential_images(pts, sig, eps, pos, rv, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    Vp = O.external_potential_images(pts.reshape(-1, 3), sig, eps, pos, rv, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    k = int(np.ceil(2 * 12.0 / min(edges)))
+    k += (k + 1) % 2     # odd, so the replicas are centred on the cell
+    sh = np.array(np.meshgrid(*[np.arange(k) - k // 2] * 3, indexing='ij')).reshape(3, -1).T @ rv
+    rep = (pos[None, :, :] + sh[:, None, :]).reshape(-1, 3)
+    Vs = O.external_potential(pts, np.tile(sig, len(sh)), np.tile(eps, len(sh)), rep, k * rv, W.GUEST_SIGMA,
+                              W.GUEST_EPSILON)
+    # pointwise, relative to the value or to 1 kJ/mol where the sum is small
+    scale = np.maximum(np.abs(Vs), U.kjmol)
+    assert np.max(np.abs(Vi - Vs) / scale) <= 1e-13
+    assert np.max(np.abs(Vp - Vs) / scale) <= 1e-13
+    # the short cell sees more than the minimum image
+    Vm = O.external_potential(pts, sig, eps, pos, rv, W.GUEST_SIGMA, W.GUEST_EPSILON)
+    assert np.abs(Vi - Vm).max() > 0.05 * U.kjmol
