@@ -23,6 +23,7 @@ S_PHI, S_FEX, S_N3MAX = 0, 1, 2
 S_NCUR, S_FID, S_EXT = 3, 4, 5
 S_PIC_NNEW, S_PIC_IUE, S_PIC_DMAX, S_PIC_NONFIN = 6, 7, 8, 9
 S_AND_NONFIN, S_GRAM = 6, 7
+S_CHK_NONFIN, S_CHK_RES2, S_CHK_DER2, S_CHK_DERN2 = 6, 7, 8, 9
 S_PN, S_PIUE, S_PDMAX, S_PNONFIN, S_PFID, S_PEXT = 16, 17, 18, 19, 20, 21
 S_ADJ_PQ, S_ADJ_RR, S_ADJ_LAM = 24, 25, 27
 S_FMFA = 28
@@ -69,7 +70,13 @@ EXPORTS = {
                            C.c_double, C.c_double], C.c_int),
     'cmm_finish_anderson': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int,
                              C.c_void_p, C.c_double, C.c_double], C.c_int),
+    'cmm_finish_check': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                          C.c_double], C.c_int),
+    'cmm_boltzmann': ([C.c_void_p, C.c_longlong, C.c_double, C.c_double, C.c_void_p, C.c_void_p], C.c_int),
     'cmm_anderson_init': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_anderson_store_row': ([C.c_void_p, C.c_int, C.c_int], C.c_int),
+    'cmm_local_density': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_hs_weight_table': ([C.c_void_p, C.c_double, C.c_void_p], C.c_int),
     'cmm_anderson_coef': ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
                            C.c_double, C.c_double, C.c_double], C.c_int),
     'cmm_anderson_mix': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
@@ -284,6 +291,14 @@ class Plan:
             raise ValueError('weighted_densities: out must hold nbatch x %d x npts values' % self.nfields)
         check(self.lib.cmm_weighted_densities(self.h, _ptr(out)))
 
+    def local_density(self, nfld=None, rbar=None, phi=None, fex=None, dfex=None):
+        check(self.lib.cmm_local_density(self.h, _ptr(nfld), _ptr(rbar), _ptr(phi), _ptr(fex), _ptr(dfex)))
+
+    def hs_weight_table(self, R, out):
+        if out.numel() != 2 * int(np.prod(self.npoints)):
+            raise ValueError('hs_weight_table: out must hold 2 x npts values')
+        check(self.lib.cmm_hs_weight_table(self.h, float(R), _ptr(out)))
+
     def finish_c1(self, rho, vext, vext_batched, c1, lam3=1.0):
         check(self.lib.cmm_finish_c1(self.h, _ptr(rho), _ptr(vext), int(vext_batched), _ptr(c1), float(lam3)))
 
@@ -294,6 +309,13 @@ class Plan:
     def finish_anderson(self, rho, vext, vext_batched, rhist, m, slot_new, c1_out, beta, lam3):
         check(self.lib.cmm_finish_anderson(self.h, _ptr(rho), _ptr(vext), int(vext_batched), _ptr(rhist), int(m),
                                            int(slot_new), _ptr(c1_out), float(beta), float(lam3)))
+
+    def finish_check(self, rho, vext, vext_batched, beta, lam3, a_tol=1e-6, r_tol=1e-4):
+        check(self.lib.cmm_finish_check(self.h, _ptr(rho), _ptr(vext), int(vext_batched), float(beta), float(lam3),
+                                        float(a_tol), float(r_tol)))
+
+    def anderson_store_row(self, m, slot_new):
+        check(self.lib.cmm_anderson_store_row(self.h, int(m), int(slot_new)))
 
     def anderson_init(self, damping, cf):
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(damping, dtype=np.float64), (self.nbatch,)))
@@ -378,6 +400,15 @@ class Plan:
 
     def synchronize(self):
         check(self.lib.cmm_synchronize(self.h))
+
+
+def boltzmann(c1, beta, fug, out, stream=None):
+    """out = beta exp(-beta c1) fug on the device (cmm_boltzmann)."""
+    torch = _torch()
+    lib = load()
+    s = stream if stream is not None else torch.cuda.current_stream()
+    check(lib.cmm_boltzmann(_ptr(c1), int(c1.numel()), float(beta), float(fug), _ptr(out),
+                            C.c_void_p(s.cuda_stream)))
 
 
 def lj_grid(npoints, rvecs, atoms_dev, natom, sigma_f, epsilon_f, cutoff, images, limit, out, stream=None):
@@ -537,7 +568,7 @@ def fold(field, reps, stream=None):
     return out
 
 
-FIELD_OPS = {'sum': 0, 'boltzmann': 1, 'boltzmann_clip': 2, 'min': 3, 'max': 4, 'sumsq': 5, 'fid': 6}
+FIELD_OPS = {'sum': 0, 'boltzmann': 1, 'boltzmann_clip': 2, 'min': 3, 'max': 4, 'sumsq': 5, 'fid': 6, 'fid_track': 7}
 
 
 def field_reduce(x, op, beta=1.0, lam3=1.0, mask=None, stream=None):

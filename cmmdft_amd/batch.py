@@ -39,7 +39,7 @@ from . import units as U
 from ._device import torch_mod, as_device
 from . import _lib as L
 from .functionals import configure_plan, device_parts
-from .solver import Anderson
+from .solver import Anderson, NoSolutionError
 
 __all__ = ['BatchedAnderson', 'UnevenBatch', 'footprint_bytes']
 
@@ -86,6 +86,8 @@ class BatchedAnderson:
         self.chunk = int(chunk)
         self._groups = None
         self._graph = None
+        self._graph_key = None
+        self._graph_V = None
 
     # ------------------------------------------------------------------
     def _setup(self):
@@ -120,6 +122,17 @@ class BatchedAnderson:
             g['mu'] = mu
             configure_plan(g['plan'], hs, wda, mfa, 1.0 / self.beta)
             g['plan'].set_instances(np.exp(self.beta * mu) / self.beta / self.lam3, np.zeros(g['n']))
+        # A captured graph holds by value what its launches were given: the
+        # external-potential pointer, beta, Lambda^3, the functional and the
+        # mean-field table pointer.  Re-capture when any of them changed since
+        # (a new temperature, another potential array, other parts).
+        p0 = self._groups[0]['plan']
+        key = (0 if self.V is None else self.V.data_ptr(), self.beta, self.lam3, bytes(p0._func),
+               0 if p0._mfa is None else p0._mfa.data_ptr())
+        if self._graph is not None and key != self._graph_key:
+            p0.graph_destroy(self._graph)
+            self._graph = None
+        self._graph_key = key
 
     def _iteration(self, s):
         """Enqueue batched Anderson step s (1-based) for every group."""
@@ -185,6 +198,8 @@ class BatchedAnderson:
                         self._iteration(m + 1 + k)
                 finally:
                     self._graph = p0.graph_end()
+                # the graph reads this potential array: keep it alive with the graph
+                self._graph_V = self.V
         return self
 
     def running(self):
@@ -213,9 +228,11 @@ class BatchedAnderson:
         return self.running()
 
     def finish(self):
-        """Gather the converged densities and scalars; re-solve fallback
-        instances with the plain solver.  Returns dict(N, rho (B, nx, ny, nz)
-        device tensor, converged, iters, status, omega, resolved)."""
+        """Gather the converged densities and scalars; re-solve fallback,
+        step-limit and non-finite instances with the plain solver (an instance
+        whose re-solve raises NoSolutionError keeps its status, with N and rho
+        NaN).  Returns dict(N, rho (B, nx, ny, nz) device tensor, converged,
+        iters, status, omega, resolved)."""
         torch = torch_mod()
         B = len(self.chempots)
         st = self._st
@@ -240,7 +257,15 @@ class BatchedAnderson:
             mu = self.chempots[i]
             r0 = torch.exp(self.beta * (mu - self._Vd.reshape(self.grid.npoints))) / self.lam3 \
                 if self._rho0 is None else as_device(self._rho0)[i]
-            Ni, ri, ci = solver.solve(mu, r0)
+            try:
+                Ni, ri, ci = solver.solve(mu, r0)
+            except NoSolutionError:
+                # the plain solver fails too (solver.py:335-345): the state
+                # point keeps its status, without a density
+                rho[i].fill_(float('nan'))
+                N[i] = float('nan')
+                iters[i] = solver.curr_step
+                continue
             rho[i].copy_(ri.reshape(self.grid.npoints))
             N[i] = Ni
             iters[i] = solver.curr_step
@@ -265,6 +290,7 @@ class BatchedAnderson:
         if self._graph is not None and self._groups:
             self._groups[0]['plan'].graph_destroy(self._graph)
         self._graph = None
+        self._graph_V = None
 
 
 class UnevenBatch:

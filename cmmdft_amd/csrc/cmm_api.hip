@@ -472,7 +472,7 @@ int cmm_field_reduce(const double* x, const unsigned char* mask, long long n, in
                      double* work, double* out, void* stream) {
   if (!x || !work || !out) return fail(CMM_E_ARG, "null argument");
   if (n < 1) return fail(CMM_E_ARG, "empty field");
-  if (op < FOP_SUM || op > FOP_FID) return fail(CMM_E_ARG, "unknown reduction");
+  if (op < FOP_SUM || op > FOP_FID_TRACK) return fail(CMM_E_ARG, "unknown reduction");
   const long long want = (n + RED_NT - 1) / RED_NT;
   const int nb = (int)std::min<long long>(RED_BLOCKS, want);
   hipLaunchKernelGGL(k_field_reduce, dim3(nb), dim3(RED_NT), 0, (hipStream_t)stream, x, mask, n, op, beta, lam3, work);
@@ -806,6 +806,37 @@ static int launch_final(cmm_plan* p, const ElDev& el, const char* name) {
   return launch_reduce(p, grid.x, NQ, MODE == MODE_PICARD ? (1u << 5) : 0u, S_NCUR);
 }
 
+int cmm_local_density(cmm_plan* p, const double* nfld, const double* rbar, double* phi, double* fex, double* dfex) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (!p->has_func) return fail(CMM_E_ARG, "functional not set");
+  if (nfld && phi && !p->fp.use_hs) return fail(CMM_E_ARG, "the functional has no hard-sphere part");
+  if (rbar && (fex || dfex) && !p->fp.use_wda) return fail(CMM_E_ARG, "the functional has no WDA part");
+  const long long n = p->npts;
+  const unsigned nb = (unsigned)std::min<long long>((n + NT - 1) / NT, 65536);
+  for (int b = 0; b < p->nb; ++b) {
+    const double* nf = nfld ? nfld + (long long)b * p->nf * n : nullptr;
+    const double* rb = rbar ? rbar + (long long)b * n : nullptr;
+    double* ph = phi ? phi + (long long)b * n : nullptr;
+    double* f = fex ? fex + (long long)b * n : nullptr;
+    double* df = dfex ? dfex + (long long)b * n : nullptr;
+    if (p->nf == 12)
+      hipLaunchKernelGGL(k_local<12>, dim3(nb), dim3(NT), 0, p->stream, nf, n, rb, n, p->fp, ph, f, df);
+    else
+      hipLaunchKernelGGL(k_local<6>, dim3(nb), dim3(NT), 0, p->stream, nf, n, rb, n, p->fp, ph, f, df);
+    LAUNCHCHK("k_local");
+  }
+  return CMM_OK;
+}
+
+int cmm_hs_weight_table(cmm_plan* p, double R, double* out) {
+  if (!p || !out) return fail(CMM_E_ARG, "null argument");
+  if (!(R > 0.0)) return fail(CMM_E_ARG, "hard-sphere radius must be positive");
+  const unsigned nb = (unsigned)std::min<long long>((p->npts + NT - 1) / NT, 65536);
+  hipLaunchKernelGGL(k_weight_table, dim3(nb), dim3(NT), 0, p->stream, p->g, R, out);
+  LAUNCHCHK("k_weight_table");
+  return CMM_OK;
+}
+
 int cmm_finish_c1(cmm_plan* p, const double* rho, const double* vext, int vb, double* c1, double lam3) {
   if (!p || !rho || !c1) return fail(CMM_E_ARG, "null argument");
   ElDev el = el_base(p, rho, vext, vb, lam3);
@@ -836,6 +867,43 @@ int cmm_finish_anderson(cmm_plan* p, const double* rho, const double* vext, int 
   el.c1out = c1_out;
   el.beta = beta;
   return launch_final<MODE_ANDRES>(p, el, "k_zfinal(anderson)");
+}
+
+int cmm_finish_check(cmm_plan* p, const double* rho, const double* vext, int vb, double beta, double lam3,
+                     double a_tol, double r_tol) {
+  if (!p || !rho) return fail(CMM_E_ARG, "null argument");
+  if (!(beta > 0.0) || !(lam3 > 0.0)) return fail(CMM_E_ARG, "beta and lam3 must be positive");
+  ElDev el = el_base(p, rho, vext, vb, lam3);
+  el.beta = beta;
+  el.a_tol = a_tol;
+  el.r_tol = r_tol;
+  return launch_final<MODE_CHECK>(p, el, "k_zfinal(check)");
+}
+
+int cmm_boltzmann(const double* c1, long long n, double beta, double fug, double* out, void* stream) {
+  if (!c1 || !out || n < 0) return fail(CMM_E_ARG, "null argument");
+  if (n == 0) return CMM_OK;
+  const long long want = (n + NT - 1) / NT;
+  const unsigned nb = (unsigned)std::min<long long>(want, 65536);
+  hipLaunchKernelGGL(k_boltzmann, dim3(nb), dim3(NT), 0, (hipStream_t)stream, c1, n, beta, fug, out);
+  LAUNCHCHK("k_boltzmann");
+  return CMM_OK;
+}
+
+int cmm_anderson_store_row(cmm_plan* p, int m, int slot_new) {
+  if (!p) return fail(CMM_E_ARG, "null plan");
+  if (m < 1 || m > CMM_MMAX || slot_new < 0 || slot_new >= m) return fail(CMM_E_ARG, "bad history slot");
+  AndDev a;
+  std::memset(&a, 0, sizeof(a));
+  a.gram = p->gram;
+  a.m = m;
+  a.mk = 1;
+  a.slot_new = slot_new;
+  a.slot_prev = -1;
+  a.store_only = 1;
+  hipLaunchKernelGGL(k_anderson_coef, dim3(p->nb), dim3(64), 0, p->stream, a, p->active, red_of(p));
+  LAUNCHCHK("k_anderson_coef(store)");
+  return CMM_OK;
 }
 
 int cmm_anderson_init(cmm_plan* p, const double* damping, const double* cf) {

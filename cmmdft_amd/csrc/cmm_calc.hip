@@ -18,6 +18,7 @@ enum FieldOp {
   FOP_MAX = 4,
   FOP_SUMSQ = 5,      // x^2
   FOP_FID = 6,        // c (ln(lam3 c) - 1) with c = max(x, 1e-20): ideal-gas free energy density / kT
+  FOP_FID_TRACK = 7,  // the same with c = 1e-30 where x <= 1e-8 (FreeEnergy.track, functionals.py:151-155)
 };
 
 __device__ __forceinline__ double fop_value(int op, double x, double beta, double lam3) {
@@ -28,6 +29,10 @@ __device__ __forceinline__ double fop_value(int op, double x, double beta, doubl
     case FOP_FID: {
       const double c = fmax(x, 1e-20);
       return c * (log(c * lam3) - 1.0);
+    }
+    case FOP_FID_TRACK: {
+      const double c = (x <= 1e-8) ? 1e-30 : x;
+      return c * (log(lam3 * c) - 1.0);
     }
     default: return x;
   }

@@ -7,7 +7,7 @@
 
 namespace cmm {
 
-enum { MODE_C1 = 0, MODE_PICARD = 1, MODE_ANDRES = 2 };
+enum { MODE_C1 = 0, MODE_PICARD = 1, MODE_ANDRES = 2, MODE_CHECK = 3 };
 enum { MODE_MIX = 0, MODE_TRIAL = 1, MODE_SEED = 2 };
 
 // Per-instance reduction results (doubles, on the device), NSCAL per instance.
@@ -16,6 +16,8 @@ constexpr int S_PHI = 0;    // sum Phi, sum f_ex(rho_bar), max n3            (fu
 constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (final pass)
                             //   picard: N', sum|d|, max|d|, nonfinite
                             //   anderson: nonfinite, Gram row[CMM_MMAX]
+                            //   check: nonfinite, |G - rho|^2, DER sum, DER sum
+                            //          without the tolerance weights
 constexpr int S_GRAM = S_NCUR + 4;
 constexpr int S_FMFA = 28;  // (1/2) sum rho c_MFA (mean-field energy / dr)   (x weights pass)
 constexpr int S_PN = 16;    // N', sum|d|, max|d|, nonfinite, ideal', ext'  (density producers)
@@ -87,6 +89,7 @@ struct ElDev {
   int order[CMM_MMAX];
   int order_rho[CMM_MMAX];
   int clip_trial;
+  double a_tol, r_tol;   // MODE_CHECK: tolerances of the derivative criterion
 };
 
 struct AndDev {
@@ -100,6 +103,7 @@ struct AndDev {
   double fac_up, fac_down, dmin, dmax;
   double* coef;
   double* alpha_out;
+  int store_only;   // store this step's Gram row into the ring and stop
 };
 
 // Batched solve bookkeeping (one entry per instance).

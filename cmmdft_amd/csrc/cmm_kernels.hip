@@ -994,12 +994,76 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   grid_reduce_store<3>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
+// Pointwise free-energy densities of given weighted densities: kT Phi from
+// the NF fields nfld[nf][n] of cmm_weighted_densities (n2, n3 unclipped, nv2
+// x/y/z, rho_bar, then the tensor components), and the WDA terms f_ex(rb),
+// f_ex'(rb) of a weighted density rb[n].  The local values of
+// HardSphereFunctional.value and WDAVFunctional.value / derive with a given
+// weighted density (functionals.py:611-618, 1160-1184).
+template <int NF>
+__global__ void __launch_bounds__(NT) k_local(const double* __restrict__ nfld, long long fs, const double* __restrict__ rb,
+                                             long long n, FuncDev fp, double* __restrict__ phi, double* __restrict__ f,
+                                             double* __restrict__ df) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (nfld && phi) {
+      double r[NF];
+#pragma unroll
+      for (int c = 0; c < NF; ++c) r[c] = nfld[c * fs + i];
+      PhiOut<double> o;
+      if constexpr (NF == 12) o = fmt_point<double, true>(fp, r[0], r[1], r[2], r[3], r[4], &r[6]);
+      else o = fmt_point<double>(fp, r[0], r[1], r[2], r[3], r[4]);
+      phi[i] = fp.kT * o.phi;
+    }
+    if (rb && (f || df)) {
+      double fv, dfv;
+      wda_point<double>(fp, rb[i], fv, dfv);
+      if (f) f[i] = fv;
+      if (df) df[i] = dfv;
+    }
+  }
+}
+
+// The reference's k-space hard-sphere weights on the full grid
+// (HardSphereFunctional._init_weight_functions, functionals.py:421-455):
+// out[0] = w0 = sinc(kR) sigma_L (sigma_L at k R <= 1e-8) and
+// out[1] = 3 (sin kR - kR cos kR) / (kR)^3 sigma_L (sigma_L there); w1, w2,
+// w3 and the vector weights are multiples of these.
+__global__ void __launch_bounds__(NT) k_weight_table(GridDev g, double R, double* __restrict__ out) {
+  const long long n = (long long)g.nx * g.ny * g.nz;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int iz = (int)(i % g.nz);
+    const long long r = i / g.nz;
+    const int iy = (int)(r % g.ny), ix = (int)(r / g.ny);
+    const double kx = g.kx.k[ix], ky = g.ky.k[iy], kz = g.kz.k[iz];
+    const double sl = g.kx.lanczos[ix] * g.ky.lanczos[iy] * g.kz.lanczos[iz];
+    const double w = sqrt(kx * kx + ky * ky + kz * kz) * R;
+    if (w <= 1e-8) {
+      out[i] = sl;
+      out[n + i] = sl;
+    } else {
+      double sw, cw;
+      sincos(w, &sw, &cw);
+      out[i] = sw / w * sl;
+      out[n + i] = j_ball(w, sw, cw) * sl;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Final z pass: C2R of the back-convolved sum -> C1 = c_ex + V, then one of
 //   MODE_C1       write C1
 //   MODE_PICARD   rho' = (1-a) rho + a G, G = beta exp(-beta C1) z ;
 //                 zero below the threshold; write rho', R2C(rho') -> rz
 //   MODE_ANDRES   r = G - rho into the residual ring + Gram row
+//   MODE_CHECK    nothing written; |G - rho|^2 and the derivative criterion
+//                 of Solver._check_convergence (solver.py:131-135, 201-205):
+//                 with beta dOmega = ln(Lambda^3 rho_c) + beta C1 - beta mu
+//                 (rho_c: rho < 1e-12 -> 1e-30) and over |rho| > 1e-8 (the
+//                 complement of isclose(rho, 0)), the sums of
+//                 (|rho| beta dOmega / (a_tol + r_tol |rho|))^2 and of
+//                 (|rho| beta dOmega)^2 (the solver history's variant)
 // Always reduces (for the density rho entering this pass) N, the ideal-gas
 // sum and the external-potential sum.
 // ---------------------------------------------------------------------------
@@ -1021,6 +1085,8 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
   const double* rho = el.rho + inst * el.fstride;
   const double* V = el.vext ? el.vext + (el.vext_batched ? inst * el.fstride : 0) : nullptr;
   const InstDev ip = el.inst[inst];
+  // beta mu from the fugacity exp(beta mu) / (beta Lambda^3)
+  const double bmu = (MODE == MODE_CHECK) ? log(ip.fug * el.beta * el.lam3) : 0.0;
   constexpr int NQ = (MODE == MODE_ANDRES) ? (4 + CMM_MMAX) : 8;
   double acc[NQ];
 #pragma unroll
@@ -1060,11 +1126,24 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
       const double C1 = cv + vv[u];
       // ideal-gas and external terms of the grand potential of rho
       const double rc = (r[u] < 1e-12) ? 1e-30 : r[u];
+      const double lg = log(el.lam3 * rc);
       acc[0] += r[u];
-      acc[1] += rc * (log(el.lam3 * rc) - 1.0);
+      acc[1] += rc * (lg - 1.0);
       acc[2] += r[u] * vv[u];
       if (MODE == MODE_C1) {
         el.c1out[inst * el.fstride + idx[u]] = C1;
+      } else if (MODE == MODE_CHECK) {
+        const double G = el.beta * exp(-el.beta * C1) * ip.fug;
+        const double d = G - r[u];
+        acc[3] += isfinite(d) ? 0.0 : 1.0;
+        acc[4] += d * d;
+        const double ar = fabs(r[u]);
+        if (ar > 1e-8) {
+          const double t = ar * (lg + el.beta * C1 - bmu);
+          const double q = t / (el.a_tol + el.r_tol * ar);
+          acc[5] += q * q;
+          acc[6] += t * t;
+        }
       } else {
         if (el.c1out) el.c1out[inst * el.fstride + idx[u]] = C1;
         const double G = el.beta * exp(-el.beta * C1) * ip.fug;
@@ -1260,6 +1339,15 @@ __global__ void __launch_bounds__(NT) k_reduce_partials(RedDev red, int nblocks,
   }
 }
 
+// G = beta exp(-beta C1) z, the density update of Solver.get_new_rho
+// (solver.py:128-129), elementwise over n values.
+__global__ void __launch_bounds__(NT) k_boltzmann(const double* __restrict__ c1, long long n, double beta, double fug,
+                                                 double* __restrict__ out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = beta * exp(-beta * c1[i]) * fug;
+}
+
 // ---------------------------------------------------------------------------
 // Anderson coefficients: minimise a^T G a over the simplex {a >= 0, sum a = 1}
 // exactly, by solving the equality-constrained problem on every support and
@@ -1348,6 +1436,7 @@ __global__ void __launch_bounds__(64) k_anderson_coef(AndDev ad, const int* __re
     Gring[ad.slot_new * CMM_MMAX + lane] = row[lane];
     Gring[lane * CMM_MMAX + ad.slot_new] = row[lane];
   }
+  if (ad.store_only) return;   // a step without mixing keeps the ring current
   __syncthreads();
   const int mk = ad.mk;
   // ordered sub-matrix (oldest -> newest), normalised by its largest diagonal

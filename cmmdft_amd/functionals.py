@@ -177,8 +177,15 @@ class HardSphereFunctional(Functional):
         return like_input(c1.reshape(self.grid.npoints), krho)
 
     def value(self, krho, local=False):
+        """F_HS = int Phi / beta, or Phi / beta per point with ``local``
+        (functionals.py:611-618)."""
         if local:
-            raise NotImplementedError('local free-energy densities are not produced by the device path')
+            torch = torch_mod()
+            nf = self.weighted_densities_dev(krho)
+            plan = self.grid.plan()
+            phi = torch.empty(tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
+            plan.local_density(nfld=nf, phi=phi)
+            return like_input(phi, krho)
         rho = density_from(self.grid, krho)
         _, s = self._engine.run(self._func(), rho, want_c1=False)
         return s[S_PHI] * self.grid.dr / self.beta
@@ -198,6 +205,46 @@ class HardSphereFunctional(Functional):
     def get_n3(self, krho):                           # functionals.py:554-556
         n = self.weighted_densities_dev(krho)
         return like_input(n[1], krho)
+
+    def get_n2_nv2(self, krho):                       # functionals.py:558-563
+        """|n2| / n_v2 per vector component, shape (nx, ny, nz, 3)."""
+        torch = torch_mod()
+        n = self.weighted_densities_dev(krho)
+        out = n[0].abs()[..., None] / torch.stack([n[2], n[3], n[4]], dim=-1)
+        return like_input(out, krho)
+
+    def _weight_table(self):
+        torch = torch_mod()
+        key = float(self.R)
+        if getattr(self, '_wtab_key', None) != key:
+            plan = self.grid.plan()
+            t = torch.empty((2,) + tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
+            plan.hs_weight_table(self.R, t)
+            self._wtab, self._wtab_key = t, key
+        return self._wtab
+
+    @property
+    def scalar_weight_functions(self):
+        """[w0, w1, w2, w3]: the reference's k-space weights on the full grid
+        (complex numpy arrays, functionals.py:433-447), from the device."""
+        t = self._weight_table()
+        R = self.R
+        w0 = t[0].cpu().numpy().astype(np.complex128)
+        w3 = (4.0 * np.pi * R ** 3 / 3.0) * t[1].cpu().numpy().astype(np.complex128)
+        return [w0, R * w0, 4.0 * np.pi * R ** 2 * w0, w3]
+
+    @property
+    def vector_weight_functions(self):
+        """[w_v1, w_v2] with w_v2 = -i k w3 (zero where kR <= 1e-8) and
+        w_v1 = w_v2 / (4 pi R) (functionals.py:449-451), shape (nx, ny, nz, 3)."""
+        R = self.R
+        w3 = self.scalar_weight_functions[3]
+        k = [self.grid.k1d(a) for a in range(3)]
+        kv = np.stack(np.meshgrid(*k, indexing='ij'), axis=-1)
+        omega = np.sqrt(np.sum(kv * kv, axis=-1)) * R
+        wv2 = -1j * w3[..., None] * kv
+        wv2[omega <= 1e-8] = 0.0
+        return [wv2 / (4 * np.pi * R), wv2]
 
     def set_density(self, krho):
         self.krho = krho
@@ -242,16 +289,45 @@ class WDAVFunctional(Functional):
         plan.weighted_densities(out)
         return like_input(out[5], krho)
 
+    def _eos_fields(self, wd, f=False, df=False):
+        """f_ex(wd) and/or f_ex'(wd) per point on the device."""
+        torch = torch_mod()
+        rb = as_device(wd).reshape(tuple(self.grid.npoints)).contiguous()
+        plan = self.grid.plan()
+        plan.set_functional(self._func())
+        fo = torch.empty_like(rb) if f else None
+        dfo = torch.empty_like(rb) if df else None
+        plan.local_density(rbar=rb, fex=fo, dfex=dfo)
+        return fo, dfo
+
     def derive(self, krho, wd=None):
+        """c_WDA = ifft(fft(f_ex'(wd)) w_D) with wd the weighted density of
+        krho unless given (functionals.py:1160-1174)."""
         if wd is not None:
-            raise NotImplementedError('derive(krho, wd) with a precomputed weighted density')
+            torch = torch_mod()
+            _, g = self._eos_fields(wd, df=True)
+            plan = self.grid.plan()
+            plan.set_functional(self._func())
+            plan.seed(g.reshape(-1))
+            out = torch.empty((plan.nfields,) + tuple(self.grid.npoints), dtype=torch.float64, device='cuda')
+            plan.weighted_densities(out)
+            return like_input(out[5].clone(), wd if isinstance(wd, np.ndarray) else krho)
         rho = density_from(self.grid, krho)
         c1, _ = self._engine.run(self._func(), rho)
         return like_input(c1.reshape(self.grid.npoints), krho)
 
     def value(self, krho, wd=None, local=False):
-        if local or wd is not None:
-            raise NotImplementedError('local / precomputed-wd WDA values')
+        """int f_ex(wd), or f_ex(wd) per point with ``local``; wd is the
+        weighted density of krho unless given (functionals.py:1176-1184)."""
+        if wd is not None or local:
+            from ._lib import field_reduce
+            ref = wd if (wd is not None and isinstance(wd, np.ndarray)) else krho
+            if wd is None:
+                wd = self._get_weighted_density(krho)
+            f, _ = self._eos_fields(wd, f=True)
+            if local:
+                return like_input(f, ref)
+            return field_reduce(f, 'sum') * self.grid.dr
         rho = density_from(self.grid, krho)
         _, s = self._engine.run(self._func(), rho, want_c1=False)
         return s[S_FEX] * self.grid.dr
@@ -407,7 +483,8 @@ class ExternalPotential(Functional):
     def __init__(self, grid, system, epot_dr='.', limit_potential=1e4 * U.kjmol, degree=9, interpolate=False,
                  cutoff=12 * U.angstrom):
         self.grid = grid
-        self.potential = None
+        self._np = None           # numpy copy of the potential (made on first access)
+        self._dev = None          # device potential
         self.kpotential = None
         self.system = system
         self.host = system.host
@@ -417,15 +494,30 @@ class ExternalPotential(Functional):
         self.epot_dr = epot_dr
         self.limit_potential = limit_potential
         self.degree = degree
-        self._dev = None
         self.generated = None     # {'images': bool} when generated from the host
 
     def copy(self, grid=None):
         e = type(self)(grid if grid is not None else self.grid.copy(), self.system, self.epot_dr,
                        self.limit_potential, self.degree, self.interpolate, self.cutoff)
-        if self.potential is not None:
-            e.potential = self.potential.copy()
+        if self._dev is not None:
+            e._dev = self._dev.clone()
+        elif self._np is not None:
+            e._np = self._np.copy()
         return e
+
+    @property
+    def potential(self):
+        """The potential on the grid as a numpy array (functionals.py:941-1083);
+        the device copy is authoritative and the numpy one is made on first
+        access."""
+        if self._np is None and self._dev is not None:
+            self._np = self._dev.reshape(self.grid.npoints).cpu().numpy()
+        return self._np
+
+    @potential.setter
+    def potential(self, value):
+        self._np = None if value is None else np.asarray(value, dtype=np.float64)
+        self._dev = None
 
     def generate_potential(self, temperature=None, rewrite=False, images=False):
         """The host's potential on the grid (functionals.py:1030-1066): the LJ
@@ -452,7 +544,7 @@ class ExternalPotential(Functional):
         else:
             raise NotImplementedError('no external potential for guests of type %s' % type(self.guest).__name__)
         self._dev = V.reshape(self.grid.npoints)
-        self.potential = self._dev.cpu().numpy()
+        self._np = None
 
     def _interpolators(self, images=False):
         """The reference's interpolate=True set-up (functionals.py:1035-1049):
@@ -485,17 +577,18 @@ class ExternalPotential(Functional):
 
     def clamp(self):
         """Non-finite and larger-than-limit values -> limit
-        (FreeEnergy.add_external_potential, functionals.py:247-251)."""
-        V = np.array(self.potential, dtype=np.float64)
-        V[~np.isfinite(V)] = self.limit_potential
-        V[V > self.limit_potential] = self.limit_potential
-        self.potential = V
-        self._dev = None
+        (FreeEnergy.add_external_potential, functionals.py:247-251), on the
+        device."""
+        torch = torch_mod()
+        V = self.potential_dev
+        lim = torch.full((), self.limit_potential, dtype=V.dtype, device=V.device)
+        self._dev = torch.where(torch.isfinite(V) & (V <= lim), V, lim)
+        self._np = None
 
     @property
     def potential_dev(self):
         if self._dev is None:
-            self._dev = as_device(self.potential).reshape(self.grid.npoints)
+            self._dev = as_device(self._np).reshape(self.grid.npoints)
         return self._dev
 
     def load_potential(self, fn):                      # functionals.py:1009-1018
@@ -508,7 +601,6 @@ class ExternalPotential(Functional):
         else:
             raise ValueError('Grid shape %s does not match potential shape %s' % (self.grid.npoints, potential.shape))
         self.generated = None
-        self._dev = None
 
     def dump_potential(self, fn):
         assert self.potential is not None
@@ -572,20 +664,61 @@ class FreeEnergy:
         wda.set_temperature(self.temperature, self.system.guest.Rhs)
         self.add_part(wda)
 
+    def _epot_cache_file(self, temperature):
+        """File of the cached external potential and its directory
+        (functionals.py:218-235): <prefix>/<hostname>/<guestname>/<ff_suffix>/
+        <grid_suffix>/<suffix>/epot.npy, or eff_epot_<T>K.npy for a
+        many-atom non-spherical guest; the work directory gets an 'ExtPots'
+        link to that directory."""
+        nd = self.name_dict
+        epot_dr = Path(nd['prefix']) / nd['hostname'] / nd['guestname'] / nd['ff_suffix'] / nd['grid_suffix'] / \
+            nd['suffix']
+        epot_dr.mkdir(parents=True, exist_ok=True)
+        guest = self.system.guest
+        if isinstance(guest, NonSphericalGuest) and guest.mol.natom != 1:
+            assert temperature is not None, 'Temperature must be provided for non-spherical particles'
+            fn = epot_dr / f'eff_epot_{temperature:#3.2f}K.npy'
+        else:
+            fn = epot_dr / 'epot.npy'
+        sym = self.workdir / 'ExtPots'
+        if not sym.is_symlink():
+            self.workdir.mkdir(parents=True, exist_ok=True)
+            sym.symlink_to(epot_dr.absolute())
+        return fn
+
     def add_external_potential(self, temperature=None, cutoff=12 * U.angstrom, limit_potential=1e4 * U.kjmol,
                                degree=9, interpolate=False, rewrite=False, load_fn=None, save_fn=None,
                                potential=None, images=False, **kwargs):
+        """The host's external potential as a part (functionals.py:176-251):
+        loaded from ``load_fn``; or generated on the device and written to
+        ``save_fn``; or (with file output on and a program's name_dict)
+        taken from the epot.npy cache, generated and written there when the
+        file is missing or ``rewrite``/``overwrite`` is set.  ``potential``
+        gives the array directly.  Non-finite and larger values are clamped
+        to ``limit_potential``."""
         epot = ExternalPotential(self.grid, self.system, '.', limit_potential=limit_potential, degree=degree,
                                  interpolate=interpolate, cutoff=cutoff)
         if potential is not None:
             epot.potential = np.asarray(potential, dtype=np.float64).reshape(self.grid.npoints)
         elif load_fn is not None:
-            assert str(load_fn).endswith('.npy') and os.path.isfile(load_fn)
+            assert str(load_fn).endswith('.npy'), 'fn must be a filename of an external potential'
+            assert os.path.isfile(load_fn), f'fn must be a filename of an external potential, {load_fn}'
+            epot.epot_dr = Path(load_fn).parent
             epot.load_potential(load_fn)
         else:
-            epot.generate_potential(temperature, rewrite=rewrite, images=images)
+            fn = None
             if save_fn is not None:
-                epot.dump_potential(save_fn)
+                fn = Path(save_fn)
+            elif self.write_files and 'prefix' in self.name_dict:
+                fn = self._epot_cache_file(temperature)
+            if fn is not None:
+                epot.epot_dr = fn.parent
+            if fn is None or not os.path.isfile(fn) or self.overwrite or rewrite:
+                epot.generate_potential(temperature, rewrite=rewrite, images=images)
+                if fn is not None:
+                    epot.dump_potential(str(fn))
+            else:
+                epot.load_potential(fn)
         epot.clamp()
         self.add_part(epot)
         return epot
@@ -631,8 +764,48 @@ class FreeEnergy:
         self.add_part(corr)
         return corr
 
+    def init_tracking(self, fn, rewrite=False):
+        """Start the energy-tracking file (functionals.py:90-110): a header of
+        the columns, or continue an existing file's step count."""
+        self.fn_tracking = fn
+        if not os.path.isfile(fn) or self.overwrite or rewrite:
+            with open(self.fn_tracking, 'w') as f:
+                print('#phase\tstep\t     loading\t         -\u00b5N\t        IdGas\t'
+                      + '\t'.join(['%s%s' % (' ' * (13 - len(part.name)), part.name) for part in self.parts])
+                      + '\t        Total', file=f)
+            self.tracking_step = 0
+        else:
+            with open(self.fn_tracking, 'r') as f:
+                lines = f.readlines()
+            self.tracking_step = 0 if len(lines) <= 1 else int(lines[-1].split()[1]) + 1
+
+    def track(self, chempot, rho, krho=None, iphase=0, write=True, print_out=False, fn=None, unit=1):
+        """Grand potential of a density and, with ``write``, its line in the
+        tracking file (functionals.py:112-174).  The ideal-gas term takes
+        rho <= 1e-8 as 1e-30, as the reference's does here."""
+        from ._lib import field_reduce
+        r = as_device(rho).reshape(tuple(self.grid.npoints))
+        N = field_reduce(r, 'sum') * self.grid.dr
+        Fid = field_reduce(r, 'fid_track', lam3=self.wavelength ** 3) * self.grid.dr / self.beta
+        G = Fid - chempot * N
+        line = '%6i\t%4i\t%.6e\t%.6e\t% .6e' % (iphase, getattr(self, 'tracking_step', 0), N, (-chempot * N / unit),
+                                                   Fid / unit)
+        k = r if krho is None else krho
+        for part in self.parts:
+            Fpart = part.value(k)
+            if print_out:
+                print(part.name, round(Fpart / U.kjmol, 2))
+            G += Fpart
+            line += '\t% .6e' % (Fpart / unit)
+        line += '\t% .6e' % (G / unit)
+        if write:
+            with open(fn if fn is not None else self.fn_tracking, 'a') as f:
+                print(line, file=f)
+            self.tracking_step = getattr(self, 'tracking_step', 0) + 1
+        return G
+
     def track_omega(self, chempot, rho):
-        """Grand potential of a density (FreeEnergy.track without file output,
-        functionals.py:121-174)."""
+        """Grand potential of a density (the solver's form,
+        solver.py:103-119), without file output."""
         from .solver import grand_potential
         return grand_potential(self, chempot, rho)

@@ -9,6 +9,7 @@ tools.py:64-79).
 from __future__ import annotations
 
 import os
+import warnings
 from pathlib import Path
 
 import numpy as np
@@ -160,6 +161,9 @@ class Program:
         self.solver = solver
 
     def cascade_solver(self, solvers, chempot, **kwargs):
+        """Try the solvers in turn until one converges (program.py:315-329).
+        Returns that solve's (N, rho, converged); when none converges the
+        reference only logs a warning: here a RuntimeWarning, and None."""
         for solver in solvers:
             self.set_solver(solver)
             try:
@@ -168,12 +172,18 @@ class Program:
                     return N, rho, converged
             except NoSolutionError:
                 continue
+        warnings.warn('All solvers failed, at %7.5fkJmol %7.5fK.' % (chempot / U.kjmol, self.fener.temperature),
+                      RuntimeWarning)
         return None
 
-    def solve(self, chempot, Ninit=None, rewrite=False, energy_tracking=False, silent=False, continue_solving=False):
+    def solve(self, chempot, Ninit=None, rewrite=False, energy_tracking=True, silent=False, continue_solving=False):
         """Equilibrium density at ``chempot`` (program.py:331-404).  Returns
-        (N, rho (numpy), converged)."""
+        (N, rho (numpy), converged).  With file output on, ``energy_tracking``
+        writes the grand potential and its parts at every step to
+        convergence_<suffix>.txt in the work directory."""
         self.file_suffix = get_file_suffix(chempot, self.fener.temperature)
+        if energy_tracking and self.write_files:
+            self.fener.init_tracking(os.path.join(self.workdir, 'convergence_%s.txt' % self.file_suffix))
         self.rho_fn = os.path.join(self.workdir, 'rho_%s.npy' % self.file_suffix) if self.write_files else None
         if self.rho_fn and os.path.isfile(self.rho_fn) and not self.overwrite and not rewrite and not continue_solving:
             rho = np.load(self.rho_fn)

@@ -134,8 +134,10 @@ int cmm_set_functional(cmm_plan* plan, const cmm_functional* f);
 /* Deterministic reduction of a device field x[n] (points with mask[i] = 0
  * skipped when mask is not NULL) into device out[0]: op 0 sum x, 1 sum
  * exp(-beta x), 2 sum min(max(exp(-beta x), 0), 1), 3 min x, 4 max x, 5 sum
- * x^2, 6 sum c (ln(lam3 c) - 1) with c = max(x, 1e-20).  work: device
- * scratch of at least 1024 doubles. */
+ * x^2, 6 sum c (ln(lam3 c) - 1) with c = max(x, 1e-20), 7 the same with
+ * c = 1e-30 where x <= 1e-8 (the floor of FreeEnergy.track,
+ * cmmdft/functionals.py:151-155).  work: device scratch of at least 1024
+ * doubles. */
 int cmm_field_reduce(const double* x, const unsigned char* mask, long long n, int op, double beta, double lam3,
                      double* work, double* out, void* stream);
 /* Mean-field (MFA) part: kd is a caller-owned complex half spectrum
@@ -187,6 +189,26 @@ int cmm_value(cmm_plan* plan);
  * out[nbatch][nf][npts], nf = 6 or 12 (n3 unclipped) */
 int cmm_weighted_densities(cmm_plan* plan, double* out);
 
+/* Pointwise free-energy densities (per instance, on the plan's stream):
+ * phi[npts] = kT Phi of the weighted densities nfld[nbatch][nf][npts] (as
+ * cmm_weighted_densities writes them), fex/dfex[npts] = f_ex(rbar),
+ * f_ex'(rbar) of the WDA part at the weighted density rbar[nbatch][npts].
+ * NULL inputs or outputs are skipped.
+ * Replaces HardSphereFunctional.value(local=True) (get_phi) and the EOS
+ * terms of WDAVFunctional.derive / value with a given weighted density
+ *                                                      cmmdft/functionals.py:611-643, 1160-1184 */
+int cmm_local_density(cmm_plan* plan, const double* nfld, const double* rbar, double* phi, double* fex,
+                      double* dfex);
+/* The reference's k-space hard-sphere weights on the full nx*ny*nz grid of
+ * the plan: out[0][..] = sinc(kR) sigma_Lanczos, out[1][..] =
+ * 3 (sin kR - kR cos kR)/(kR)^3 sigma_Lanczos (both sigma_Lanczos where
+ * kR <= 1e-8); w0, w1 = R w0, w2 = 4 pi R^2 w0, w3 = 4 pi R^3/3 out[1],
+ * w_v2 = -i k w3, w_v1 = w_v2 / (4 pi R) follow.
+ * Replaces HardSphereFunctional._init_weight_functions (the attributes
+ * scalar_weight_functions, vector_weight_functions)
+ *                                                      cmmdft/functionals.py:421-455 */
+int cmm_hs_weight_table(cmm_plan* plan, double R, double* out);
+
 /* launch one pass of the functional chain on the current workspace
  * (measurement aid, kernel only, no reduction of its partials): 0 functional
  * z pass, 1 forward x pass with weights, 2 back-convolution x pass, 3 y pass
@@ -200,10 +222,27 @@ int cmm_finish_picard(cmm_plan* plan, const double* rho, const double* vext, int
                       double* c1_out, double beta, double lam3, double zero_thr);
 int cmm_finish_anderson(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double* rhist,
                         int m, int slot_new, double* c1_out, double beta, double lam3);
+/* convergence measures of rho (writes no field): scalars [3..9] = N, ideal
+ * sum, external sum, non-finite count of G - rho, sum (G - rho)^2, and over
+ * |rho| > 1e-8 the sums of (|rho| beta dOmega / (a_tol + r_tol |rho|))^2 and
+ * (|rho| beta dOmega)^2 with beta dOmega = ln(lam3 rho_c) + beta C1 - beta mu
+ * (rho_c = rho, or 1e-30 below 1e-12).
+ * Replaces the RES and DER criteria of Solver._check_convergence and its
+ * history columns                              cmmdft/solver.py:131-135, 186-233 */
+int cmm_finish_check(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double beta,
+                     double lam3, double a_tol, double r_tol);
+/* out[i] = beta exp(-beta c1[i]) fug (device arrays of n), on stream.
+ * Replaces Solver.get_new_rho                  cmmdft/solver.py:128-129 */
+int cmm_boltzmann(const double* c1, long long n, double beta, double fug, double* out, void* stream);
 
 /* Anderson coefficients on the device for the mk newest residual slots
  * order[0..mk) (oldest first); reads this step's Gram row from the scalars. */
 int cmm_anderson_init(cmm_plan* plan, const double* damping, const double* cf);
+/* store this step's Gram row (scalars of the last cmm_finish_anderson) into
+ * the device Gram ring without mixing: a hybrid (Picard) step of the
+ * Anderson solver still saves its residual into the history, as
+ * Anderson._save_previous_rhos does on every step  cmmdft/solver.py:527-535, 550-552 */
+int cmm_anderson_store_row(cmm_plan* plan, int m, int slot_new);
 int cmm_anderson_coef(cmm_plan* plan, int m, int mk, const int* order, int slot_new, int slot_prev, int adaptive,
                       double fac_up, double fac_down, double dmin, double dmax);
 int cmm_anderson_mix(cmm_plan* plan, const double* rhohist, const int* order_rho, const double* rhist,

@@ -1113,16 +1113,18 @@ class Solver:
         if self.track_energy:
             self.omega(rho_new, krho_new)
         ok = True
+        self.RES = self.DER = np.nan
         for c, th in zip(self.criterion, self.threshold):
             if c == 'riue':
                 val = self.RIUE
             elif c == 'res':
-                val = np.linalg.norm(self.grho(C1_new) - rho_new) / np.sqrt(N_new) / np.sqrt(g.npts)
+                val = self.RES = np.linalg.norm(self.grho(C1_new) - rho_new) / np.sqrt(N_new) / np.sqrt(g.npts)
             elif c == 'der':
                 mask = np.isclose(rho_new, 0)
                 dO = np.log(self.wavelength ** 3 * self.clip(rho_new)) / self.beta + C1_new - self.chempot
-                val = np.linalg.norm((np.abs(rho_new) * self.beta * dO
-                                      / (self.a_tol + self.r_tol * np.abs(rho_new)))[~mask]) / np.sqrt(g.npts)
+                val = self.DER = np.linalg.norm((np.abs(rho_new) * self.beta * dO
+                                                 / (self.a_tol + self.r_tol * np.abs(rho_new)))[~mask]) \
+                    / np.sqrt(g.npts)
             else:
                 val = self.maxdrho
             ok = ok and (val < th)

@@ -115,3 +115,39 @@ def test_batch_equals_plain_larger_grid():
         assert conv and s.curr_step == res['iters'][i], (i, s.curr_step, res['iters'][i])
         assert torch.equal(rho.reshape(-1), res['rho'][i].reshape(-1))
     b.close()
+
+
+def test_graph_recaptured_when_captured_values_change():
+    """The replayed graph holds beta, Lambda^3, the functional and the
+    external-potential pointer by value: a solve after a temperature change,
+    and solves whose potential is re-summed from two parts on every start,
+    equal a fresh solver's bit for bit."""
+    import torch
+    from cmmdft_amd.batch import BatchedAnderson
+    host = W.host_case1()
+    mus = _mus(4)
+    prog = product_program(host, SHAPE, 'WBII')
+    b = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400)
+    r300 = b.solve()
+    prog.fener.set_temperature(330.0)
+    r330 = b.solve()
+    fresh = BatchedAnderson(prog, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400).solve()
+    assert np.array_equal(r330['N'], fresh['N']) and np.array_equal(r330['iters'], fresh['iters'])
+    assert torch.equal(r330['rho'], fresh['rho'])
+    assert not np.array_equal(r300['N'], r330['N'])
+    b.close()
+    # two external-potential parts (half the potential each): V is a new sum per start
+    prog2 = product_program(host, SHAPE, 'WBII')
+    ep = prog2.fener.part_dict['ExtPot']
+    half = ep.potential * 0.5
+    prog2.fener.remove_part('ExtPot')
+    prog2.fener.add_external_potential(potential=half)
+    prog2.fener.add_external_potential(potential=half)
+    b2 = BatchedAnderson(prog2, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400)
+    first = b2.solve()
+    torch.cuda.empty_cache()
+    second = b2.solve()
+    fresh2 = BatchedAnderson(prog2, mus, m=5, damping=0.1, threshold=1e-10, nsteps=400).solve()
+    for r in (first, second):
+        assert np.array_equal(r['N'], fresh2['N']) and torch.equal(r['rho'], fresh2['rho'])
+    b2.close()
