@@ -106,6 +106,9 @@ EXPORTS = {
     'cmm_lj_grid_grad': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                           C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_void_p,
                           C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_lj_grid_grad_noalloc': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                  C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_void_p], C.c_int),
     'cmm_effpot': ([C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                     C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                     C.c_double, C.c_double, C.c_int, C.c_double, C.c_void_p, C.c_void_p], C.c_int),

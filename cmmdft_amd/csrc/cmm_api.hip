@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1349,6 +1350,55 @@ int cmm_lj_grid_grad(int nx, int ny, int nz, const double* rvecs, const double* 
   hipLaunchKernelGGL(k_lj_bwd_reduce, dim3(2 * ntypes), dim3(LJ_NT), 0, (hipStream_t)stream, work, nblocks, grad);
   LAUNCHCHK("k_lj_bwd_reduce");
   return CMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the allocation-free entry point.  Each GPU the
+// library runs on holds one copy of this array until the process ends; its
+// device address is looked up once per device with that device current
+// (hipGetSymbolAddress) and used from then on -- the array's name is a
+// host-side stand-in in host code, never passed to a kernel.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[CMM_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_ptr(double** out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(CMM_E_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(CMM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return CMM_OK;
+}
+}  // namespace
+
+int cmm_lj_grid_grad_noalloc(int nx, int ny, int nz, const double* rvecs, const double* atoms, const int* types,
+                             int natom, int ntypes, double sigma_f, double epsilon_f, double cutoff, int images,
+                             double limit, const double* w, int nw, double* grad, void* stream) {
+  long long need = 0;
+  int rc = cmm_lj_grad_workspace(nx, ny, nz, ntypes, &need);
+  if (rc) return rc;
+  if (need > (long long)CMM_NOALLOC_SCRATCH_DOUBLES)
+    return fail(CMM_E_ARG, "cmm_lj_grid_grad_noalloc: the grid and type count need " + std::to_string(need) +
+                               " doubles of scratch, above the fixed " +
+                               std::to_string((long long)CMM_NOALLOC_SCRATCH_DOUBLES) +
+                               "; use cmm_lj_grid_grad with a workspace");
+  double* work = nullptr;
+  rc = scratch_ptr(&work);
+  if (rc) return rc;
+  return cmm_lj_grid_grad(nx, ny, nz, rvecs, atoms, types, natom, ntypes, sigma_f, epsilon_f, cutoff, images, limit,
+                          w, nw, work, grad, stream);
 }
 
 int cmm_batch_reset(cmm_plan* p, const double* mu, double beta, double lam3, double dr, double tol, int nsteps) {

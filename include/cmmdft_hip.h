@@ -381,6 +381,20 @@ int cmm_lj_grid_grad(int nx, int ny, int nz, const double* rvecs, const double* 
                      int ntypes, double sigma_f, double epsilon_f, double cutoff, int images, double limit,
                      const double* w, int nw, double* work, double* grad, void* stream);
 
+/* cmm_lj_grid_grad without the workspace argument and without any device
+ * allocation: its per-block partials go to a fixed device array of
+ * CMM_NOALLOC_SCRATCH_DOUBLES doubles (4 MiB) that the library holds on each
+ * GPU it runs on until the process ends.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Grids
+ * and type counts whose workspace (cmm_lj_grad_workspace) exceeds the array
+ * are rejected with CMM_E_ARG.  Calls that overlap on one GPU share the
+ * array and race; the caller still owns w and grad. */
+#define CMM_NOALLOC_SCRATCH_DOUBLES (1 << 19)
+int cmm_lj_grid_grad_noalloc(int nx, int ny, int nz, const double* rvecs, const double* atoms, const int* types,
+                             int natom, int ntypes, double sigma_f, double epsilon_f, double cutoff, int images,
+                             double limit, const double* w, int nw, double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
