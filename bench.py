@@ -102,6 +102,17 @@ def time_passes(plan, reps=20):
     return out
 
 
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample, cores=1):
     """The oracle (NumPy/SciPy restatement of the reference path; scipy.fft
     with ``cores`` workers, BLAS/OpenMP pools limited to as many) timed for
@@ -134,7 +145,7 @@ def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample, cores=1):
         s.solve(mu, rho0)
     per_iter = float(np.median(np.diff(stamps))) if len(stamps) > 1 else float('nan')
     pts = 1.0 / (per_iter * gpu_iters_per_point)
-    return {'value': pts, 'unit': 'points/s', 'cores': cores, 'kind': 'port',
+    return {'value': pts, 'unit': 'points/s', 'cores': cores, 'kind': 'port', 'host_cpu': host_cpu_model(),
             'sample': ('oracle Anderson (SLSQP mixing, scipy.fft %d worker%s) at %dx%dx%d, %d iterations timed, '
                        'median %.3f s/iteration, extrapolated with the GPU mean of %.1f iterations/point'
                        % ((cores, '' if cores == 1 else 's') + shape + (nsample, per_iter, gpu_iters_per_point))),
@@ -271,16 +282,20 @@ def uneven_leg(nframe=16, npress=8):
     batched on one GPU (UnevenBatch, admission by device footprint), then the
     batched adjoint of every framework: dN_i/dmu_i and the gradient of the
     framework's total uptake with respect to its 8 host parameters.  External
-    potentials are generated before the timed region (vext_s)."""
+    potentials are generated before the timed region: vext_s is the set-up of
+    the programs (image-sum LJ grids, clamps, mu(p)), frameworks_s the
+    host-side random placement of the synthetic atoms."""
     import torch
     from cmmdft_amd import workloads as W
     from cmmdft_amd.batch import UnevenBatch
     from cmmdft_amd.adjoint import BatchedAdjoint
+    t0 = time.perf_counter()
+    hosts = [W.host_case5(k) for k in range(nframe)]
+    frameworks_s = time.perf_counter() - t0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     items = []
-    for k in range(nframe):
-        host = W.host_case5(k)
+    for host in hosts:
         prog = W.framework_program(host, host.notes['npoints'], images=True)
         items.append((prog, W.chempots_at(W.pressures_uneven(npress))))
     torch.cuda.synchronize()
@@ -305,7 +320,8 @@ def uneven_leg(nframe=16, npress=8):
             'iterations_mean': float(iters.mean()), 'iterations_max': int(iters.max()),
             'peak_concurrent_frameworks': ub.stats['peak_concurrent'], 'peak_bytes': ub.stats['peak_bytes'],
             'adjoint_s': adj_s, 'adjoint_vs_solve': adj_s / solve_s,
-            'cg_iterations_max': int(max(g['iters'] for g in grads)), 'vext_s': vext_s,
+            'cg_iterations_max': int(max(g['iters'] for g in grads)), 'frameworks_s': frameworks_s,
+            'vext_s': vext_s,
             'total_uptake_grad_norm': [float(np.linalg.norm(np.concatenate([g['dNtot_dsigma'], g['dNtot_depsilon']])))
                                        for g in grads][:4]}
 
@@ -391,6 +407,15 @@ def main():
     prog.fener.add_external_potential(cutoff=W.CUTOFF)
     torch.cuda.synchronize()
     vext_ms = (time.perf_counter() - tv) * 1e3
+    # the same once the device code is loaded (grid sum and clamp), on a
+    # part that is then dropped again
+    tv = time.perf_counter()
+    prog.fener.add_external_potential(cutoff=W.CUTOFF)
+    torch.cuda.synchronize()
+    vext_warm_ms = (time.perf_counter() - tv) * 1e3
+    prog.fener.parts.pop()
+    prog.fener.part_names.pop()
+    prog.fener.part_dict['ExtPot'] = prog.fener.parts[-1]
     solver = Anderson(prog, nsteps=500, method='anderson', m=5, damping=0.1, criterion='RIUE', threshold=1e-8)
     prog.set_solver(solver)
     from cmmdft_amd.batch import BatchedAnderson
@@ -510,6 +535,7 @@ def main():
         'iteration_algorithmic_GBps': iter_gbps,
         'all_converged': bool(all(convs) and all(bconvs)),
         'vext_ms': vext_ms,
+        'vext_warm_ms': vext_warm_ms,
         'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
         'pass_ms_batched': ({PASS_NAMES[k]: v for k, v in tpr.items()} if nbr > 1 else None),
         'pass_GBps_batched': ({PASS_NAMES[k]: nbr * pb[k] / (v * 1e-3) / 1e9 for k, v in tpr.items()}

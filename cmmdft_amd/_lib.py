@@ -34,7 +34,7 @@ class CmmFunctional(C.Structure):
         ('hs_base', C.c_int), ('hs_antisym', C.c_int), ('hs_tensor', C.c_int), ('use_wda', C.c_int),
         ('n_eos', C.c_int),
         ('eos_kind', C.c_int * 4), ('eos_factor', C.c_double * 4), ('eos_par', (C.c_double * 16) * 4),
-        ('R', C.c_double), ('D', C.c_double), ('kT', C.c_double),
+        ('R', C.c_double), ('D', C.c_double), ('kT', C.c_double), ('n3_series', C.c_double),
     ]
 
 

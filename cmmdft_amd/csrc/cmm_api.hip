@@ -370,6 +370,9 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
     if (t < f->n_eos && (f->eos_kind[t] < 0 || f->eos_kind[t] > 2)) return fail(CMM_E_ARG, "unknown EOS kind");
   }
   d.R = d.use_hs ? f->R : 1.0;
+  if (!(f->n3_series >= 0.0 && f->n3_series <= 1e-3))
+    return fail(CMM_E_ARG, "n3_series must lie in [0, 1e-3] (the series forms are second order)");
+  d.n3_series = f->n3_series;
   d.D = f->D;
   d.kT = f->kT;
   d.c_w2 = 4.0 * M_PI * d.R * d.R;

@@ -136,12 +136,53 @@ __device__ __forceinline__ void dft8(double2* v) {
   v[7] = csub(e[3], o3);
 }
 
+// x (c + SGN i s): multiply by exp(SGN i theta) given c = cos, s = sin
+template <int SGN>
+__device__ __forceinline__ double2 crot(double2 x, double c, double s) {
+  return make_double2(x.x * c - SGN * x.y * s, x.y * c + SGN * x.x * s);
+}
+
+// 16-point DFT as 4 x 4: radix-4 transforms over the stride-4 subsequences,
+// the twiddles exp(SGN 2 pi i n1 k2 / 16), radix-4 transforms across them;
+// the outputs are in natural order.
+template <int SGN>
+__device__ __forceinline__ void dft16(double2* v) {
+  const double c1 = 0.92387953251128675613;   // cos(pi / 8)
+  const double s1 = 0.38268343236508977173;   // sin(pi / 8)
+  const double h = 0.70710678118654752440;    // cos(pi / 4)
+  double2 a[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    double2 x[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    dft4<SGN>(x);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) a[n1][k2] = x[k2];
+  }
+  a[1][1] = crot<SGN>(a[1][1], c1, s1);    // exponent 1
+  a[1][2] = crot<SGN>(a[1][2], h, h);      // 2
+  a[1][3] = crot<SGN>(a[1][3], s1, c1);    // 3
+  a[2][1] = crot<SGN>(a[2][1], h, h);      // 2
+  a[2][2] = cmul_i(a[2][2], (double)SGN);  // 4
+  a[2][3] = crot<SGN>(a[2][3], -h, h);     // 6
+  a[3][1] = crot<SGN>(a[3][1], s1, c1);    // 3
+  a[3][2] = crot<SGN>(a[3][2], -h, h);     // 6
+  a[3][3] = crot<SGN>(a[3][3], -c1, -s1);  // 9
+#pragma unroll
+  for (int k2 = 0; k2 < 4; ++k2) {
+    double2 y[4] = {a[0][k2], a[1][k2], a[2][k2], a[3][k2]};
+    dft4<SGN>(y);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) v[k2 + 4 * k1] = y[k1];
+  }
+}
+
 template <int R, int SGN>
 __device__ __forceinline__ void dftr(double2* v) {
   if constexpr (R == 2) dft2<SGN>(v);
   else if constexpr (R == 3) dft3<SGN>(v);
   else if constexpr (R == 4) dft4<SGN>(v);
   else if constexpr (R == 5) dft5<SGN>(v);
+  else if constexpr (R == 16) dft16<SGN>(v);
   else dft8<SGN>(v);
 }
 
@@ -275,9 +316,16 @@ __device__ __forceinline__ void lds_fft_c(double2* buf, int batch, const FastDiv
 // one thread holds per stage: the caller must guarantee
 // batch * n <= VPT * blockDim.x, and that a barrier separates the data's
 // producers from this call.  Ends with a barrier.
-template <int SGN, bool EM, int VPT>
+template <int SGN, bool EM, int VPT, bool R16 = false>
 __device__ __forceinline__ void lds_fft(double2* buf, int batch, const FastDiv& fb, int bs, int es, const Fft1D& p,
                         const double2* tws) {
+  // R16 (LDS-bound callers): length 128 as two stages, radix 16 then 8 --
+  // two LDS round trips and one twiddled stage instead of three and two.
+  // Needs batch * n / 16 <= blockDim.x, which batch * n <= VPT * blockDim.x
+  // with VPT <= 16 guarantees.
+  if constexpr (R16) {
+    if (p.n == 128) return lds_fft_c<SGN, EM, VPT, 128, 1, 16, 8>(buf, batch, fb, bs, es, tws);
+  }
   // compile-time stage sequences: the tuned plans of 128, 96, 64 and 48 and
   // the default plans of the 5-smooth lengths 48..160
 #define CMM_CODELET(NN, ...)                                                          \

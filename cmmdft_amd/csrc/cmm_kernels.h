@@ -48,6 +48,7 @@ struct FuncDev {
   double eos_fac[4];
   double eos_par[4][16];
   double R, D, kT;
+  double n3_series;  // MFMT / White-Bear II: series forms of phi2, phi3 for n3 <= this
   double c_w2, c_w3, inv4piR2, inv4piR;
   double c_tor;     // 3 / R: tensor weight B = w2 - (3/R) w3
   const double2* kmfa;   // mean-field (MFA) kernel, half spectrum [nx][ny][nz/2+1]; null when off

@@ -121,17 +121,17 @@ __device__ __forceinline__ double kz_of(const double* kzk, int kzn, int k) { ret
 // VZ (six-field functional): field 4, the z component of n_v2, is not stored;
 // it is -i kz times field 1's (n3 = w3 rho^) spectrum, zero on the Nyquist
 // plane, formed here from field 1 per pencil.
-template <bool VZ = false>
+template <bool VZ = false, int NU = LU>
 __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* src, long long fs, int nf, long long p0,
                                                   const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn,
                                                   const double* kzk = nullptr, int kzn = -1) {
   const int per = t.B * nzh;
   const int tot = nf * per;
   const int ip0 = (int)p0, ilast = (int)npen - 1;
-  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
-    double2 A[LU], B[LU];
+  for (int i0 = threadIdx.x; i0 < tot; i0 += NU * blockDim.x) {
+    double2 A[NU], B[NU];
 #pragma unroll
-    for (int u = 0; u < LU; ++u) {
+    for (int u = 0; u < NU; ++u) {
       const int i = min(i0 + u * (int)blockDim.x, tot - 1);
       const int f = (int)fdiv((unsigned)i, fBn);
       const int r = i - f * per;
@@ -142,7 +142,7 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
       B[u] = sf[min(pa + 1, ilast) * nzh + k];
     }
 #pragma unroll
-    for (int u = 0; u < LU; ++u) {
+    for (int u = 0; u < NU; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       if (i < tot) {
         const int f = (int)fdiv((unsigned)i, fBn);
@@ -205,10 +205,11 @@ __device__ __forceinline__ void z_store_half_multi(const double2* lds, double2* 
   }
 }
 
-// all nf*B sequences of a z tile, in place
-template <int SGN>
+// all nf*B sequences of a z tile, in place (R16: the two-stage 16 x 8 form
+// of length 128, for the LDS-bound functional z pass)
+template <int SGN, bool R16 = false>
 __device__ __forceinline__ void z_fft(double2* lds, const ZTile& t, const Fft1D& fz, const double2* tws) {
-  lds_fft<SGN, true, VPT>(lds, t.S, t.fS, 1, t.ES, fz, tws);
+  lds_fft<SGN, true, VPT, R16>(lds, t.S, t.fS, 1, t.ES, fz, tws);
 }
 
 // ---------------------------------------------------------------------------
@@ -725,8 +726,10 @@ struct PhiOut {
 };
 
 template <typename T>
-__device__ __forceinline__ void phi23(int base, T n3, T ln3, T r1, T& p2, T& p3, T& d2, T& d3) {
-  // Same forms as the reference (functionals.py:726-767).  Every
+__device__ __forceinline__ void phi23(int base, double n3_series, T n3, T ln3, T r1, T& p2, T& p3, T& d2, T& d3) {
+  // Same forms as the reference (functionals.py:726-767), whose series
+  // branches hold for n3 <= 1e-8 (n3_series; a larger value keeps the closed
+  // forms' cancellation at low n3 out).  Every
   // denominator is a product of powers of (1 - n3) and n3, so the divisions
   // are multiplications by r1 = 1/(1-n3) and (full White-Bear/MFMT branch)
   // 1/n3: q = (1-n3)^2, c = (1-n3)^3, n3 - n3^2 = n3 (1-n3).
@@ -743,7 +746,7 @@ __device__ __forceinline__ void phi23(int base, T n3, T ln3, T r1, T& p2, T& p3,
   } else if (base == 1) {                // modified FMT
     p2 = r1;
     d2 = iq;
-    if (val(n3) <= 1e-8) {
+    if (val(n3) <= n3_series) {
       p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) * i24 * iq;
       d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) * i36 * ic;
     } else {
@@ -753,7 +756,7 @@ __device__ __forceinline__ void phi23(int base, T n3, T ln3, T r1, T& p2, T& p3,
       d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) * i36 * (in2 * in1) * ic;
     }
   } else {                               // White-Bear II
-    if (val(n3) <= 1e-8) {
+    if (val(n3) <= n3_series) {
       p2 = (1.0 + n3_2 / 9.0) * r1;
       d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) * iq;
       p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) * i24 * iq;
@@ -791,7 +794,7 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
   const T ln3 = log(1.0 - n3);
   const T r1 = 1.0 / (1.0 - n3);
   T p2, p3, d2, d3;
-  phi23<T>(fp.base, n3, ln3, r1, p2, p3, d2, d3);
+  phi23<T>(fp.base, fp.n3_series, n3, ln3, r1, p2, p3, d2, d3);
   const T v2 = vx * vx + vy * vy + vz * vz;
   const T dot12 = ux * vx + uy * vy + uz * vz;
   T q3, dq3_n2, fv;
@@ -964,7 +967,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
   z_load_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
-  z_fft<1>(sm, t, g.fz, s_tw);
+  z_fft<1, true>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
   const int tot = t.P * n;
   // pencil index fastest: consecutive lanes walk along an LDS row (no bank
@@ -986,7 +989,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   }
   if (store) {
     __syncthreads();
-    z_fft<-1>(sm, t, g.fz, s_tw);
+    z_fft<-1, true>(sm, t, g.fz, s_tw);
     z_store_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};

@@ -46,11 +46,13 @@ def build_functional(hs=None, wda=None, kT=None):
     f = CmmFunctional()
     f.hs_base = -1
     f.R = 1.0
+    f.n3_series = 1e-8
     if hs is not None:
         f.hs_base = hs_base_of(hs.version)
         f.hs_antisym = 1 if 'a' in hs.version else 0
         f.hs_tensor = 1 if 't' in hs.version else 0
         f.R = hs.R
+        f.n3_series = hs.n3_series
     if wda is not None:
         terms = wda.eos.device_terms()
         f.use_wda = 1
@@ -150,17 +152,27 @@ class HardSphereFunctional(Functional):
     'MFMT', White-Bear II 'WBII', each with the optional 'a' prefix."""
     name = 'HardSphere'
 
-    def __init__(self, grid, Rhs, version='atWBII'):
+    def __init__(self, grid, Rhs, version='atWBII', n3_series=1e-8):
+        """``n3_series``: the modified and White-Bear II forms of phi2, phi3
+        and their derivatives switch to their second-order series below this
+        packing fraction.  The reference's 1e-8 (functionals.py:726-767) is
+        the default; above it the closed forms lose digits by cancellation
+        (e.g. dphi3/dn3 keeps none for 3e-8 < n3 < 1e-6), which makes the
+        solution a non-smooth function of the host parameters at that level.
+        Up to 1e-5 the series is exact to rounding, and derivatives of the
+        solution (finite differences, the adjoint) become smooth."""
         assert version in FMT_NAMES + MFMT_NAMES + WBII_NAMES, 'Unknown hard sphere functional version: %s' % version
+        assert 0.0 <= n3_series <= 1e-3, 'n3_series must lie in [0, 1e-3]'
         self.temperature = None
         self.beta = None
         self.grid = grid
         self.R = Rhs
         self.version = version
+        self.n3_series = float(n3_series)
         self._engine = _PartEngine(grid)
 
     def copy(self, grid=None):
-        return type(self)(grid if grid is not None else self.grid.copy(), self.R, self.version)
+        return type(self)(grid if grid is not None else self.grid.copy(), self.R, self.version, self.n3_series)
 
     def set_temperature(self, temperature, Rhs, **kwargs):
         self.temperature = temperature
@@ -654,8 +666,8 @@ class FreeEnergy:
         self.part_names.pop(i)
         self.part_dict.pop(part_name, None)
 
-    def add_hard_sphere(self, version='atWBII'):
-        hs = HardSphereFunctional(self.grid, self.system.guest.Rhs, version=version)
+    def add_hard_sphere(self, version='atWBII', n3_series=1e-8):
+        hs = HardSphereFunctional(self.grid, self.system.guest.Rhs, version=version, n3_series=n3_series)
         hs.set_temperature(self.temperature, self.system.guest.Rhs)
         self.add_part(hs)
 

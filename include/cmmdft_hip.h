@@ -104,6 +104,10 @@ typedef struct {
   double R;    /* hard-sphere radius */
   double D;    /* WDA weight diameter */
   double kT;   /* 1 / beta */
+  double n3_series;  /* MFMT / White-Bear II: phi2, phi3 and their n3
+                        derivatives by their second-order series where
+                        n3 <= n3_series; 1e-8 is the reference's
+                        (functionals.py:726-767), at most 1e-3 */
 } cmm_functional;
 
 int cmm_version(void);

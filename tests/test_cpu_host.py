@@ -32,7 +32,7 @@ def test_functional_struct_layout_matches_header():
     import ctypes as C
     from cmmdft_amd._lib import CmmFunctional
     # 5 ints + int[4] (+ 4 bytes of padding) + double[4] + double[4][16] + 3 doubles
-    assert C.sizeof(CmmFunctional) == 5 * 4 + 4 * 4 + 4 + 8 * 4 + 8 * 64 + 3 * 8
+    assert C.sizeof(CmmFunctional) == 5 * 4 + 4 * 4 + 4 + 8 * 4 + 8 * 64 + 4 * 8
     assert CmmFunctional.eos_factor.offset == 40
     assert [n for n, _ in CmmFunctional._fields_][:5] == ['hs_base', 'hs_antisym', 'hs_tensor', 'use_wda', 'n_eos']
 
