@@ -55,16 +55,29 @@ def chempots(n=32):
     return np.array([eos.calculate_mu(W.TEMPERATURE, p) for p in W.pressures_isotherm(n)])
 
 
-PASS_NAMES = {0: 'k_zphi', 1: 'k_xweights', 2: 'k_xback', 3: 'k_ypass(5 fields)'}
+PASSES = (0, 1, 2, 3)
 
 
-def pass_bytes(nx, ny, nz):
+def chain_fields(plan):
+    """Fields that travel between the passes of the six-field functional:
+    five (n2, n3, n_v2 x and y, rho_bar; the z component of n_v2 is i kz
+    times n3 and is formed in the z pass), four when the WDA is linear and
+    applied in k space (rho_bar is never formed)."""
+    return 4 if plan.info['wda_kspace'] else 5
+
+
+def pass_names(nyf):
+    return {0: 'k_zphi', 1: 'k_xweights', 2: 'k_xback', 3: 'k_ypass(%d fields)' % nyf}
+
+
+def pass_bytes(nx, ny, nz, nyf):
     """Algorithmic HBM bytes per launch of each functional pass (one
-    instance) of the six-field functional: every complex half-spectrum
-    element read or written once; five fields travel between the passes
-    (the z component of n_v2 is i kz times n3 and is formed in the z pass)."""
+    instance) with ``nyf`` travelling fields: every complex half-spectrum
+    element read or written once.  k_xweights reads rho^ and writes the
+    fields; k_xback reads them (with a linear WDA: four of them and rho^) and
+    writes one."""
     nh = nx * ny * (nz // 2 + 1) * 16
-    return {0: 10 * nh, 1: 6 * nh, 2: 6 * nh, 3: 10 * nh}
+    return {0: 2 * nyf * nh, 1: (1 + nyf) * nh, 2: 6 * nh, 3: 2 * nyf * nh}
 
 
 def measured_traffic(kernel, n, nbatch=1):
@@ -88,7 +101,7 @@ def time_passes(plan, reps=20):
     """Average duration of each pass, HIP events on the plan's stream."""
     import torch
     out = {}
-    for which in PASS_NAMES:
+    for which in PASSES:
         plan.launch_pass(which)
         torch.cuda.synchronize()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -353,7 +366,8 @@ def passes_only(shape, reps=5, nbatch=1):
     plan.seed(rho, eng.V, False, eng.lam3)
     plan.derive()
     torch.cuda.synchronize()
-    for which in PASS_NAMES:
+    names = pass_names(chain_fields(plan))
+    for which in PASSES:
         for _ in range(reps):
             plan.launch_pass(which)
     torch.cuda.synchronize()
@@ -364,8 +378,9 @@ def passes_only(shape, reps=5, nbatch=1):
     torch.cuda.synchronize()
     t = time_passes(plan)
     print(json.dumps({'passes_only': True, 'grid': list(shape), 'nbatch': nbatch,
-                      'pass_ms': {PASS_NAMES[k]: v for k, v in t.items()},
-                      'pass_bytes': {PASS_NAMES[k]: nbatch * v for k, v in pass_bytes(*shape).items()},
+                      'pass_ms': {names[k]: v for k, v in t.items()},
+                      'pass_bytes': {names[k]: nbatch * v
+                                     for k, v in pass_bytes(*shape, chain_fields(plan)).items()},
                       'copy_bytes': a.numel() * 8}))
 
 
@@ -479,7 +494,9 @@ def main():
     plan = prog.grid.plan()
     tp = time_passes(plan)
     per_iter_weight = {0: 1, 1: 1, 2: 1, 3: 2}
-    pb = pass_bytes(*shape)
+    nyf = chain_fields(plan)
+    names = pass_names(nyf)
+    pb = pass_bytes(*shape, nyf)
     # the same passes as the batched isotherm launches them: one launch per
     # pass covering all nb instances of the batch plan (every instance
     # active); the roofline is taken from these when the batch ran
@@ -492,7 +509,11 @@ def main():
     dom = max(tpr, key=lambda w: tpr[w] * per_iter_weight[w])
     achieved = nbr * pb[dom] / (tpr[dom] * 1e-3) / 1e9
     npts = int(np.prod(shape))
-    b_iter = npts * (16 * 14 + 24) + npts * 8 * (3 * 5 + 2)
+    # real-field 3-D transforms per iteration: rho, the six weighted
+    # densities, their six derivatives, c1 (12 without rho_bar's pair when
+    # the WDA is applied in k space), each read and written once
+    ntr = 12 if nyf == 4 else 14
+    b_iter = npts * (16 * ntr + 24) + npts * 8 * (3 * 5 + 2)
     iter_gbps = b_iter * int(sum(int(v) for v in (biters if biters else iters))) / dt / 1e9
 
     adjoint = None if args.no_adjoint else adjoint_leg(shape)
@@ -520,9 +541,9 @@ def main():
                    'mode': 'batched' if batched_pps is not None else 'plain'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': PEAK_HBM_GBPS, 'unit': 'GB/s',
                      'frac': achieved / PEAK_HBM_GBPS,
-                     'traffic': (measured_traffic(PASS_NAMES[dom], shape[0], nbr) if shape[0] == shape[1] == shape[2]
+                     'traffic': (measured_traffic(names[dom], shape[0], nbr) if shape[0] == shape[1] == shape[2]
                                  else None),
-                     'kernel': PASS_NAMES[dom], 'instances_per_launch': nbr,
+                     'kernel': names[dom], 'instances_per_launch': nbr,
                      'avg_launch_ms': tpr[dom], 'algorithmic_bytes_per_launch': nbr * pb[dom]},
         'cpu_baseline': cpu,
         'plain_points_per_s': plain_pps,
@@ -536,9 +557,9 @@ def main():
         'all_converged': bool(all(convs) and all(bconvs)),
         'vext_ms': vext_ms,
         'vext_warm_ms': vext_warm_ms,
-        'pass_ms': {PASS_NAMES[k]: v for k, v in tp.items()},
-        'pass_ms_batched': ({PASS_NAMES[k]: v for k, v in tpr.items()} if nbr > 1 else None),
-        'pass_GBps_batched': ({PASS_NAMES[k]: nbr * pb[k] / (v * 1e-3) / 1e9 for k, v in tpr.items()}
+        'pass_ms': {names[k]: v for k, v in tp.items()},
+        'pass_ms_batched': ({names[k]: v for k, v in tpr.items()} if nbr > 1 else None),
+        'pass_GBps_batched': ({names[k]: nbr * pb[k] / (v * 1e-3) / 1e9 for k, v in tpr.items()}
                               if nbr > 1 else None),
         'adjoint': adjoint,
         'uneven_batch': uneven,

@@ -35,6 +35,7 @@ class CmmFunctional(C.Structure):
         ('n_eos', C.c_int),
         ('eos_kind', C.c_int * 4), ('eos_factor', C.c_double * 4), ('eos_par', (C.c_double * 16) * 4),
         ('R', C.c_double), ('D', C.c_double), ('kT', C.c_double), ('n3_series', C.c_double),
+        ('wda_pointwise', C.c_int),
     ]
 
 
@@ -211,10 +212,10 @@ class Plan:
 
     @property
     def info(self):
-        out = np.zeros(11, dtype=np.int64)
-        check(self.lib.cmm_plan_get_info(self.h, out.ctypes.data_as(C.c_void_p), 11))
-        return dict(zip(['nx', 'ny', 'nz', 'nbatch', 'tb_x', 'tb_y', 'p_z', 'p_phi', 'workspace_bytes', 'nzh', 'nf'],
-                        [int(v) for v in out]))
+        out = np.zeros(12, dtype=np.int64)
+        check(self.lib.cmm_plan_get_info(self.h, out.ctypes.data_as(C.c_void_p), 12))
+        return dict(zip(['nx', 'ny', 'nz', 'nbatch', 'tb_x', 'tb_y', 'p_z', 'p_phi', 'workspace_bytes', 'nzh', 'nf',
+                         'wda_kspace'], [int(v) for v in out]))
 
     @property
     def nfields(self):

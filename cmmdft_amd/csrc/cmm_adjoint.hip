@@ -42,6 +42,7 @@ struct AdjDev {
 // z pass of the Hessian product: NF weighted densities of v (x,y inverse-
 // transformed) -> C2R -> directional derivative of the pointwise derivative
 // fields at the weighted densities of rho -> R2C -> F[0..NF) in place.
+// NF = 5: a linear WDA, whose Hessian term the x passes apply in k space.
 template <int NF>
 __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long Fs, long long Ffs,
                                              const double* __restrict__ nfld, long long npts, GridDev g, FuncDev fp,
@@ -55,10 +56,11 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
-  z_load_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  constexpr bool VZ = (NF == 5 || NF == 6);
+  z_load_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft<1>(sm, t, g.fz, s_tw);
-  const double* nb = nfld + (long long)inst * NF * npts;
+  const double* nb = nfld + (long long)inst * (NF == 5 ? 6 : NF) * npts;   // six fields when NF = 5
   const int tot = t.P * n;
   const FastDiv fP = make_fastdiv((unsigned)t.P);
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
@@ -70,7 +72,6 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
     double& r2 = z_val(sm, t, 2, pl, e);
     double& r3 = z_val(sm, t, 3, pl, e);
     double& r4 = z_val(sm, t, 4, pl, e);
-    double& r5 = z_val(sm, t, 5, pl, e);
     double g0 = 0.0, g1 = 0.0, gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g5 = 0.0;
     double gt[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (fp.use_hs) {
@@ -93,17 +94,19 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
       gv1 = o.gv[1].d;
       gv2 = o.gv[2].d;
     }
-    if (fp.use_wda) {
-      Dual fw, dfw;
-      wda_point<Dual>(fp, Dual(nb[5 * npts + idx], r5), fw, dfw);
-      g5 = dfw.d;
+    if constexpr (NF != 5) {
+      if (fp.use_wda) {
+        Dual fw, dfw;
+        wda_point<Dual>(fp, Dual(nb[5 * npts + idx], z_val(sm, t, 5, pl, e)), fw, dfw);
+        g5 = dfw.d;
+      }
     }
     r0 = g0;
     r1 = g1;
     r2 = gv0;
     r3 = gv1;
     r4 = gv2;
-    r5 = g5;
+    if constexpr (NF != 5) z_val(sm, t, 5, pl, e) = g5;
     if constexpr (NF == 12) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) z_val(sm, t, 6 + c, pl, e) = gt[c];
@@ -111,7 +114,7 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   }
   __syncthreads();
   z_fft<-1>(sm, t, g.fz, s_tw);
-  z_store_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  z_store_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
 }
 
 // CG direction and its transform:

@@ -132,9 +132,9 @@ void default_tiles(cmm_plan* p) {
 RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->scal}; }
 
 // totals of the partials the previous kernel stored (see k_reduce_partials)
-int launch_reduce(cmm_plan* p, long long nblocks, int nq, unsigned maxmask, int slot) {
+int launch_reduce(cmm_plan* p, long long nblocks, int nq, unsigned maxmask, int slot, unsigned skipmask = 0u) {
   hipLaunchKernelGGL(k_reduce_partials, dim3(p->nb, nq), dim3(NT), 0, p->stream, red_of(p), (int)nblocks, maxmask,
-                     slot, p->active);
+                     slot, p->active, skipmask);
   hipError_t e_ = hipGetLastError();
   if (e_ != hipSuccess) return fail(CMM_E_HIP, std::string("k_reduce_partials: ") + hipGetErrorString(e_));
   return CMM_OK;
@@ -324,8 +324,9 @@ int cmm_plan_get_radix(cmm_plan* p, int axis, int* nstages, int* radices) {
 
 int cmm_plan_get_info(cmm_plan* p, long long* info, int n) {
   if (!p || !info) return fail(CMM_E_ARG, "null argument");
-  long long v[11] = {p->nx, p->ny, p->nz, p->nb, p->tbx, p->tby, p->pz, p->pphi, (long long)p->ws_bytes, p->nzh, p->nf};
-  for (int i = 0; i < n && i < 11; ++i) info[i] = v[i];
+  long long v[12] = {p->nx, p->ny, p->nz, p->nb, p->tbx, p->tby, p->pz, p->pphi, (long long)p->ws_bytes, p->nzh, p->nf,
+                     (p->nf == 6 && p->fp.wl2 != 0.0) ? 1 : 0};
+  for (int i = 0; i < n && i < 12; ++i) info[i] = v[i];
   return CMM_OK;
 }
 
@@ -368,6 +369,19 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
     d.eos_fac[t] = f->eos_factor[t];
     for (int k = 0; k < 16; ++k) d.eos_par[t][k] = f->eos_par[t][k];
     if (t < f->n_eos && (f->eos_kind[t] < 0 || f->eos_kind[t] > 2)) return fail(CMM_E_ARG, "unknown EOS kind");
+  }
+  // Every EOS term mean field: f_ex = A rho_bar^2 is quadratic, its
+  // derivative w_D * (2A rho_bar) linear in rho, and the x passes apply it in
+  // k space (2A w_D^2 rho^ / N, energy by Parseval) unless the caller asks
+  // for the pointwise route.  Six-field functionals only.
+  if (d.use_wda && nf == 6 && !f->wda_pointwise) {
+    bool lin = true;
+    double A = 0.0;
+    for (int t = 0; t < d.neos; ++t) {
+      lin = lin && d.eos_kind[t] == 0;
+      A += d.eos_fac[t] * d.eos_par[t][0];
+    }
+    if (lin) d.wl2 = 2.0 * A;
   }
   d.R = d.use_hs ? f->R : 1.0;
   if (!(f->n3_series >= 0.0 && f->n3_series <= 1e-3))
@@ -525,9 +539,13 @@ static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
 // them -- the z component of n_v2 (field 4) is i kz times n3 on both sides of
 // the y and x passes and is not carried (k_xweights, z_load_half_multi),
 // unless `full` asks for it (callers that read the weighted densities).
+// A linear WDA (wda_kspace) leaves out rho_bar's field as well: four fields.
+static bool wda_kspace(const cmm_plan* p) { return p->nf == 6 && p->fp.wl2 != 0.0; }
+
 static int launch_ypass_chain(cmm_plan* p, int sgn, bool full = false) {
   const long long is = (long long)p->fcap * p->nhalf;
-  if (p->nf == 6 && !full) return launch_ypass_is(p, p->F, is, 5, sgn, 4);
+  if (p->nf == 6 && !full)
+    return wda_kspace(p) ? launch_ypass_is(p, p->F, is, 4, sgn) : launch_ypass_is(p, p->F, is, 5, sgn, 4);
   return launch_ypass_is(p, p->F, is, p->nf, sgn);
 }
 
@@ -547,8 +565,10 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
 // x passes of the functional chain (see k_xweights / k_xback): tiles of
 // nx * tb columns with all fields in LDS (nl nx tb complex values for nl
 // fields, the nx twiddles and the reduction scratch, at most 64 KB together)
-// fields an x pass weights and transforms (XFields: four of the six)
-static int xfields(const cmm_plan* p) { return p->nf == 6 ? XFields<6>::nc : p->nf; }
+// fields an x pass weights and transforms (XFields: four of the six, three
+// with a linear WDA) in the chain's layout nfx (5, 6 or 12)
+static int xfields(int nfx) { return nfx == 6 ? XFields<6>::nc : (nfx == 5 ? XFields<5>::nc : nfx); }
+static int chain_nf(const cmm_plan* p, bool full = false) { return (wda_kspace(p) && !full) ? 5 : p->nf; }
 
 static int xpass_tb(cmm_plan* p, int nl) {
   const int lim = (65536 - CMM_RED_SH * 8 - 16 * p->nx) / (16 * nl * p->nx);
@@ -587,7 +607,8 @@ static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, b
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NC);
   const size_t lds = (size_t)(NC * T + p->nx) * sizeof(double2);
-  auto kern = xpass_full_occ(nt, lds + (MFA ? CMM_RED_SH * sizeof(double) : 0)) ? k_xweights<NF, MFA, XOCC_FULL>
+  const bool red = MFA || XFields<NF>::wlin;
+  auto kern = xpass_full_occ(nt, lds + (red ? CMM_RED_SH * sizeof(double) : 0)) ? k_xweights<NF, MFA, XOCC_FULL>
                                                                                  : k_xweights<NF, MFA, 1>;
   dim3 grid((unsigned)(8 * per), 1, p->nb);
   hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream,
@@ -596,26 +617,36 @@ static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, b
                      ntiles, p->active, red_of(p), full ? 1 : 0);
 }
 
+// Energies reduced here: the MFA energy into S_FMFA; with a linear WDA its
+// sum f_ex(rho_bar) into S_PHI + 1 (the z pass then leaves that slot alone).
 static int launch_xweights(cmm_plan* p, bool full = false) {
   const long long nb = (long long)p->ny * p->nzh;
-  const int tb = xpass_tb(p, xfields(p));
+  const int nfx = chain_nf(p, full);
+  const int tb = xpass_tb(p, xfields(nfx));
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
   const bool mfa = p->fp.kmfa != nullptr;
-  if (p->nf == 12) {
+  if (nfx == 12) {
     if (mfa) launch_xweights_nf<12, true>(p, tb, ntiles, per, full);
     else launch_xweights_nf<12, false>(p, tb, ntiles, per, full);
+  } else if (nfx == 5) {
+    if (mfa) launch_xweights_nf<5, true>(p, tb, ntiles, per, full);
+    else launch_xweights_nf<5, false>(p, tb, ntiles, per, full);
   } else {
     if (mfa) launch_xweights_nf<6, true>(p, tb, ntiles, per, full);
     else launch_xweights_nf<6, false>(p, tb, ntiles, per, full);
   }
   LAUNCHCHK("k_xweights");
-  return mfa ? launch_reduce(p, ntiles, 1, 0u, S_FMFA) : CMM_OK;
+  if (mfa) {
+    const int rc = launch_reduce(p, ntiles, 1, 0u, S_FMFA);
+    if (rc) return rc;
+  }
+  return nfx == 5 ? launch_reduce(p, ntiles, 2, 0u, S_PHI, 1u) : CMM_OK;
 }
 
 template <int NF, bool MFA>
 static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
-  constexpr int NL = XFields<NF>::nc + (MFA ? 1 : 0);
+  constexpr int NL = XFields<NF>::nc + ((MFA || XFields<NF>::wlin) ? 1 : 0);
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NL);
   const size_t lds = (size_t)(NL * T + p->nx) * sizeof(double2);
@@ -630,12 +661,16 @@ static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
 static int launch_xback(cmm_plan* p) {
   const long long nb = (long long)p->ny * p->nzh;
   const bool mfa = p->fp.kmfa != nullptr;
-  const int tb = xpass_tb(p, xfields(p) + (mfa ? 1 : 0));
+  const int nfx = chain_nf(p);
+  const int tb = xpass_tb(p, xfields(nfx) + ((mfa || nfx == 5) ? 1 : 0));
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
-  if (p->nf == 12) {
+  if (nfx == 12) {
     if (mfa) launch_xback_nf<12, true>(p, tb, ntiles, per);
     else launch_xback_nf<12, false>(p, tb, ntiles, per);
+  } else if (nfx == 5) {
+    if (mfa) launch_xback_nf<5, true>(p, tb, ntiles, per);
+    else launch_xback_nf<5, false>(p, tb, ntiles, per);
   } else {
     if (mfa) launch_xback_nf<6, true>(p, tb, ntiles, per);
     else launch_xback_nf<6, false>(p, tb, ntiles, per);
@@ -723,14 +758,18 @@ static int forward_chain(cmm_plan* p, bool full = false) {
 static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
   const int P = zphi_p(p);
   dim3 grid((unsigned)zblocks(p, P), p->nb);
+  const bool wl = wda_kspace(p);
   if (p->nf == 12)
     hipLaunchKernelGGL(k_zphi<12>, grid, dim3(NT), zlds(p, P, 12), p->stream, p->F, (long long)p->fcap * p->nhalf,
                        p->nhalf, p->g, p->fp, ztile(p, P, 12), store, p->active, red_of(p));
+  else if (wl)
+    hipLaunchKernelGGL(k_zphi<5>, grid, dim3(NT), zlds(p, P, 5), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                       p->nhalf, p->g, p->fp, ztile(p, P, 5), store, p->active, red_of(p));
   else
     hipLaunchKernelGGL(k_zphi<6>, grid, dim3(NT), zlds(p, P, 6), p->stream, p->F, (long long)p->fcap * p->nhalf,
                        p->nhalf, p->g, p->fp, ztile(p, P, 6), store, p->active, red_of(p));
   LAUNCHCHK("k_zphi");
-  return reduce ? launch_reduce(p, grid.x, 3, 1u << 2, S_PHI) : CMM_OK;
+  return reduce ? launch_reduce(p, grid.x, 3, 1u << 2, S_PHI, wl ? 2u : 0u) : CMM_OK;
 }
 
 // back-convolution of the nf derivative fields in F: inverse y, the x pass
@@ -761,6 +800,9 @@ int cmm_hessian_apply(cmm_plan* p, const double* nfld) {
   if (p->nf == 12)
     hipLaunchKernelGGL(k_zhess<12>, grid, dim3(NT), zlds(p, P, 12), p->stream, p->F, (long long)p->fcap * p->nhalf,
                        p->nhalf, nfld, p->npts, p->g, p->fp, ztile(p, P, 12), p->active);
+  else if (wda_kspace(p))
+    hipLaunchKernelGGL(k_zhess<5>, grid, dim3(NT), zlds(p, P, 5), p->stream, p->F, (long long)p->fcap * p->nhalf,
+                       p->nhalf, nfld, p->npts, p->g, p->fp, ztile(p, P, 5), p->active);
   else
     hipLaunchKernelGGL(k_zhess<6>, grid, dim3(NT), zlds(p, P, 6), p->stream, p->F, (long long)p->fcap * p->nhalf,
                        p->nhalf, nfld, p->npts, p->g, p->fp, ztile(p, P, 6), p->active);

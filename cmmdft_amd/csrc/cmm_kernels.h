@@ -52,6 +52,8 @@ struct FuncDev {
   double c_w2, c_w3, inv4piR2, inv4piR;
   double c_tor;     // 3 / R: tensor weight B = w2 - (3/R) w3
   const double2* kmfa;   // mean-field (MFA) kernel, half spectrum [nx][ny][nz/2+1]; null when off
+  double wl2;       // linear WDA (every EOS term mean field, f_ex = A rho_bar^2): 2A, its terms applied
+                    // in k space; 0 when the WDA takes the pointwise route
 };
 
 struct InstDev {

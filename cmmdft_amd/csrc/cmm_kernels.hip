@@ -443,11 +443,18 @@ constexpr int NTX = 512;
 // the z component is not stored between the passes at all, see
 // z_load_half_multi; vz = 1 stores it for callers that read the fields).
 // Slots 0..3 hold the fields n2, n3, nv2_x, rho_bar (weights 0, 1, 2, 5).
+// NF = 5: the six-field functional whose WDA is linear (every EOS term mean
+// field, f_ex = A rho_bar^2): rho_bar is not formed; its energy
+// A sum rho_bar^2 = (A/N) sum_k |w_D rho^|^2 and its derivative term
+// w_D * (2A rho_bar) = IFFT(2A w_D^2 rho^) are taken in k space by the x
+// passes, so slots 0..2 hold n2, n3, nv2_x and four fields travel through
+// the y and z passes instead of five.
 template <int NF>
 struct XFields {
-  static constexpr bool red = (NF == 6);
-  static constexpr int nc = red ? 4 : NF;   // fields weighted and transformed
-  __device__ static constexpr int field(int c) { return (red && c == 3) ? 5 : c; }
+  static constexpr bool red = (NF == 6 || NF == 5);
+  static constexpr bool wlin = (NF == 5);
+  static constexpr int nc = NF == 6 ? 4 : (NF == 5 ? 3 : NF);   // fields weighted and transformed
+  __device__ static constexpr int field(int c) { return (NF == 6 && c == 3) ? 5 : c; }
 };
 
 // ky and kz of a column, zero on their axis' Nyquist plane
@@ -463,7 +470,10 @@ __device__ __forceinline__ void column_k(const GridDev& g, int wy, int wz, doubl
 // MFA: also the mean-field energy (1/2) sum_x rho c_MFA = (1/2) sum_k
 // |rho^(k)|^2 K(k) over the full spectrum (K Hermitian, so the sum is real):
 // on the half spectrum the planes kz = 0 and kz = Nyquist count once, the
-// others twice.  Block partials are indexed by tile (ntiles of them).
+// others twice.  NF = 5 (linear WDA): also sum_x f_ex(rho_bar) =
+// (2A/N) sum_half |rho^|^2 w_D^2 with the same plane weights.  Block partials
+// (quantities: the MFA energy, then the WDA energy) are indexed by tile
+// (ntiles of them).
 template <int NF, bool MFA, int OCC>
 __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F, long long Fs, long long Ffs,
@@ -505,7 +515,8 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
   }
   __syncthreads();
   lds_fft<-1, true, VPT>(A, TB, fTB, 1, S6, g.fx, s_tw);
-  double emfa = 0.0;
+  constexpr bool E = MFA || XF::wlin;   // energies reduced here
+  double emfa = 0.0, ewl = 0.0;
   // each element's weighted copies overwrite only that element's own slots
   // (its rho^ among them), and one thread reads and writes them
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
@@ -515,20 +526,22 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     const int wz = (int)bb - wy * g.nzh;
     const KW w = kweights(g, fp, x, wy, wz);
     const double2 a = A[x * S6 + b];
-    if (MFA && b < tbv) {
+    if (E && b < tbv) {
       const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
-      emfa += half * (a.x * a.x + a.y * a.y) * fp.kmfa[(long long)x * nb + bb].x;
+      const double a2 = half * (a.x * a.x + a.y * a.y);
+      if (MFA) emfa += a2 * fp.kmfa[(long long)x * nb + bb].x;
+      if (XF::wlin) ewl += a2 * (w.wD * w.wD);
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       B[x * S6 + c * TB + b] = cscale(apply_w(g, w, x, wy, wz, XF::field(c), a, -1), g.inv_npts);
   }
-  if (MFA) {
+  if (E) {
     __shared__ double sh[CMM_RED_SH];
-    double acc[1] = {emfa};
-    const int ops[1] = {0};
-    block_reduce<1>(acc, ops, sh);
-    grid_reduce_store<1>(acc, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
+    double acc[2] = {emfa, ewl * fp.wl2 * g.inv_npts};
+    const int ops[2] = {0, 0};
+    block_reduce<2>(acc, ops, sh);
+    grid_reduce_store<2>(acc, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
   }
   __syncthreads();
   lds_fft<1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
@@ -551,7 +564,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       dst[2 * Ffs + o] = B[x * S6 + 2 * TB + b];
       dst[3 * Ffs + o] = cmul_i(cscale(n3, kyp), -1.0);
       if (vz) dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
-      dst[5 * Ffs + o] = B[x * S6 + 3 * TB + b];
+      if constexpr (!XF::wlin) dst[5 * Ffs + o] = B[x * S6 + 3 * TB + b];
     }
   } else {
 #pragma unroll
@@ -570,6 +583,8 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
 // tile of every field before it writes field 0's tile).
 // MFA: the density's spectrum (rz, z and y transformed) is loaded as one more
 // field and transformed with the others; K rho^ joins the weighted sum.
+// NF = 5 (linear WDA): rho^ is loaded likewise and (2A/N) w_D^2 rho^ joins
+// the sum in place of the back-convolved WDA derivative field.
 template <int NF, bool MFA, int OCC>
 __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB, FastDiv fTB,
@@ -577,7 +592,8 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
         const double2* __restrict__ rz, long long rzs) {
   using XF = XFields<NF>;
   constexpr int NC = XF::nc;
-  constexpr int NL = NC + (MFA ? 1 : 0);    // fields held in LDS
+  constexpr bool LR = MFA || XF::wlin;      // rho^ joins the fields
+  constexpr int NL = NC + (LR ? 1 : 0);     // fields held in LDS
   extern __shared__ double2 sm[];    // [NL T] fields, then the nx twiddles
   double2* s_tw = sm + NL * g.nx * TB;
   load_twiddles(s_tw, g.fx);
@@ -592,7 +608,7 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = F + inst * Fs + b0;
-  const double2* rsrc = MFA ? rz + inst * rzs + b0 : nullptr;
+  const double2* rsrc = LR ? rz + inst * rzs + b0 : nullptr;
   if constexpr (XF::red) {
     // one element's five fields (and rho^) per thread, loads in flight
     // together; G_n3 + i ky' G_vy formed before the transform (the z pass
@@ -600,9 +616,9 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
       const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
       const long long o = (long long)x * nb + min(b, tbv - 1);
-      const double2 v0 = src[o], v1 = src[Ffs + o], v2 = src[2 * Ffs + o], v3 = src[3 * Ffs + o],
-                    v5 = src[5 * Ffs + o];
-      const double2 r = MFA ? rsrc[o] : make_double2(0.0, 0.0);
+      const double2 v0 = src[o], v1 = src[Ffs + o], v2 = src[2 * Ffs + o], v3 = src[3 * Ffs + o];
+      const double2 v5 = XF::wlin ? make_double2(0.0, 0.0) : src[5 * Ffs + o];
+      const double2 r = LR ? rsrc[o] : make_double2(0.0, 0.0);
       const unsigned bb = (unsigned)min(b0 + b, nb - 1);
       const int wy = (int)fdiv(bb, fnzh);
       const int wz = (int)bb - wy * g.nzh;
@@ -615,8 +631,8 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
       B[x * S6 + b] = make_double2(in ? v0.x : 0.0, in ? v0.y : 0.0);
       B[x * S6 + TB + b] = make_double2(in ? h.x : 0.0, in ? h.y : 0.0);
       B[x * S6 + 2 * TB + b] = make_double2(in ? v2.x : 0.0, in ? v2.y : 0.0);
-      B[x * S6 + 3 * TB + b] = make_double2(in ? v5.x : 0.0, in ? v5.y : 0.0);
-      if (MFA) B[x * S6 + 4 * TB + b] = make_double2(in ? r.x : 0.0, in ? r.y : 0.0);
+      if constexpr (!XF::wlin) B[x * S6 + 3 * TB + b] = make_double2(in ? v5.x : 0.0, in ? v5.y : 0.0);
+      if (LR) B[x * S6 + NC * TB + b] = make_double2(in ? r.x : 0.0, in ? r.y : 0.0);
     }
   } else {
     for (int i0 = threadIdx.x; i0 < NL * T; i0 += LU * blockDim.x) {
@@ -652,7 +668,11 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
 #pragma unroll
     for (int c = 0; c < NC; ++c)
       acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, XF::field(c), B[x * S6 + c * TB + b], 1), g.inv_npts));
-    if (MFA) acc = cadd(acc, cmul(B[x * S6 + NC * TB + b], fp.kmfa[(long long)x * nb + bb]));
+    if (LR) {
+      const double2 r = B[x * S6 + NC * TB + b];
+      if (MFA) acc = cadd(acc, cmul(r, fp.kmfa[(long long)x * nb + bb]));
+      if (XF::wlin) acc = cadd(acc, cscale(r, fp.wl2 * g.inv_npts * (w.wD * w.wD)));
+    }
     B[x * S6 + b] = acc;    // this element's slots are read above by this thread only
   }
   __syncthreads();
@@ -908,7 +928,9 @@ __device__ __forceinline__ void wda_point(const FuncDev& fp, T rb, T& f, T& df) 
 
 // Pointwise stage of the functional z pass at one point: r holds the NF
 // weighted densities (n3 unclipped) on entry and the NF derivative fields on
-// exit; acc gets Phi, f_ex(rho_bar) and n3 (for the max).
+// exit; acc gets Phi, f_ex(rho_bar) and n3 (for the max).  NF = 5: the
+// six-field functional with a linear WDA (taken in k space by the x passes),
+// without rho_bar.
 template <int NF>
 __device__ __forceinline__ void zphi_point(const FuncDev& fp, double (&r)[NF], double (&acc)[3]) {
   double gv0 = 0.0, gv1 = 0.0, gv2 = 0.0, g0 = 0.0, g1 = 0.0, g5 = 0.0;
@@ -930,18 +952,20 @@ __device__ __forceinline__ void zphi_point(const FuncDev& fp, double (&r)[NF], d
     gv1 = o.gv[1];
     gv2 = o.gv[2];
   }
-  if (fp.use_wda) {
-    double fw, dfw;
-    wda_point<double>(fp, r[5], fw, dfw);
-    acc[1] = fw;
-    g5 = dfw;
+  if constexpr (NF != 5) {
+    if (fp.use_wda) {
+      double fw, dfw;
+      wda_point<double>(fp, r[5], fw, dfw);
+      acc[1] = fw;
+      g5 = dfw;
+    }
   }
   r[0] = g0;
   r[1] = g1;
   r[2] = gv0;
   r[3] = gv1;
   r[4] = gv2;
-  r[5] = g5;
+  if constexpr (NF != 5) r[5] = g5;
   if constexpr (NF == 12) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) r[6 + c] = gt[c];
@@ -950,7 +974,8 @@ __device__ __forceinline__ void zphi_point(const FuncDev& fp, double (&r)[NF], d
 
 // z pass of the functional: NF weighted densities (x,y inverse-transformed)
 // -> C2R -> pointwise Phi / WDA derivatives -> R2C -> F[0..NF) in place
-// (NF = 12: the tensor variant's six more fields).
+// (NF = 12: the tensor variant's six more fields; NF = 5: the six-field
+// functional with its linear WDA left to the x passes).
 // Reduces sum Phi, sum f_ex(rho_bar), max n3 (unclipped) per instance.
 template <int NF>
 __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp,
@@ -965,7 +990,8 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
-  z_load_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  constexpr bool VZ = (NF == 5 || NF == 6);   // n_v2's z component formed from n3 here
+  z_load_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft<1, true>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
@@ -990,7 +1016,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   if (store) {
     __syncthreads();
     z_fft<-1, true>(sm, t, g.fz, s_tw);
-    z_store_half_multi<NF == 6>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+    z_store_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
@@ -1312,11 +1338,13 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
 // wavefront tree, then the waves in order).  One block per (instance,
 // quantity); each thread issues its loads in groups of 8 before summing them
 // in order, so the latency of one load is paid per group, not per partial.
+// Quantity q goes to slot + q; quantities with bit q of skipmask are not
+// stored (their slots keep what another pass put there).
 __global__ void __launch_bounds__(NT) k_reduce_partials(RedDev red, int nblocks, unsigned maxmask, int slot,
-                                                       const int* __restrict__ active) {
+                                                       const int* __restrict__ active, unsigned skipmask) {
   __shared__ double sh[NT / 64];
   const int inst = blockIdx.x, q = blockIdx.y;
-  if (active && !active[inst]) return;
+  if ((active && !active[inst]) || ((skipmask >> q) & 1u)) return;
   const double* part = red.partials + (long long)inst * red.pstride + (long long)q * nblocks;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   const bool mx = (maxmask >> q) & 1u;

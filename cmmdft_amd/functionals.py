@@ -63,6 +63,9 @@ def build_functional(hs=None, wda=None, kT=None):
             for k, v in enumerate(par):
                 f.eos_par[i][k] = v
         f.D = wda.D
+        # a WDA of mean-field EOS terms only is linear in rho: the device
+        # applies it in k space unless the part asks for the pointwise route
+        f.wda_pointwise = 0 if getattr(wda, 'kspace', True) else 1
     f.kT = kT
     return f
 
@@ -270,15 +273,22 @@ class WDAVFunctional(Functional):
     (functionals.py:1116-1184)."""
     name = 'WDA-V'
 
-    def __init__(self, grid, Rhs, eos):
+    def __init__(self, grid, Rhs, eos, kspace=True):
+        """``kspace``: when every EOS term is mean field (f_ex = A rho_bar^2,
+        the MFA EOS) the functional is quadratic and its derivative
+        w_D * 2A rho_bar is linear in rho; the device then applies it in k
+        space (2A w_D^2 rho^) without forming rho_bar.  False keeps the
+        pointwise route that every other EOS takes.  The two agree to
+        rounding."""
         self.temperature = None
         self.grid = grid
         self.eos = eos
         self.R = Rhs
+        self.kspace = bool(kspace)
         self._engine = _PartEngine(grid)
 
     def copy(self, grid=None):
-        return type(self)(grid if grid is not None else self.grid.copy(), self.R, self.eos)
+        return type(self)(grid if grid is not None else self.grid.copy(), self.R, self.eos, self.kspace)
 
     def set_temperature(self, temperature, Rhs, **kwargs):
         self.temperature = temperature
@@ -671,8 +681,8 @@ class FreeEnergy:
         hs.set_temperature(self.temperature, self.system.guest.Rhs)
         self.add_part(hs)
 
-    def add_wdav(self, eos, **kwargs):
-        wda = WDAVFunctional(self.grid, self.system.guest.Rhs, eos)
+    def add_wdav(self, eos, kspace=True, **kwargs):
+        wda = WDAVFunctional(self.grid, self.system.guest.Rhs, eos, kspace=kspace)
         wda.set_temperature(self.temperature, self.system.guest.Rhs)
         self.add_part(wda)
 

@@ -108,6 +108,10 @@ typedef struct {
                         derivatives by their second-order series where
                         n3 <= n3_series; 1e-8 is the reference's
                         (functionals.py:726-767), at most 1e-3 */
+  int wda_pointwise; /* 0: a WDA whose EOS terms are all mean field (f_ex =
+                        A rho_bar^2, linear derivative) is applied in k space
+                        by the x passes, its weighted density never formed;
+                        1: the pointwise route of every other WDA */
 } cmm_functional;
 
 int cmm_version(void);
@@ -128,8 +132,9 @@ int cmm_plan_set_tiles(cmm_plan* plan, int tb_x, int tb_y, int p_z, int p_phi);
  * the default is 5s, 3s, 8s, 4s, 2s) */
 int cmm_plan_set_radix(cmm_plan* plan, int axis, int nstages, const int* radices);
 int cmm_plan_get_radix(cmm_plan* plan, int axis, int* nstages, int* radices);
-/* info[0..10] = nx, ny, nz, nbatch, tb_x, tb_y, p_z, p_phi, workspace bytes, nzh,
- * nf (weighted-density fields of the functional set: 6 or 12) */
+/* info[0..11] = nx, ny, nz, nbatch, tb_x, tb_y, p_z, p_phi, workspace bytes, nzh,
+ * nf (weighted-density fields of the functional set: 6 or 12), 1 when the
+ * functional's WDA is applied in k space (see wda_pointwise) */
 int cmm_plan_get_info(cmm_plan* plan, long long* info, int n);
 /* device int[nbatch] mask of instances to process (NULL = all) */
 int cmm_plan_set_active(cmm_plan* plan, const int* dev_mask);

@@ -28,13 +28,28 @@ def test_library_exports_every_declared_symbol():
     assert lib.cmm_version() >= 10000
 
 
-def test_functional_struct_layout_matches_header():
+def test_functional_struct_layout_matches_header(tmp_path):
+    """The ctypes mirror of cmm_functional against the C compiler's layout
+    of the header's struct: size and every member's offset."""
     import ctypes as C
+    import shutil
+    import subprocess
     from cmmdft_amd._lib import CmmFunctional
-    # 5 ints + int[4] (+ 4 bytes of padding) + double[4] + double[4][16] + 3 doubles
-    assert C.sizeof(CmmFunctional) == 5 * 4 + 4 * 4 + 4 + 8 * 4 + 8 * 64 + 4 * 8
-    assert CmmFunctional.eos_factor.offset == 40
-    assert [n for n, _ in CmmFunctional._fields_][:5] == ['hs_base', 'hs_antisym', 'hs_tensor', 'use_wda', 'n_eos']
+    names = [n for n, _ in CmmFunctional._fields_]
+    assert names[:5] == ['hs_base', 'hs_antisym', 'hs_tensor', 'use_wda', 'n_eos']
+    cc = shutil.which('gcc') or shutil.which('cc')
+    if cc is None:
+        pytest.skip('no C compiler')
+    src = tmp_path / 'layout.c'
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "cmmdft_hip.h"\nint main(void) {\n'
+                   '  printf("%zu\\n", sizeof(cmm_functional));\n' +
+                   ''.join('  printf("%%zu\\n", offsetof(cmm_functional, %s));\n' % n for n in names) +
+                   '  return 0;\n}\n')
+    exe = tmp_path / 'layout'
+    subprocess.run([cc, '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert out[0] == C.sizeof(CmmFunctional)
+    assert out[1:] == [getattr(CmmFunctional, n).offset for n in names]
 
 
 def test_api_rejects_bad_arguments_without_gpu():
