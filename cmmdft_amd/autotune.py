@@ -153,7 +153,10 @@ def _tile_candidates(plan):
     even = [-(-nzh // k) for k in range(1, 9)]
     tby = sorted({t for t in [1, 2, 4, 8, 16, 32] + even if t * ny <= budget and t * ny * 16 <= 65536})
     pz = [p for p in (2, 4, 8, 16, 32) if (p // 2) * nz <= budget and nz * ((p // 2) | 1) * 16 <= 65536]
-    pphi = [p for p in (2, 4, 6, 8, 10, 12, 16) if 3 * p * nz <= budget and nz * ((3 * p) | 1) * 16 <= 65536]
+    if nz == 128:
+        pphi = [2, 4, 6, 8]   # the compile-time z tiles of the functional pass
+    else:
+        pphi = [p for p in (2, 4, 6, 8, 10, 12, 16) if 3 * p * nz <= budget and nz * ((3 * p) | 1) * 16 <= 65536]
     return tbx, tby, pz, pphi
 
 

@@ -292,6 +292,62 @@ __device__ __forceinline__ void stockham_stage_c(double2* buf, int batch, const 
   __syncthreads();
 }
 
+// stockham_stage_c with the sequence count BATCH (the fast index: bs = 1)
+// and the element stride ES fixed too, for blocks of NTB threads: every LDS
+// offset of a butterfly is an immediate, the butterfly index splits by
+// constant divisions, and each thread takes ceil(BATCH N / R / NTB)
+// butterflies.  Same operations in the same order as stockham_stage, so the
+// same results bit for bit.
+template <int R, int SGN, int N, int NS, int BATCH, int ES, int NTB>
+__device__ __forceinline__ void stockham_stage_t(double2* buf, const double2* tws) {
+  constexpr int nq = N / R;
+  constexpr int twstride = N / (NS * R);
+  constexpr int total = BATCH * nq;
+  constexpr int NB = (total + NTB - 1) / NTB;
+  const int tid = threadIdx.x;
+  double2 v[NB][R];
+  int dst[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    const int b = tid + i * NTB;
+    dst[i] = -1;
+    if (b < total) {
+      const int j = (int)((unsigned)b / (unsigned)BATCH), s = b - j * BATCH;
+      const int jn = (int)((unsigned)j / (unsigned)NS), k = j - jn * NS;
+      const double2* base = buf + s + j * ES;
+#pragma unroll
+      for (int t = 0; t < R; ++t) v[i][t] = base[t * nq * ES];
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int t = 1; t < R; ++t) {
+          double2 w = tws[t * k * twstride];
+          if (SGN > 0) w.y = -w.y;
+          v[i][t] = cmul(v[i][t], w);
+        }
+      }
+      dftr<R, SGN>(v[i]);
+      dst[i] = s + (jn * NS * R + k) * ES;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    if (dst[i] >= 0) {
+#pragma unroll
+      for (int t = 0; t < R; ++t) buf[dst[i] + t * NS * ES] = v[i][t];
+    }
+  }
+  __syncthreads();
+}
+
+// A whole transform of BATCH sequences (radices RS in order) with the tile
+// fixed at compile time; ends with a barrier.
+template <int SGN, int N, int BATCH, int ES, int NTB, int NS, int R, int... RS>
+__device__ __forceinline__ void lds_fft_t(double2* buf, const double2* tws) {
+  stockham_stage_t<R, SGN, N, NS, BATCH, ES, NTB>(buf, tws);
+  if constexpr (sizeof...(RS) > 0) lds_fft_t<SGN, N, BATCH, ES, NTB, NS * R, RS...>(buf, tws);
+}
+
 template <int... RS>
 __device__ __forceinline__ bool same_radices(const Fft1D& p) {
   constexpr int rs[] = {RS...};

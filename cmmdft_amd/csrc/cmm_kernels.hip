@@ -213,6 +213,111 @@ __device__ __forceinline__ void z_fft(double2* lds, const ZTile& t, const Fft1D&
 }
 
 // ---------------------------------------------------------------------------
+// The same z tile with its shape fixed at compile time (P pencils, NF fields,
+// length N): the loaders, the storer and the transforms below do the
+// operations of their runtime-tile counterparts above in the same order, with
+// LDS offsets that are immediates and index splits by constant divisions.
+// Blocks of NT threads.
+// ---------------------------------------------------------------------------
+template <int P_, int NF_, int N_>
+struct ZTileC {
+  static constexpr int P = P_, NF = NF_, N = N_;
+  static constexpr int B = P_ / 2, S = NF_ * (P_ / 2), ES = (NF_ * (P_ / 2)) | 1, NZH = N_ / 2 + 1;
+};
+
+template <class TC>
+__device__ __forceinline__ double& z_val_c(double2* lds, int f, int pl, int e) {
+  return reinterpret_cast<double*>(lds)[2 * (e * TC::ES + f * TC::B + (pl >> 1)) + (pl & 1)];
+}
+
+template <bool VZ, class TC>
+__device__ __forceinline__ void z_load_half_multi_c(double2* lds, const double2* src, long long fs, long long p0,
+                                                    long long npen, const double* kzk, int kzn) {
+  constexpr int n = TC::N, nzh = TC::NZH;
+  constexpr int per = TC::B * nzh;
+  constexpr int tot = TC::NF * per;
+  const int ip0 = (int)p0, ilast = (int)npen - 1;
+  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * NT) {
+    double2 A[LU], B[LU];
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = min(i0 + u * NT, tot - 1);
+      const int f = (int)((unsigned)i / (unsigned)per);
+      const int r = i - f * per;
+      const int q = (int)((unsigned)r / (unsigned)nzh), k = r - q * nzh;
+      const int pa = ip0 + 2 * q;
+      const double2* sf = src + ((VZ && f == 4) ? 1 : f) * fs;
+      A[u] = sf[min(pa, ilast) * nzh + k];
+      B[u] = sf[min(pa + 1, ilast) * nzh + k];
+    }
+#pragma unroll
+    for (int u = 0; u < LU; ++u) {
+      const int i = i0 + u * NT;
+      if (i < tot) {
+        const int f = (int)((unsigned)i / (unsigned)per);
+        const int r = i - f * per;
+        const int q = (int)((unsigned)r / (unsigned)nzh), k = r - q * nzh;
+        const int pa = ip0 + 2 * q;
+        double2 a = (pa <= ilast) ? A[u] : make_double2(0.0, 0.0);
+        double2 b = (pa + 1 <= ilast) ? B[u] : make_double2(0.0, 0.0);
+        if (VZ && f == 4) {
+          const double kk = kz_of(kzk, kzn, k);
+          a = cmul_i(cscale(a, kk), -1.0);
+          b = cmul_i(cscale(b, kk), -1.0);
+        }
+        const bool selfconj = (k == 0) || (2 * k == n);
+        if (selfconj) { a.y = 0.0; b.y = 0.0; }
+        lds[k * TC::ES + f * TC::B + q] = make_double2(a.x - b.y, a.y + b.x);
+        if (!selfconj) lds[(n - k) * TC::ES + f * TC::B + q] = make_double2(a.x + b.y, b.x - a.y);
+      }
+    }
+  }
+}
+
+template <class TC>
+__device__ __forceinline__ void z_unpack_c(const double2* lds, int f, int q, int k, double2& A, double2& Bv) {
+  const double2 zk = lds[k * TC::ES + f * TC::B + q];
+  const double2 zm = lds[(k == 0 ? 0 : TC::N - k) * TC::ES + f * TC::B + q];
+  A = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+  Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
+}
+
+template <bool VZ, class TC>
+__device__ __forceinline__ void z_store_half_multi_c(const double2* lds, double2* dst, long long fs, long long p0,
+                                                     long long npen, const double* kzk, int kzn) {
+  constexpr int nzh = TC::NZH;
+  constexpr int per = TC::B * nzh;
+  constexpr int tot = (VZ ? TC::NF - 1 : TC::NF) * per;
+#pragma unroll 4
+  for (int i = threadIdx.x; i < tot; i += NT) {
+    const int fs0 = (int)((unsigned)i / (unsigned)per);
+    const int r = i - fs0 * per;
+    const int f = (VZ && fs0 >= 4) ? fs0 + 1 : fs0;
+    const int q = (int)((unsigned)r / (unsigned)nzh), k = r - q * nzh;
+    double2 A, Bv;
+    z_unpack_c<TC>(lds, f, q, k, A, Bv);
+    if (VZ && f == 1) {
+      double2 A4, B4;
+      z_unpack_c<TC>(lds, 4, q, k, A4, B4);
+      const double kk = kz_of(kzk, kzn, k);
+      A = cadd(A, cmul_i(cscale(A4, kk), 1.0));
+      Bv = cadd(Bv, cmul_i(cscale(B4, kk), 1.0));
+    }
+    const int pa = (int)p0 + 2 * q, o = pa * nzh + k;
+    double2* df = dst + f * fs;
+    if (pa < (int)npen) df[o] = A;
+    if (pa + 1 < (int)npen) df[o + nzh] = Bv;
+  }
+}
+
+// all S sequences of the tile, length 128 as radix 16 then 8 (z_fft<SGN, true>)
+template <int SGN, class TC>
+__device__ __forceinline__ void z_fft_c(double2* lds, const double2* tws) {
+  static_assert(TC::N == 128, "compile-time z tiles are built for length 128");
+  lds_fft_t<SGN, 128, TC::S, TC::ES, NT, 1, 16, 8>(lds, tws);
+}
+
+// ---------------------------------------------------------------------------
 // Plain transform passes (3-D FFT entry points and building blocks)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ x, long long xs, double2* __restrict__ X, long long Xs,
@@ -1017,6 +1122,52 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
     __syncthreads();
     z_fft<-1, true>(sm, t, g.fz, s_tw);
     z_store_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  }
+  const int ops[3] = {0, 0, 1};
+  block_reduce<3>(acc, ops, sh);
+  grid_reduce_store<3>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
+}
+
+// k_zphi for nz = 128 with the tile fixed at compile time (P pencils): the
+// same operations in the same order, so the same results bit for bit at the
+// same P.  Its LDS addressing and index arithmetic are constants, which cuts
+// the integer work that otherwise issues beside the FP64 work.
+template <int NF, int P>
+__global__ void __launch_bounds__(NT) k_zphi128(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
+                                               FuncDev fp, int store, const int* __restrict__ active, RedDev red) {
+  using TC = ZTileC<P, NF, 128>;
+  extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[128];
+  load_twiddles(s_tw, g.fz);
+  __shared__ double sh[CMM_RED_SH];
+  const int inst = blockIdx.y;
+  if (active && !active[inst]) return;
+  const long long npen = (long long)g.nx * g.ny;
+  const long long p0 = (long long)blockIdx.x * P;
+  constexpr bool VZ = (NF == 5 || NF == 6);
+  z_load_half_multi_c<VZ, TC>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
+  __syncthreads();
+  z_fft_c<1, TC>(sm, s_tw);
+  double acc[3] = {0.0, 0.0, -INFINITY};
+  constexpr int tot = P * 128;
+  for (int i = threadIdx.x; i < tot; i += NT) {
+    const int e = (int)((unsigned)i / (unsigned)P), pl = i - e * P;
+    if (p0 + pl >= npen) continue;
+    double r[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) r[f] = z_val_c<TC>(sm, f, pl, e);
+    double a[3] = {0.0, 0.0, -INFINITY};
+    zphi_point<NF>(fp, r, a);
+    acc[0] += a[0];
+    acc[1] += a[1];
+    acc[2] = fmax(acc[2], a[2]);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) z_val_c<TC>(sm, f, pl, e) = r[f];
+  }
+  if (store) {
+    __syncthreads();
+    z_fft_c<-1, TC>(sm, s_tw);
+    z_store_half_multi_c<VZ, TC>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
