@@ -568,14 +568,14 @@ static int launch_xpass(cmm_plan* p, double2* F, long long is, int sgn) {
   return CMM_OK;
 }
 
-// x passes of the functional chain (see k_xweights / k_xback): tiles of
-// nx * tb columns with all fields in LDS (nl nx tb complex values for nl
-// fields, the nx twiddles and the reduction scratch, at most 64 KB together)
 // fields an x pass weights and transforms (XFields: four of the six, three
 // with a linear WDA) in the chain's layout nfx (5, 6 or 12)
 static int xfields(int nfx) { return nfx == 6 ? XFields<6>::nc : (nfx == 5 ? XFields<5>::nc : nfx); }
 static int chain_nf(const cmm_plan* p, bool full = false) { return (wda_kspace(p) && !full) ? 5 : p->nf; }
 
+// x passes of the functional chain (see k_xweights / k_xback): tiles of
+// nx * tb columns with all fields in LDS (nl nx tb complex values for nl
+// fields, the nx twiddles and the reduction scratch, at most 64 KB together)
 static int xpass_tb(cmm_plan* p, int nl) {
   const int lim = (65536 - CMM_RED_SH * 8 - 16 * p->nx) / (16 * nl * p->nx);
   return std::max(1, std::min(p->tbx, lim));
@@ -607,6 +607,16 @@ static bool xpass_full_occ(unsigned nt, size_t lds_bytes) {
   return nt == (unsigned)NTX && 3 * lds_bytes <= CU_LDS;
 }
 
+// The compile-time x tile (k_xweights / k_xback with TBC = tb): nx = 128
+// with the radix sequence 8, 4, 4 and two or four columns; 0 = runtime tile.
+// Its blocks are 128 tb threads, which is what xpass_threads gives for the
+// five- and six-field chains at those widths.
+static int xpass_tbc(const cmm_plan* p, int tb) {
+  const Fft1D& f = p->g.fx;
+  if (p->nx != 128 || f.nst != 3 || f.rad[0] != 8 || f.rad[1] != 4 || f.rad[2] != 4) return 0;
+  return (tb == 2 || tb == 4) ? tb : 0;
+}
+
 template <int NF, bool MFA>
 static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, bool full) {
   constexpr int NC = XFields<NF>::nc;
@@ -614,8 +624,13 @@ static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, b
   const unsigned nt = xpass_threads(T, NC);
   const size_t lds = (size_t)(NC * T + p->nx) * sizeof(double2);
   const bool red = MFA || XFields<NF>::wlin;
-  auto kern = xpass_full_occ(nt, lds + (red ? CMM_RED_SH * sizeof(double) : 0)) ? k_xweights<NF, MFA, XOCC_FULL>
-                                                                                 : k_xweights<NF, MFA, 1>;
+  const bool occ = xpass_full_occ(nt, lds + (red ? CMM_RED_SH * sizeof(double) : 0));
+  auto kern = occ ? k_xweights<NF, MFA, XOCC_FULL> : k_xweights<NF, MFA, 1>;
+  if constexpr (XFields<NF>::red) {
+    const int tbc = xpass_tbc(p, tb);
+    if (tbc == 4) kern = occ ? k_xweights<NF, MFA, XOCC_FULL, 4> : k_xweights<NF, MFA, 1, 4>;
+    if (tbc == 2) kern = occ ? k_xweights<NF, MFA, XOCC_FULL, 2> : k_xweights<NF, MFA, 1, 2>;
+  }
   dim3 grid((unsigned)(8 * per), 1, p->nb);
   hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream,
                      p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
@@ -656,7 +671,13 @@ static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NL);
   const size_t lds = (size_t)(NL * T + p->nx) * sizeof(double2);
-  auto kern = xpass_full_occ(nt, lds) ? k_xback<NF, MFA, XOCC_FULL> : k_xback<NF, MFA, 1>;
+  const bool occ = xpass_full_occ(nt, lds);
+  auto kern = occ ? k_xback<NF, MFA, XOCC_FULL> : k_xback<NF, MFA, 1>;
+  if constexpr (XFields<NF>::red) {
+    const int tbc = xpass_tbc(p, tb);
+    if (tbc == 4) kern = occ ? k_xback<NF, MFA, XOCC_FULL, 4> : k_xback<NF, MFA, 1, 4>;
+    if (tbc == 2) kern = occ ? k_xback<NF, MFA, XOCC_FULL, 2> : k_xback<NF, MFA, 1, 2>;
+  }
   dim3 grid((unsigned)(8 * per), 1, p->nb);
   hipLaunchKernelGGL(kern, grid, dim3(nt), lds, p->stream, p->F,
                      (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb, make_fastdiv((unsigned)tb),

@@ -554,6 +554,12 @@ constexpr int NTX = 512;
 // w_D * (2A rho_bar) = IFFT(2A w_D^2 rho^) are taken in k space by the x
 // passes, so slots 0..2 hold n2, n3, nv2_x and four fields travel through
 // the y and z passes instead of five.
+//
+// TBC > 0: the tile fixed at compile time -- nx = 128 with the radix sequence
+// 8, 4, 4 and TB = TBC columns, in blocks of 128 TBC threads -- so LDS
+// offsets are immediates and index splits constant divisions; the same
+// operations in the same order as the runtime tile (TBC = 0) with that radix
+// sequence and TB, so the same results bit for bit.
 template <int NF>
 struct XFields {
   static constexpr bool red = (NF == 6 || NF == 5);
@@ -561,6 +567,29 @@ struct XFields {
   static constexpr int nc = NF == 6 ? 4 : (NF == 5 ? 3 : NF);   // fields weighted and transformed
   __device__ static constexpr int field(int c) { return (NF == 6 && c == 3) ? 5 : c; }
 };
+
+// Tile shape of an x pass: runtime (TBC = 0) or compile-time (TBC columns
+// of nx = 128).
+template <int TBC>
+struct XTile {
+  static constexpr bool CT = TBC > 0;
+  int TB, nx;
+  FastDiv fTB;
+  __device__ XTile(int tb, const FastDiv& f, int n) : TB(CT ? TBC : tb), nx(CT ? 128 : n), fTB(f) {}
+  __device__ __forceinline__ int div(int i) const {
+    if constexpr (CT) return (int)((unsigned)i / (unsigned)TBC);
+    else return (int)fdiv((unsigned)i, fTB);
+  }
+};
+
+// in-LDS x transform of `batch` sequences (NS of them, element stride ES):
+// the compile-time stages for TBC > 0, else the plan's radix sequence
+template <int SGN, int TBC, int NS, int ES>
+__device__ __forceinline__ void x_fft(double2* buf, int batch, const FastDiv& fb, int es, const GridDev& g,
+                                      const double2* tws) {
+  if constexpr (TBC > 0) lds_fft_t<SGN, 128, NS, ES, 128 * TBC, 1, 8, 4, 4>(buf, tws);
+  else lds_fft<SGN, true, VPT>(buf, batch, fb, 1, es, g.fx, tws);
+}
 
 // ky and kz of a column, zero on their axis' Nyquist plane
 __device__ __forceinline__ void column_k(const GridDev& g, int wy, int wz, double& kyp, double& kzp) {
@@ -579,21 +608,23 @@ __device__ __forceinline__ void column_k(const GridDev& g, int wy, int wz, doubl
 // (2A/N) sum_half |rho^|^2 w_D^2 with the same plane weights.  Block partials
 // (quantities: the MFA energy, then the WDA energy) are indexed by tile
 // (ntiles of them).
-template <int NF, bool MFA, int OCC>
+template <int NF, bool MFA, int OCC, int TBC = 0>
 __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F, long long Fs, long long Ffs,
-           GridDev g, FuncDev fp, int TB, FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
+           GridDev g, FuncDev fp, int TBr, FastDiv fTB, FastDiv fS6, FastDiv fnzh, int per, long long ntiles,
            const int* __restrict__ active, RedDev red, int vz) {
   using XF = XFields<NF>;
   constexpr int NC = XF::nc;
+  const XTile<TBC> xt(TBr, fTB, g.nx);
+  const int TB = xt.TB;
   extern __shared__ double2 sm[];    // [NC T] fields, then the nx twiddles
-  double2* s_tw = sm + NC * g.nx * TB;
+  double2* s_tw = sm + NC * xt.nx * TB;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
-  const int T = g.nx * TB;
+  const int T = xt.nx * TB;
   const int S6 = NC * TB;
   double2* B = sm;                    // [nx][NC TB]
   double2* A = sm + (NC - 1) * TB;    // rho^ in the slot of the last field: [x * S6 + b]
@@ -606,26 +637,26 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = min(i0 + u * (int)blockDim.x, T - 1);
-      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+      const int x = xt.div(i), b = i - x * TB;
       v[u] = src[(long long)x * nb + min(b, tbv - 1)];
     }
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
       const int i = i0 + u * (int)blockDim.x;
       if (i < T) {
-        const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+        const int x = xt.div(i), b = i - x * TB;
         A[x * S6 + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
       }
     }
   }
   __syncthreads();
-  lds_fft<-1, true, VPT>(A, TB, fTB, 1, S6, g.fx, s_tw);
+  x_fft<-1, TBC, TBC, NC * TBC>(A, TB, fTB, S6, g, s_tw);
   constexpr bool E = MFA || XF::wlin;   // energies reduced here
   double emfa = 0.0, ewl = 0.0;
   // each element's weighted copies overwrite only that element's own slots
   // (its rho^ among them), and one thread reads and writes them
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
-    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+    const int x = xt.div(i), b = i - x * TB;
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
@@ -649,13 +680,13 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     grid_reduce_store<2>(acc, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
   }
   __syncthreads();
-  lds_fft<1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
+  x_fft<1, TBC, NC * TBC, NC * TBC>(B, S6, fS6, S6, g, s_tw);
   if constexpr (XF::red) {
     // one element's six fields per thread; consecutive threads write
     // consecutive columns of a row of each field
     double2* dst = F + inst * Fs + b0;
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
-      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+      const int x = xt.div(i), b = i - x * TB;
       if (b >= tbv) continue;
       const unsigned bb = (unsigned)(b0 + b);
       const int wy = (int)fdiv(bb, fnzh);
@@ -676,7 +707,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     for (int f = 0; f < NF; ++f) {
       double2* dst = F + inst * Fs + f * Ffs + b0;
       for (int i = threadIdx.x; i < T; i += blockDim.x) {
-        const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+        const int x = xt.div(i), b = i - x * TB;
         if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + f * TB + b];
       }
     }
@@ -690,23 +721,25 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
 // field and transformed with the others; K rho^ joins the weighted sum.
 // NF = 5 (linear WDA): rho^ is loaded likewise and (2A/N) w_D^2 rho^ joins
 // the sum in place of the back-convolved WDA derivative field.
-template <int NF, bool MFA, int OCC>
+template <int NF, bool MFA, int OCC, int TBC = 0>
 __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
-k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp, int TB, FastDiv fTB,
+k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev fp, int TBr, FastDiv fTB,
         FastDiv fS6, FastDiv fT, FastDiv fnzh, int per, long long ntiles, const int* __restrict__ active,
         const double2* __restrict__ rz, long long rzs) {
   using XF = XFields<NF>;
   constexpr int NC = XF::nc;
   constexpr bool LR = MFA || XF::wlin;      // rho^ joins the fields
   constexpr int NL = NC + (LR ? 1 : 0);     // fields held in LDS
+  const XTile<TBC> xt(TBr, fTB, g.nx);
+  const int TB = xt.TB;
   extern __shared__ double2 sm[];    // [NL T] fields, then the nx twiddles
-  double2* s_tw = sm + NL * g.nx * TB;
+  double2* s_tw = sm + NL * xt.nx * TB;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
   const long long tile = xcd_tile(per);
   if (tile >= ntiles) return;
-  const int T = g.nx * TB;
+  const int T = xt.nx * TB;
   const int S6 = NL * TB;
   double2* B = sm;    // [nx][NL TB]; the weighted sum goes to field 0's slot
   const long long nb = (long long)g.ny * g.nzh;
@@ -719,7 +752,7 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     // together; G_n3 + i ky' G_vy formed before the transform (the z pass
     // has already folded i kz' G_vz into G_n3)
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
-      const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+      const int x = xt.div(i), b = i - x * TB;
       const long long o = (long long)x * nb + min(b, tbv - 1);
       const double2 v0 = src[o], v1 = src[Ffs + o], v2 = src[2 * Ffs + o], v3 = src[3 * Ffs + o];
       const double2 v5 = XF::wlin ? make_double2(0.0, 0.0) : src[5 * Ffs + o];
@@ -762,9 +795,9 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     }
   }
   __syncthreads();
-  lds_fft<-1, true, VPT>(B, S6, fS6, 1, S6, g.fx, s_tw);
+  x_fft<-1, TBC, NL * TBC, NL * TBC>(B, S6, fS6, S6, g, s_tw);
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
-    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+    const int x = xt.div(i), b = i - x * TB;
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
@@ -781,10 +814,10 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     B[x * S6 + b] = acc;    // this element's slots are read above by this thread only
   }
   __syncthreads();
-  lds_fft<1, true, VPT>(B, TB, fTB, 1, S6, g.fx, s_tw);
+  x_fft<1, TBC, TBC, NL * TBC>(B, TB, fTB, S6, g, s_tw);
   double2* dst = F + inst * Fs + b0;
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
-    const int x = (int)fdiv((unsigned)i, fTB), b = i - x * TB;
+    const int x = xt.div(i), b = i - x * TB;
     if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + b];
   }
 }

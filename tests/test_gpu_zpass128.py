@@ -1,10 +1,13 @@
-"""The functional z pass for nz = 128 with its tile fixed at compile time
-(k_zphi128: 2, 4, 6 or 8 pencils per block), the one every 128-point z axis
-takes: the derivative is the same bit for bit for every tile (each pencil's
-transforms and pointwise stage do the same operations whatever the tile),
-the energy sums agree to rounding (their block partials differ), and at
-128^3 the hard-sphere derivative and value match the oracle's restatement
-of HardSphereFunctional (cmmdft/functionals.py:379-618)."""
+"""The functional passes with their tiles fixed at compile time for 128-point
+axes: the z pass (k_zphi128: 2, 4, 6 or 8 pencils per block), the one every
+128-point z axis takes, and the x passes (k_xweights / k_xback with 2 or 4
+columns, nx = 128, radix sequence 8, 4, 4; 1 or 3 columns take the runtime
+tile).  The derivative is the same bit for bit for every tile (each
+pencil's and column's transforms and pointwise stage do the same operations
+whatever the tile), the energy sums agree to rounding (their block partials
+differ), and at 128^3 the hard-sphere derivative and value match the
+oracle's restatement of HardSphereFunctional
+(cmmdft/functionals.py:379-618)."""
 import numpy as np
 import pytest
 
@@ -15,32 +18,54 @@ from helpers import T, mu_at, product_program
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize('shape', [(128, 128, 128), (16, 20, 128)])
-@pytest.mark.parametrize('kspace', [True, False])
-def test_tiles_give_the_same_derivative(shape, kspace):
+def _derive_at_tiles(shape, kspace, tiles, mfa=False):
     import torch
-    from cmmdft_amd import _lib as L
     from cmmdft_amd.solver import _Engine
     prog = product_program(W.host_case2(), shape, 'WBII')
     prog.fener.part_dict['WDA-V'].kspace = kspace
+    if mfa:
+        prog.fener.add_mean_field()
     eng = _Engine(prog.grid, prog.fener)
     assert eng.plan.info['wda_kspace'] == int(kspace)
+    if shape[0] == 128:
+        eng.plan.set_radix(0, [8, 4, 4])
     rho = prog._set_initial_density(chempot=mu_at(10.0), temperature=T).reshape(-1).contiguous()
     g = torch.Generator(device='cpu').manual_seed(2)
     rho = (rho * (1 + 0.3 * torch.rand(rho.shape, generator=g, dtype=torch.float64).cuda())).contiguous()
     out = {}
-    for P in (2, 4, 6, 8):
-        eng.plan.set_tiles(p_phi=P)
+    for key, kw in tiles:
+        eng.plan.set_tiles(**kw)
         c = torch.empty_like(rho)
         eng.plan.seed(rho, None, False, eng.lam3)
         eng.plan.derive()
         eng.plan.finish_c1(rho, None, False, c, eng.lam3)
-        out[P] = (c.clone(), eng.plan.scalars()[0].copy())
-    for P in (2, 4, 8):
-        assert torch.equal(out[P][0], out[6][0]), P
-        for q in (L.S_PHI, L.S_FEX):
-            assert out[P][1][q] == pytest.approx(out[6][1][q], rel=1e-13)
-        assert out[P][1][L.S_N3MAX] == out[6][1][L.S_N3MAX]
+        out[key] = (c.clone(), eng.plan.scalars()[0].copy())
+    return out
+
+
+def _same(out, ref):
+    import torch
+    from cmmdft_amd import _lib as L
+    for key in out:
+        assert torch.equal(out[key][0], out[ref][0]), key
+        for q in (L.S_PHI, L.S_FEX, L.S_FMFA):
+            assert out[key][1][q] == pytest.approx(out[ref][1][q], rel=1e-13, abs=1e-300)
+        assert out[key][1][L.S_N3MAX] == out[ref][1][L.S_N3MAX]
+
+
+@pytest.mark.parametrize('shape', [(128, 128, 128), (16, 20, 128)])
+@pytest.mark.parametrize('kspace', [True, False])
+def test_tiles_give_the_same_derivative(shape, kspace):
+    out = _derive_at_tiles(shape, kspace, [(P, dict(p_phi=P)) for P in (2, 4, 6, 8)])
+    _same(out, 6)
+
+
+@pytest.mark.parametrize('shape', [(128, 128, 128), (128, 24, 20)])
+@pytest.mark.parametrize('kspace,mfa', [(True, False), (False, False), (True, True)])
+def test_x_tiles_give_the_same_derivative(shape, kspace, mfa):
+    """2 and 4 columns (compile-time tiles) against 1 and 3 (runtime)."""
+    out = _derive_at_tiles(shape, kspace, [(tb, dict(tb_x=tb)) for tb in (1, 2, 3, 4)], mfa=mfa)
+    _same(out, 1)
 
 
 @pytest.mark.parametrize('version', ['WBII', 'aFMT'])
