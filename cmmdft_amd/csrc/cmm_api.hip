@@ -50,6 +50,7 @@ struct cmm_plan {
   int has_func;
   double2* rz;
   double2* F;
+  double* kwt;       // the functional's weights w2, w3, w_D on this grid [3][nhalf] (x passes)
   long long nhalf;   // complex elements per half spectrum
   long long npts;
   double* partials;
@@ -260,7 +261,7 @@ int cmm_plan_destroy(cmm_plan* p) {
     if (p->ktab[a]) (void)hipFree(p->ktab[a]);
     if (p->ltab[a]) (void)hipFree(p->ltab[a]);
   }
-  void* bufs[] = {p->rz, p->F, p->partials, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef,
+  void* bufs[] = {p->rz, p->F, p->kwt, p->partials, p->scal, p->inst, p->gram, p->damping, p->cf, p->coef,
                   p->alpha_out, p->b_active, p->b_iters, p->b_status, p->b_final, p->b_omega_prev, p->b_omega,
                   p->b_mu};
   for (void* b : bufs)
@@ -400,6 +401,14 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
   d.inv4piR2 = 1.0 / (4.0 * M_PI * d.R * d.R);
   d.inv4piR = 1.0 / (4.0 * M_PI * d.R);
   d.c_tor = 3.0 / d.R;
+  // the x passes' weights of every half-spectrum element, computed once per
+  // functional (they depend on R, D and the grid) on the plan's stream
+  if (!p->kwt) HIPCHK(hipMalloc(&p->kwt, (size_t)3 * p->nhalf * sizeof(double)));
+  hipLaunchKernelGGL(k_kw_table, dim3((unsigned)std::min<long long>((p->nhalf + NT - 1) / NT, 65536)), dim3(NT), 0,
+                     p->stream, p->g, d, p->kwt);
+  LAUNCHCHK("k_kw_table");
+  d.kwt = p->kwt;
+  d.kwt_n = p->nhalf;
   p->has_func = 1;
   return CMM_OK;
 }

@@ -16,6 +16,7 @@ constexpr int S_PHI = 0;    // sum Phi, sum f_ex(rho_bar), max n3            (fu
 constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (final pass)
                             //   picard: N', sum|d|, max|d|, nonfinite
                             //   anderson: nonfinite, Gram row[CMM_MMAX]
+                            //     (and 0 for the ideal and ext sums)
                             //   check: nonfinite, |G - rho|^2, DER sum, DER sum
                             //          without the tolerance weights
 constexpr int S_GRAM = S_NCUR + 4;
@@ -54,6 +55,8 @@ struct FuncDev {
   const double2* kmfa;   // mean-field (MFA) kernel, half spectrum [nx][ny][nz/2+1]; null when off
   double wl2;       // linear WDA (every EOS term mean field, f_ex = A rho_bar^2): 2A, its terms applied
                     // in k space; 0 when the WDA takes the pointwise route
+  const double* kwt;   // the weights w2, w3, w_D of every half-spectrum element [3][kwt_n], or null
+  long long kwt_n;
 };
 
 struct InstDev {

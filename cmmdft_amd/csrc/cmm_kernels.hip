@@ -491,6 +491,22 @@ __device__ __forceinline__ KW kweights(const GridDev& g, const FuncDev& fp, int 
   return r;
 }
 
+// The weights w2, w3, w_D of every half-spectrum element (flat index
+// x (ny nzh) + y nzh + z) into out[3][n]: the table the x passes read.
+__global__ void __launch_bounds__(NT) k_kw_table(GridDev g, FuncDev fp, double* __restrict__ out) {
+  const long long n = (long long)g.nx * g.ny * g.nzh;
+  const long long nb = (long long)g.ny * g.nzh;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int x = (int)(i / nb);
+    const long long bb = i - x * nb;
+    const int y = (int)(bb / g.nzh), z = (int)(bb - (long long)y * g.nzh);
+    const KW w = kweights(g, fp, x, y, z);
+    out[i] = w.w2;
+    out[n + i] = w.w3;
+    out[2 * n + i] = w.wD;
+  }
+}
+
 // Weight of field f at grid indices (ix, iy, iz).  Forward (DIR = -1): the
 // weights that produce n2, n3, n_v2 (x,y,z) and the WDA density from rho^.
 // Back (DIR = +1): the weights that convolve the derivative fields (vector
@@ -591,6 +607,23 @@ __device__ __forceinline__ void x_fft(double2* buf, int batch, const FastDiv& fb
   else lds_fft<SGN, true, VPT>(buf, batch, fb, 1, es, g.fx, tws);
 }
 
+// Weights of x-pass element (x, wy, wz), flat half-spectrum index idx: from
+// the plan's table (filled by k_kw_table with kweights' values) when there is
+// one, else computed.  The k = 0 flag and wt are formed as in kweights.
+__device__ __forceinline__ KW x_kw(const GridDev& g, const FuncDev& fp, int x, int wy, int wz, long long idx) {
+  if (fp.kwt) {
+    KW w;
+    w.w2 = fp.kwt[idx];
+    w.w3 = fp.kwt[fp.kwt_n + idx];
+    w.wD = fp.kwt[2 * fp.kwt_n + idx];
+    const double kx = g.kx.k[x], ky = g.ky.k[wy], kz = g.kz.k[wz];
+    w.k0 = (sqrt(kx * kx + ky * ky + kz * kz) * fp.R <= 1e-8);
+    w.wt = w.w2 - fp.c_tor * w.w3;
+    return w;
+  }
+  return kweights(g, fp, x, wy, wz);
+}
+
 // ky and kz of a column, zero on their axis' Nyquist plane
 __device__ __forceinline__ void column_k(const GridDev& g, int wy, int wz, double& kyp, double& kzp) {
   kyp = (wy == g.ky.nyq) ? 0.0 : g.ky.k[wy];
@@ -660,7 +693,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
-    const KW w = kweights(g, fp, x, wy, wz);
+    const KW w = x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
     const double2 a = A[x * S6 + b];
     if (E && b < tbv) {
       const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
@@ -801,7 +834,7 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
-    const KW w = kweights(g, fp, x, wy, wz);
+    const KW w = x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
@@ -1337,12 +1370,16 @@ __global__ void __launch_bounds__(NT) k_zfinal(const double2* __restrict__ F, lo
       const int pl = (int)fdiv((unsigned)i, t.fn), e = i - pl * n;
       double& cv = z_val(sm, t, 0, pl, e);
       const double C1 = cv + vv[u];
-      // ideal-gas and external terms of the grand potential of rho
+      // ideal-gas and external terms of the grand potential of rho (the
+      // Anderson residual pass forms only N: its callers take the other two
+      // sums from the producer pass of the same density)
       const double rc = (r[u] < 1e-12) ? 1e-30 : r[u];
-      const double lg = log(el.lam3 * rc);
+      const double lg = (MODE == MODE_ANDRES) ? 0.0 : log(el.lam3 * rc);
       acc[0] += r[u];
-      acc[1] += rc * (lg - 1.0);
-      acc[2] += r[u] * vv[u];
+      if (MODE != MODE_ANDRES) {
+        acc[1] += rc * (lg - 1.0);
+        acc[2] += r[u] * vv[u];
+      }
       if (MODE == MODE_C1) {
         el.c1out[inst * el.fstride + idx[u]] = C1;
       } else if (MODE == MODE_CHECK) {

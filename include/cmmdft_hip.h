@@ -224,7 +224,11 @@ int cmm_hs_weight_table(cmm_plan* plan, double R, double* out);
  * over the 6 fields, 4 y pass over rho */
 int cmm_launch_pass(cmm_plan* plan, int which);
 
-/* final passes after cmm_derive; rho is the density the derivative belongs to */
+/* final passes after cmm_derive; rho is the density the derivative belongs to.
+ * Scalars [3..5] = N, ideal-gas sum sum rho_c (ln(lam3 rho_c) - 1) and
+ * external sum sum rho V of rho; cmm_finish_anderson forms N only (its ideal
+ * and external slots read 0: the producer pass gives them for the same
+ * density) */
 int cmm_finish_c1(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double* c1,
                   double lam3);
 int cmm_finish_picard(cmm_plan* plan, const double* rho, const double* vext, int vext_batched, double* rho_out,
