@@ -72,12 +72,15 @@ def pass_names(nyf):
 
 def pass_bytes(nx, ny, nz, nyf):
     """Algorithmic HBM bytes per launch of each functional pass (one
-    instance) with ``nyf`` travelling fields: every complex half-spectrum
-    element read or written once.  k_xweights reads rho^ and writes the
-    fields; k_xback reads them (with a linear WDA: four of them and rho^) and
-    writes one."""
+    instance) with ``nyf`` fields travelling through the z pass: every
+    complex half-spectrum element read or written once.  The y component of
+    n_v2 is formed by the paired y pass from n3 (and its derivative folded
+    into n3's there), so the x passes move one field fewer: k_xweights reads
+    rho^ and writes nyf - 1 fields; k_xback reads nyf - 1 (with a linear WDA
+    three of them and rho^) and writes one; the y pass timed (the back
+    chain's) reads nyf fields and writes nyf - 1."""
     nh = nx * ny * (nz // 2 + 1) * 16
-    return {0: 2 * nyf * nh, 1: (1 + nyf) * nh, 2: 6 * nh, 3: 2 * nyf * nh}
+    return {0: 2 * nyf * nh, 1: nyf * nh, 2: 5 * nh, 3: (2 * nyf - 1) * nh}
 
 
 def measured_traffic(kernel, n, nbatch=1):

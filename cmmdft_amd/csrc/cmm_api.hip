@@ -527,21 +527,46 @@ int cmm_set_instances(cmm_plan* p, const double* fug, const double* alpha) {
 // --------------------------------------------------------------------------
 // launch helpers
 // --------------------------------------------------------------------------
-// y passes over nf fields (field stride nhalf, instance stride is); with
-// fskip >= 0, over nf fields of the nf + 1 that skip field fskip
-static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sgn, int fskip = -1) {
+// The compile-time y pass (k_ypass128): ny = 128 with the radix sequence 8,
+// 4, 4 and a tile of 8, 13 or 16 columns
+static bool ypass128_ok(const cmm_plan* p) {
+  const Fft1D& f = p->g.fy;
+  return p->ny == 128 && f.nst == 3 && f.rad[0] == 8 && f.rad[1] == 4 && f.rad[2] == 4 &&
+         (p->tby == 8 || p->tby == 13 || p->tby == 16);
+}
+
+template <int TB>
+static void launch_ypass128(cmm_plan* p, dim3 grid, size_t lds, double2* F, long long is, int ntz, int per, int sgn,
+                            unsigned fmap, bool pair) {
+  auto kern = sgn < 0 ? (pair ? k_ypass128<-1, true, TB> : k_ypass128<-1, false, TB>)
+                      : (pair ? k_ypass128<1, true, TB> : k_ypass128<1, false, TB>);
+  hipLaunchKernelGGL(kern, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, ntz, per, p->active, fmap);
+}
+
+// y passes over nf fields (field stride nhalf, instance stride is); fmap != 0
+// lists the fields, one byte per block row; pair: the PAIR kernel (field 1
+// carries field 3, see k_ypass)
+static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sgn, unsigned fmap = 0,
+                           bool pair = false) {
   const int ntz = (p->nzh + p->tby - 1) / p->tby;
   const long long ntiles = (long long)p->nx * ntz;
   const int per = (int)((ntiles + 7) / 8);
   dim3 grid((unsigned)(8 * per), nf, p->nb);
-  size_t lds = (size_t)p->ny * p->tby * sizeof(double2);
+  // the paired block needs room for two sequences when the tile is one column
+  size_t lds = (size_t)p->ny * (pair ? std::max(2, p->tby) : p->tby) * sizeof(double2);
   const FastDiv fT = make_fastdiv((unsigned)p->tby);
-  if (sgn < 0)
-    hipLaunchKernelGGL(k_ypass<-1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per,
-                       p->active, fskip);
-  else
-    hipLaunchKernelGGL(k_ypass<1>, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per,
-                       p->active, fskip);
+  if (ypass128_ok(p)) {
+    lds = (size_t)p->ny * p->tby * sizeof(double2);
+    if (p->tby == 8) launch_ypass128<8>(p, grid, lds, F, is, ntz, per, sgn, fmap, pair);
+    else if (p->tby == 13) launch_ypass128<13>(p, grid, lds, F, is, ntz, per, sgn, fmap, pair);
+    else launch_ypass128<16>(p, grid, lds, F, is, ntz, per, sgn, fmap, pair);
+    LAUNCHCHK("k_ypass128");
+    return CMM_OK;
+  }
+  auto kern = sgn < 0 ? (pair ? k_ypass<-1, true> : k_ypass<-1, false>)
+                      : (pair ? k_ypass<1, true> : k_ypass<1, false>);
+  hipLaunchKernelGGL(kern, grid, dim3(NT), lds, p->stream, F, is, p->nhalf, p->g, p->tby, fT, ntz, per, p->active,
+                     fmap);
   LAUNCHCHK("k_ypass");
   return CMM_OK;
 }
@@ -550,17 +575,21 @@ static int launch_ypass(cmm_plan* p, double2* F, int nf, int sgn) {
   return launch_ypass_is(p, F, (nf == 1 && F == p->rz) ? p->nhalf : (long long)p->fcap * p->nhalf, nf, sgn);
 }
 
-// y passes of the functional chain's fields.  Six-field functional: five of
-// them -- the z component of n_v2 (field 4) is i kz times n3 on both sides of
-// the y and x passes and is not carried (k_xweights, z_load_half_multi),
-// unless `full` asks for it (callers that read the weighted densities).
-// A linear WDA (wda_kspace) leaves out rho_bar's field as well: four fields.
+// y passes of the functional chain's fields.  Six-field functional: the z
+// component of n_v2 (field 4) is i kz times n3 on both sides of the y and x
+// passes and is not carried (k_zphi forms it, see z_load_half_multi), and
+// its y component (field 3) is formed from n3 (forward chain) or folded into
+// G_n3 (back chain) by the paired y pass, so the x passes neither write nor
+// read it: three block rows (n2, n3 with n_v2,y, n_v2,x) plus rho_bar's
+// unless the WDA is linear (wda_kspace: rho_bar is never formed).  `full`
+// (callers that read the weighted densities) carries every field.
 static bool wda_kspace(const cmm_plan* p) { return p->nf == 6 && p->fp.wl2 != 0.0; }
 
 static int launch_ypass_chain(cmm_plan* p, int sgn, bool full = false) {
   const long long is = (long long)p->fcap * p->nhalf;
   if (p->nf == 6 && !full)
-    return wda_kspace(p) ? launch_ypass_is(p, p->F, is, 4, sgn) : launch_ypass_is(p, p->F, is, 5, sgn, 4);
+    return wda_kspace(p) ? launch_ypass_is(p, p->F, is, 3, sgn, 0x020100u, true)
+                         : launch_ypass_is(p, p->F, is, 4, sgn, 0x05020100u, true);
   return launch_ypass_is(p, p->F, is, p->nf, sgn);
 }
 

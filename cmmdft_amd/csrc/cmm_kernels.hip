@@ -370,12 +370,20 @@ __device__ __forceinline__ long long xcd_tile(int per) {
 
 // y pass: one block per tile of TB consecutive kz columns of one x plane (all
 // y) of one field; tiles t = x * ntz + kz-tile.
-// fskip >= 0: that field is not stored (the z component of n_v2 of a
-// six-field functional) and the grid's y index skips it.
-template <int SGN>
+// fmap != 0: the block row blockIdx.y works on field (fmap >> 8 blockIdx.y) &
+// 0xff (the chain's travelling fields), else on field blockIdx.y.
+// PAIR (six-field functionals): the block of field 1 (n3) also does field 3
+// (the y component of n_v2), which is i ky' times field 1 on either side of
+// the y transform (ky' = ky, zero on its Nyquist plane): for SGN = +1 (the
+// forward chain) n_v2,y = -i ky' n3 is formed from n3's tile and transformed
+// and stored beside it; for SGN = -1 (the back chain) both derivative tiles
+// are transformed and G_n3 + i ky' G_vy is stored in field 1 (field 3 is not
+// stored).  The x passes then move neither, and the values are those they
+// formed on the same elements.
+template <int SGN, bool PAIR = false>
 __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long is, long long fs, GridDev g, int TB,
                                              FastDiv fTB, int ntz, int per, const int* __restrict__ active,
-                                             int fskip) {
+                                             unsigned fmap) {
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fy);
@@ -386,10 +394,55 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   const int x = (int)(t / ntz);
   const int k0 = (int)(t - (long long)x * ntz) * TB;
   const int tbv = min(TB, g.nzh - k0);
-  const int fld = (fskip >= 0 && (int)blockIdx.y >= fskip) ? (int)blockIdx.y + 1 : (int)blockIdx.y;
+  const int fld = fmap ? (int)((fmap >> (8 * blockIdx.y)) & 0xffu) : (int)blockIdx.y;
   double2* base = F + inst * is + fld * fs + (long long)x * g.ny * g.nzh + k0;
   const int tot = g.ny * TB;
+  if (PAIR && fld == 1) {
+    // rounds of cw <= max(1, TB / 2) columns: the columns' field-1 and
+    // field-3 sequences side by side in LDS (2 cw sequences, element-major),
+    // transformed together; no value is held in registers across a transform
+    double2* base3 = base + 2 * fs;
+    const int c = max(1, TB >> 1);
+    for (int j0 = 0; j0 < TB; j0 += c) {
+      const int cw = min(c, TB - j0), ns = 2 * cw;
+      const FastDiv fC = make_fastdiv((unsigned)cw), fS = make_fastdiv((unsigned)ns);
+      const int tot2 = g.ny * cw;
+      for (int i = threadIdx.x; i < tot2; i += blockDim.x) {
+        const int y = (int)fdiv((unsigned)i, fC), j = i - y * cw;
+        const bool in = j0 + j < tbv;
+        const long long o = (long long)y * g.nzh + min(j0 + j, tbv - 1);
+        const double2 v = base[o];
+        double2 w;
+        if (SGN > 0) {
+          const double kyp = (y == g.ky.nyq) ? 0.0 : g.ky.k[y];
+          w = cmul_i(cscale(v, kyp), -1.0);
+        } else {
+          w = base3[o];
+        }
+        sm[y * ns + j] = make_double2(in ? v.x : 0.0, in ? v.y : 0.0);
+        sm[y * ns + cw + j] = make_double2(in ? w.x : 0.0, in ? w.y : 0.0);
+      }
+      __syncthreads();
+      lds_fft<SGN, true, VPT>(sm, ns, fS, 1, ns, g.fy, s_tw);
+      for (int i = threadIdx.x; i < tot2; i += blockDim.x) {
+        const int y = (int)fdiv((unsigned)i, fC), j = i - y * cw;
+        if (j0 + j < tbv) {
+          const long long o = (long long)y * g.nzh + j0 + j;
+          if (SGN > 0) {
+            base[o] = sm[y * ns + j];
+            base3[o] = sm[y * ns + cw + j];
+          } else {
+            const double kyp = (y == g.ky.nyq) ? 0.0 : g.ky.k[y];
+            base[o] = cadd(sm[y * ns + j], cmul_i(cscale(sm[y * ns + cw + j], kyp), 1.0));
+          }
+        }
+      }
+      __syncthreads();
+    }
+    return;
+  }
   for (int i0 = threadIdx.x; i0 < tot; i0 += LU * blockDim.x) {
+
     double2 v[LU];
 #pragma unroll
     for (int u = 0; u < LU; ++u) {
@@ -412,6 +465,121 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   for (int i = threadIdx.x; i < tot; i += blockDim.x) {
     const int y = (int)fdiv((unsigned)i, fTB), b = i - y * TB;
     if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
+  }
+}
+
+// k_ypass at ny = 128 with the radix sequence 8, 4, 4 and the tile of TB
+// columns fixed at compile time: the same operations in the same order as
+// k_ypass with that sequence and tile, so the same results bit for bit.
+// LDS offsets and index splits are constants, and the kernel holds no
+// generic transform, so the paired block (PAIR, field 1: see k_ypass) can
+// keep its tile's values in registers across a transform within the
+// five-waves-per-SIMD budget that LDS sets: for SGN = +1 it transforms
+// -i ky' n3 from the loaded values, then n3; for SGN = -1 it transforms
+// G_n3, keeps the result, transforms G_vy (loaded with G_n3) and stores
+// G_n3 + i ky' G_vy.
+template <int SGN, bool PAIR, int TB>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PAIR ? 5 : 1))) k_ypass128(double2* __restrict__ F, long long is, long long fs, GridDev g,
+                                                int ntz, int per, const int* __restrict__ active, unsigned fmap) {
+  constexpr int TOT = 128 * TB;
+  constexpr int E = (TOT + NT - 1) / NT;   // tile elements per thread: tid + k NT, k < E
+  extern __shared__ double2 sm[];
+  __shared__ double2 s_tw[128];
+  load_twiddles(s_tw, g.fy);
+  const int inst = blockIdx.z;
+  if (active && !active[inst]) return;
+  const long long t = xcd_tile(per);
+  if (t >= (long long)g.nx * ntz) return;
+  const int x = (int)(t / ntz);
+  const int k0 = (int)(t - (long long)x * ntz) * TB;
+  const int tbv = min(TB, g.nzh - k0);
+  const int fld = fmap ? (int)((fmap >> (8 * blockIdx.y)) & 0xffu) : (int)blockIdx.y;
+  double2* base = F + inst * is + fld * fs + (long long)x * 128 * g.nzh + k0;
+  double2* base3 = base + 2 * fs;
+  const bool pair = PAIR && fld == 1;
+  double2 v[E], w[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = min((int)threadIdx.x + k * NT, TOT - 1);
+    const int y = i / TB, b = i - y * TB;
+    v[k] = base[(long long)y * g.nzh + min(b, tbv - 1)];
+  }
+  if (PAIR && SGN < 0 && pair) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = min((int)threadIdx.x + k * NT, TOT - 1);
+      const int y = i / TB, b = i - y * TB;
+      w[k] = base3[(long long)y * g.nzh + min(b, tbv - 1)];
+    }
+  }
+  if (PAIR && SGN > 0 && pair) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (i < TOT) {
+        const int y = i / TB, b = i - y * TB;
+        const double kyp = (y == g.ky.nyq) ? 0.0 : g.ky.k[y];
+        const double2 u = cmul_i(cscale(v[k], kyp), -1.0);
+        sm[i] = make_double2(b < tbv ? u.x : 0.0, b < tbv ? u.y : 0.0);
+      }
+    }
+    __syncthreads();
+    lds_fft_t<SGN, 128, TB, TB, NT, 1, 8, 4, 4>(sm, s_tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (i < TOT) {
+        const int y = i / TB, b = i - y * TB;
+        if (b < tbv) base3[(long long)y * g.nzh + b] = sm[i];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = (int)threadIdx.x + k * NT;
+    if (i < TOT) {
+      const int b = i % TB;
+      sm[i] = make_double2(b < tbv ? v[k].x : 0.0, b < tbv ? v[k].y : 0.0);
+    }
+  }
+  __syncthreads();
+  lds_fft_t<SGN, 128, TB, TB, NT, 1, 8, 4, 4>(sm, s_tw);
+  if (PAIR && SGN < 0 && pair) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      v[k] = (i < TOT) ? sm[i] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (i < TOT) {
+        const int b = i % TB;
+        sm[i] = make_double2(b < tbv ? w[k].x : 0.0, b < tbv ? w[k].y : 0.0);
+      }
+    }
+    __syncthreads();
+    lds_fft_t<SGN, 128, TB, TB, NT, 1, 8, 4, 4>(sm, s_tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = (int)threadIdx.x + k * NT;
+      if (i < TOT) {
+        const int y = i / TB, b = i - y * TB;
+        const double kyp = (y == g.ky.nyq) ? 0.0 : g.ky.k[y];
+        if (b < tbv) base[(long long)y * g.nzh + b] = cadd(v[k], cmul_i(cscale(sm[i], kyp), 1.0));
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = (int)threadIdx.x + k * NT;
+    if (i < TOT) {
+      const int y = i / TB, b = i - y * TB;
+      if (b < tbv) base[(long long)y * g.nzh + b] = sm[i];
+    }
   }
 }
 
@@ -724,15 +892,17 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       const unsigned bb = (unsigned)(b0 + b);
       const int wy = (int)fdiv(bb, fnzh);
       const int wz = (int)bb - wy * g.nzh;
-      double kyp, kzp;
-      column_k(g, wy, wz, kyp, kzp);
       const long long o = (long long)x * nb + b;
       const double2 n3 = B[x * S6 + TB + b];
       dst[o] = B[x * S6 + b];
       dst[Ffs + o] = n3;
       dst[2 * Ffs + o] = B[x * S6 + 2 * TB + b];
-      dst[3 * Ffs + o] = cmul_i(cscale(n3, kyp), -1.0);
-      if (vz) dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
+      if (vz) {
+        double kyp, kzp;
+        column_k(g, wy, wz, kyp, kzp);
+        dst[3 * Ffs + o] = cmul_i(cscale(n3, kyp), -1.0);
+        dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
+      }
       if constexpr (!XF::wlin) dst[5 * Ffs + o] = B[x * S6 + 3 * TB + b];
     }
   } else {
@@ -781,21 +951,15 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
   const double2* src = F + inst * Fs + b0;
   const double2* rsrc = LR ? rz + inst * rzs + b0 : nullptr;
   if constexpr (XF::red) {
-    // one element's five fields (and rho^) per thread, loads in flight
-    // together; G_n3 + i ky' G_vy formed before the transform (the z pass
-    // has already folded i kz' G_vz into G_n3)
+    // one element's fields (and rho^) per thread, loads in flight together;
+    // field 1 holds G_n3 + i ky' G_vy + i kz' G_vz (the paired y pass and
+    // the z pass have folded the y and z components of G_v into it)
     for (int i = threadIdx.x; i < T; i += blockDim.x) {
       const int x = xt.div(i), b = i - x * TB;
       const long long o = (long long)x * nb + min(b, tbv - 1);
-      const double2 v0 = src[o], v1 = src[Ffs + o], v2 = src[2 * Ffs + o], v3 = src[3 * Ffs + o];
+      const double2 v0 = src[o], h = src[Ffs + o], v2 = src[2 * Ffs + o];
       const double2 v5 = XF::wlin ? make_double2(0.0, 0.0) : src[5 * Ffs + o];
       const double2 r = LR ? rsrc[o] : make_double2(0.0, 0.0);
-      const unsigned bb = (unsigned)min(b0 + b, nb - 1);
-      const int wy = (int)fdiv(bb, fnzh);
-      const int wz = (int)bb - wy * g.nzh;
-      double kyp, kzp;
-      column_k(g, wy, wz, kyp, kzp);
-      const double2 h = cadd(v1, cmul_i(cscale(v3, kyp), 1.0));
       // columns past the tile's end hold zeros (selected per component: a
       // select of whole double2 values is lowered through scratch memory)
       const bool in = b < tbv;

@@ -2,7 +2,8 @@
 axes: the z pass (k_zphi128: 2, 4, 6 or 8 pencils per block), the one every
 128-point z axis takes, and the x passes (k_xweights / k_xback with 2 or 4
 columns, nx = 128, radix sequence 8, 4, 4; 1 or 3 columns take the runtime
-tile).  The derivative is the same bit for bit for every tile (each
+tile) and the y pass (k_ypass128, 8, 13 or 16 columns at ny = 128 with that
+radix sequence).  The derivative is the same bit for bit for every tile (each
 pencil's and column's transforms and pointwise stage do the same operations
 whatever the tile), the energy sums agree to rounding (their block partials
 differ), and at 128^3 the hard-sphere derivative and value match the
@@ -29,6 +30,8 @@ def _derive_at_tiles(shape, kspace, tiles, mfa=False):
     assert eng.plan.info['wda_kspace'] == int(kspace)
     if shape[0] == 128:
         eng.plan.set_radix(0, [8, 4, 4])
+    if shape[1] == 128:
+        eng.plan.set_radix(1, [8, 4, 4])
     rho = prog._set_initial_density(chempot=mu_at(10.0), temperature=T).reshape(-1).contiguous()
     g = torch.Generator(device='cpu').manual_seed(2)
     rho = (rho * (1 + 0.3 * torch.rand(rho.shape, generator=g, dtype=torch.float64).cuda())).contiguous()
@@ -66,6 +69,16 @@ def test_x_tiles_give_the_same_derivative(shape, kspace, mfa):
     """2 and 4 columns (compile-time tiles) against 1 and 3 (runtime)."""
     out = _derive_at_tiles(shape, kspace, [(tb, dict(tb_x=tb)) for tb in (1, 2, 3, 4)], mfa=mfa)
     _same(out, 1)
+
+
+@pytest.mark.parametrize('shape', [(128, 128, 128), (20, 128, 24)])
+@pytest.mark.parametrize('kspace,mfa', [(True, False), (False, False), (True, True)])
+def test_y_tiles_give_the_same_derivative(shape, kspace, mfa):
+    """8, 13 and 16 columns (the compile-time y pass at ny = 128, whose
+    paired block keeps its tile in registers) against 5 and 12 (the runtime
+    y pass, whose paired block works in rounds of columns)."""
+    out = _derive_at_tiles(shape, kspace, [(tb, dict(tb_y=tb)) for tb in (5, 8, 12, 13, 16)], mfa=mfa)
+    _same(out, 12)
 
 
 @pytest.mark.parametrize('version', ['WBII', 'aFMT'])
