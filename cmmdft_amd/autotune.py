@@ -151,7 +151,9 @@ def _tile_candidates(plan):
     # k equal tiles (65 = 5 x 13 at 128^3: no near-empty last tile)
     nzh = nz // 2 + 1
     even = [-(-nzh // k) for k in range(1, 9)]
-    tby = sorted({t for t in [1, 2, 4, 8, 16, 32] + even if t * ny <= budget and t * ny * 16 <= 65536})
+    # (rounded up to even: the paired y block's two rounds of ceil(tb_y / 2))
+    tby = sorted({t for t in [1, 2, 4, 8, 16, 32] + even
+                  if (t + (t & 1)) * ny <= budget and (t + (t & 1)) * ny * 16 <= 65536})
     pz = [p for p in (2, 4, 8, 16, 32) if (p // 2) * nz <= budget and nz * ((p // 2) | 1) * 16 <= 65536]
     if nz == 128:
         pphi = [2, 4, 6, 8]   # the compile-time z tiles of the functional pass

@@ -118,7 +118,10 @@ void default_tiles(cmm_plan* p) {
   // x passes: one tile of nx * tb_x columns per NTX threads (all fields in LDS;
   // launches with twelve fields cap tb_x by LDS)
   p->tbx = std::max(1, std::min(32, NTX / p->nx));
+  // y passes: tb_y columns, and the paired block's two rounds of
+  // ceil(tb_y / 2) columns within the same staging limit
   int tby = std::min(32, std::min(budget / p->ny, 4096 / p->ny));
+  if ((tby & 1) && (tby + 1) * p->ny > budget) --tby;
   p->tby = std::max(1, tby);
   // simple z pass: (P/2)*nz <= budget and nz*((P/2)|1)*16 <= 64 KB
   int pz = std::min(32, 2 * budget / p->nz);
@@ -284,7 +287,8 @@ int cmm_plan_set_tiles(cmm_plan* p, int tbx, int tby, int pz, int pphi) {
     p->tbx = tbx;
   }
   if (tby > 0) {
-    if (tby * p->ny > budget || tby * p->ny * 16 > 65536) return fail(CMM_E_ARG, "tb_y too large");
+    const int tbe = tby + (tby & 1);   // the paired y block's two rounds of ceil(tb_y / 2) columns
+    if (tbe * p->ny > budget || tbe * p->ny * 16 > 65536) return fail(CMM_E_ARG, "tb_y too large");
     p->tby = tby;
   }
   if (pz > 0) {
@@ -552,8 +556,9 @@ static int launch_ypass_is(cmm_plan* p, double2* F, long long is, int nf, int sg
   const long long ntiles = (long long)p->nx * ntz;
   const int per = (int)((ntiles + 7) / 8);
   dim3 grid((unsigned)(8 * per), nf, p->nb);
-  // the paired block needs room for two sequences when the tile is one column
-  size_t lds = (size_t)p->ny * (pair ? std::max(2, p->tby) : p->tby) * sizeof(double2);
+  // the paired block of the runtime kernel takes its columns in two rounds
+  // of ceil(tby / 2), two sequences each
+  size_t lds = (size_t)p->ny * (pair ? 2 * ((p->tby + 1) / 2) : p->tby) * sizeof(double2);
   const FastDiv fT = make_fastdiv((unsigned)p->tby);
   if (ypass128_ok(p)) {
     lds = (size_t)p->ny * p->tby * sizeof(double2);

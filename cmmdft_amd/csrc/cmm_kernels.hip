@@ -398,11 +398,12 @@ __global__ void __launch_bounds__(NT) k_ypass(double2* __restrict__ F, long long
   double2* base = F + inst * is + fld * fs + (long long)x * g.ny * g.nzh + k0;
   const int tot = g.ny * TB;
   if (PAIR && fld == 1) {
-    // rounds of cw <= max(1, TB / 2) columns: the columns' field-1 and
-    // field-3 sequences side by side in LDS (2 cw sequences, element-major),
-    // transformed together; no value is held in registers across a transform
+    // two rounds of cw <= ceil(TB / 2) columns: the columns' field-1 and
+    // field-3 sequences side by side in LDS (2 cw sequences, element-major;
+    // the launch sizes LDS for 2 ceil(TB / 2) columns), transformed together;
+    // no value is held in registers across a transform
     double2* base3 = base + 2 * fs;
-    const int c = max(1, TB >> 1);
+    const int c = (TB + 1) >> 1;
     for (int j0 = 0; j0 < TB; j0 += c) {
       const int cw = min(c, TB - j0), ns = 2 * cw;
       const FastDiv fC = make_fastdiv((unsigned)cw), fS = make_fastdiv((unsigned)ns);
@@ -1081,18 +1082,22 @@ struct PhiOut {
 };
 
 template <typename T>
-__device__ __forceinline__ void phi23(int base, double n3_series, T n3, T ln3, T r1, T& p2, T& p3, T& d2, T& d3) {
+__device__ __forceinline__ void phi23(int base, bool closed, T n3, T ln3, T r1, T in1, T& p2, T& p3, T& d2,
+                                      T& d3) {
   // Same forms as the reference (functionals.py:726-767), whose series
-  // branches hold for n3 <= 1e-8 (n3_series; a larger value keeps the closed
-  // forms' cancellation at low n3 out).  Every
-  // denominator is a product of powers of (1 - n3) and n3, so the divisions
-  // are multiplications by r1 = 1/(1-n3) and (full White-Bear/MFMT branch)
-  // 1/n3: q = (1-n3)^2, c = (1-n3)^3, n3 - n3^2 = n3 (1-n3).
+  // branches hold for n3 <= 1e-8 (closed = false; n3_series > 1e-8 keeps the
+  // closed forms' cancellation at low n3 out).  Every denominator is a
+  // product of powers of (1 - n3) and n3, so the divisions are
+  // multiplications by r1 = 1/(1-n3) and (the closed modified-FMT and
+  // White-Bear II forms) in1 = 1/n3, and the series' constant divisors
+  // multiplications by their reciprocals: q = (1-n3)^2, c = (1-n3)^3,
+  // n3 - n3^2 = n3 (1-n3).
   const T n3_2 = n3 * n3, n3_3 = n3_2 * n3;
   const T q = 1.0 - 2.0 * n3 + n3_2;
   const T c = 1.0 - 3.0 * n3 + 3.0 * n3_2 - n3_3;
   const T iq = r1 * r1, ic = iq * r1;
-  const double i24 = 1.0 / (24.0 * M_PI), i36 = 1.0 / (36.0 * M_PI);
+  constexpr double i24 = 1.0 / (24.0 * M_PI), i36 = 1.0 / (36.0 * M_PI);
+  constexpr double r9 = 1.0 / 9.0, r18 = 1.0 / 18.0;
   if (base == 0) {                       // Rosenfeld
     p2 = r1;
     d2 = iq;
@@ -1101,23 +1106,21 @@ __device__ __forceinline__ void phi23(int base, double n3_series, T n3, T ln3, T
   } else if (base == 1) {                // modified FMT
     p2 = r1;
     d2 = iq;
-    if (val(n3) <= n3_series) {
-      p3 = (1.0 - 2.0 * n3 / 9.0 - n3_2 / 18.0) * i24 * iq;
+    if (!closed) {
+      p3 = (1.0 - (2.0 * r9) * n3 - r18 * n3_2) * i24 * iq;
       d3 = (8.0 / 3.0 - 0.5 * n3 - 0.1 * n3_2) * i36 * ic;
     } else {
-      const T in1 = 1.0 / n3;
       const T in2 = in1 * in1;
       p3 = (n3 + q * ln3) * i36 * in2 * iq;
       d3 = -(2.0 * n3 - 5.0 * n3_2 + n3_3 + 2.0 * c * ln3) * i36 * (in2 * in1) * ic;
     }
   } else {                               // White-Bear II
-    if (val(n3) <= n3_series) {
-      p2 = (1.0 + n3_2 / 9.0) * r1;
-      d2 = (1.0 + 2.0 * n3 / 9.0 + n3_2 / 18.0) * iq;
-      p3 = (1.0 - 4.0 * n3 / 9.0 + n3_2 / 18.0) * i24 * iq;
-      d3 = (7.0 / 3.0 - n3 / 2.0 + n3_2 / 10.0) * i36 * ic;
+    if (!closed) {
+      p2 = (1.0 + r9 * n3_2) * r1;
+      d2 = (1.0 + (2.0 * r9) * n3 + r18 * n3_2) * iq;
+      p3 = (1.0 - (4.0 * r9) * n3 + r18 * n3_2) * i24 * iq;
+      d3 = (7.0 / 3.0 - 0.5 * n3 + 0.1 * n3_2) * i36 * ic;
     } else {
-      const T in1 = 1.0 / n3;
       const T in2 = in1 * in1;
       p2 = (5.0 * n3 - n3_2 + 2.0 * (1.0 - n3) * ln3) * ((1.0 / 3.0) * in1 * r1);
       d2 = -2.0 * (n3 - 3.0 * n3_2 + q * ln3) * (1.0 / 3.0) * in2 * iq;
@@ -1137,28 +1140,44 @@ __device__ __forceinline__ void phi23(int base, double n3_series, T n3, T ln3, T
 //        holds each of them twice)                             with w_t
 // The 't' terms (functionals.py:636-640, 673-717) add 9/2 (nv2 . T . nv2 -
 // tr T^3) to the coefficient of phi3.
+// n0, n1 and n_v1 are n2 / 4piR^2, n2 / 4piR and n_v2 / 4piR (their weights
+// are those multiples of w2 and w_v2, functionals.py:440-451), so
+//   n1 n2 - n_v1 . n_v2 = (n2^2 - |n_v2|^2) / 4piR,
+//   g0 = kT (-ln(1-n3) / 4piR^2 + 2 n2 phi2 / 4piR + dq3/dn2 phi3),
+//   gv = kT n_v2 (fv - 2 phi2 / 4piR) (+ the tensor terms),
+// where fv n_v2 = dq3/dn_v2 phi3.
 template <typename T, bool TEN = false>
 __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw, T vx, T vy, T vz,
                                                const T* tn = nullptr) {
   PhiOut<T> o;
-  const double R = fp.R;
   const double i4pR2 = fp.inv4piR2, i4pR = fp.inv4piR;
-  const T n0 = n2 * i4pR2, n1 = n2 * i4pR;
-  const T ux = vx * i4pR, uy = vy * i4pR, uz = vz * i4pR;   // n_v1
+  const T n0 = n2 * i4pR2;
   const T n3 = clip(n3raw, 1e-30, 0.99);
-  const T ln3 = log(1.0 - n3);
-  const T r1 = 1.0 / (1.0 - n3);
+  const T om = 1.0 - n3;
+  const T ln3 = log(om);
+  // the closed modified-FMT / White-Bear II forms need 1/n3 as well: one
+  // division gives both reciprocals there
+  const bool closed = fp.base != 0 && val(n3) > fp.n3_series;
+  T r1, in1;
+  if (closed) {
+    const T t = 1.0 / (n3 * om);
+    r1 = n3 * t;
+    in1 = om * t;
+  } else {
+    r1 = 1.0 / om;
+    in1 = r1;
+  }
   T p2, p3, d2, d3;
-  phi23<T>(fp.base, fp.n3_series, n3, ln3, r1, p2, p3, d2, d3);
+  phi23<T>(fp.base, closed, n3, ln3, r1, in1, p2, p3, d2, d3);
   const T v2 = vx * vx + vy * vy + vz * vz;
-  const T dot12 = ux * vx + uy * vy + uz * vz;
+  const T s12 = (n2 * n2 - v2) * i4pR;
   T q3, dq3_n2, fv;
   if (fp.anti) {
     const T xi = clip(v2 / (n2 * n2 + 1e-16), 0.0, 1.0);
-    const T om = 1.0 - xi;
-    q3 = n2 * n2 * n2 * om * om * om;
-    dq3_n2 = 3.0 * n2 * n2 * (1.0 + xi) * om * om;
-    fv = -6.0 * n2 * om * om * p3;
+    const T om3 = 1.0 - xi;
+    q3 = n2 * n2 * n2 * om3 * om3 * om3;
+    dq3_n2 = 3.0 * n2 * n2 * (1.0 + xi) * om3 * om3;
+    fv = -6.0 * n2 * om3 * om3 * p3;
   } else {
     q3 = n2 * n2 * n2 - 3.0 * n2 * v2;
     dq3_n2 = 3.0 * (n2 * n2 - v2);
@@ -1174,21 +1193,17 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
     q3 = q3 + 4.5 * quad;
     q3 = q3 - 4.5 * tr3;
   }
-  o.phi = n0 * (-ln3) + (n1 * n2 - dot12) * p2 + q3 * p3;
-  const T dn0 = -ln3;
-  const T dn1 = n2 * p2;
-  const T dn2 = n1 * p2 + dq3_n2 * p3;
-  const T dn3 = n0 * r1 + (n1 * n2 - dot12) * d2 + q3 * d3;
+  o.phi = n0 * (-ln3) + s12 * p2 + q3 * p3;
   const double kT = fp.kT;
-  o.g0 = kT * (dn0 * i4pR2 + dn1 * i4pR + dn2);
-  o.g1 = kT * dn3;
-  // dPhi/dnv1 = -nv2 p2 ; dPhi/dnv2 = -nv1 p2 + nv2 fv
+  o.g0 = kT * (dq3_n2 * p3 + (2.0 * i4pR) * (n2 * p2) - i4pR2 * ln3);
+  o.g1 = kT * (n0 * r1 + s12 * d2 + q3 * d3);
+  const T cv = fv - (2.0 * i4pR) * p2;
   if constexpr (TEN) {
     const T xx = tn[0], xy = tn[1], xz = tn[2], yy = tn[3], yz = tn[4], zz = tn[5];
     const T p3t = 4.5 * p3;
-    o.gv[0] = kT * ((-vx * p2) * i4pR + (-ux * p2 + vx * fv + p3t * (2.0 * (xx * vx + xy * vy + xz * vz))));
-    o.gv[1] = kT * ((-vy * p2) * i4pR + (-uy * p2 + vy * fv + p3t * (2.0 * (xy * vx + yy * vy + yz * vz))));
-    o.gv[2] = kT * ((-vz * p2) * i4pR + (-uz * p2 + vz * fv + p3t * (2.0 * (xz * vx + yz * vy + zz * vz))));
+    o.gv[0] = kT * (vx * cv + p3t * (2.0 * (xx * vx + xy * vy + xz * vz)));
+    o.gv[1] = kT * (vy * cv + p3t * (2.0 * (xy * vx + yy * vy + yz * vz)));
+    o.gv[2] = kT * (vz * cv + p3t * (2.0 * (xz * vx + yz * vy + zz * vz)));
     o.gt[0] = kT * (p3t * (vx * vx - 3.0 * (xx * xx + xy * xy + xz * xz)));
     o.gt[1] = kT * (p3t * ((vx * vy - 3.0 * (xx * xy + yy * xy + xz * yz)) * 2.0));
     o.gt[2] = kT * (p3t * ((vx * vz - 3.0 * (xx * xz + zz * xz + xy * yz)) * 2.0));
@@ -1196,11 +1211,10 @@ __device__ __forceinline__ PhiOut<T> fmt_point(const FuncDev& fp, T n2, T n3raw,
     o.gt[4] = kT * (p3t * ((vy * vz - 3.0 * (yy * yz + zz * yz + xy * xz)) * 2.0));
     o.gt[5] = kT * (p3t * (vz * vz - 3.0 * (zz * zz + xz * xz + yz * yz)));
   } else {
-    o.gv[0] = kT * ((-vx * p2) * i4pR + (-ux * p2 + vx * fv));
-    o.gv[1] = kT * ((-vy * p2) * i4pR + (-uy * p2 + vy * fv));
-    o.gv[2] = kT * ((-vz * p2) * i4pR + (-uz * p2 + vz * fv));
+    o.gv[0] = kT * (vx * cv);
+    o.gv[1] = kT * (vy * cv);
+    o.gv[2] = kT * (vz * cv);
   }
-  (void)R;
   return o;
 }
 
