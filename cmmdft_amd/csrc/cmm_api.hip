@@ -665,12 +665,13 @@ static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, b
   constexpr int NC = XFields<NF>::nc;
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NC);
-  const size_t lds = (size_t)(NC * T + p->nx) * sizeof(double2);
+  // nx rows of NC tb values (padded rows for the compile-time tile), then the twiddles
+  const int tbc = XFields<NF>::red ? xpass_tbc(p, tb) : 0;
+  const size_t lds = (size_t)(p->nx * (tbc ? x_row(NC * tbc) : NC * tb) + p->nx) * sizeof(double2);
   const bool red = MFA || XFields<NF>::wlin;
   const bool occ = xpass_full_occ(nt, lds + (red ? CMM_RED_SH * sizeof(double) : 0));
   auto kern = occ ? k_xweights<NF, MFA, XOCC_FULL> : k_xweights<NF, MFA, 1>;
   if constexpr (XFields<NF>::red) {
-    const int tbc = xpass_tbc(p, tb);
     if (tbc == 4) kern = occ ? k_xweights<NF, MFA, XOCC_FULL, 4> : k_xweights<NF, MFA, 1, 4>;
     if (tbc == 2) kern = occ ? k_xweights<NF, MFA, XOCC_FULL, 2> : k_xweights<NF, MFA, 1, 2>;
   }
@@ -713,11 +714,11 @@ static void launch_xback_nf(cmm_plan* p, int tb, long long ntiles, int per) {
   constexpr int NL = XFields<NF>::nc + ((MFA || XFields<NF>::wlin) ? 1 : 0);
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NL);
-  const size_t lds = (size_t)(NL * T + p->nx) * sizeof(double2);
+  const int tbc = XFields<NF>::red ? xpass_tbc(p, tb) : 0;
+  const size_t lds = (size_t)(p->nx * (tbc ? x_row(NL * tbc) : NL * tb) + p->nx) * sizeof(double2);
   const bool occ = xpass_full_occ(nt, lds);
   auto kern = occ ? k_xback<NF, MFA, XOCC_FULL> : k_xback<NF, MFA, 1>;
   if constexpr (XFields<NF>::red) {
-    const int tbc = xpass_tbc(p, tb);
     if (tbc == 4) kern = occ ? k_xback<NF, MFA, XOCC_FULL, 4> : k_xback<NF, MFA, 1, 4>;
     if (tbc == 2) kern = occ ? k_xback<NF, MFA, XOCC_FULL, 2> : k_xback<NF, MFA, 1, 2>;
   }

@@ -799,6 +799,11 @@ __device__ __forceinline__ void column_k(const GridDev& g, int wy, int wz, doubl
   kzp = (wz == g.kz.nyq) ? 0.0 : g.kz.k[wz];
 }
 
+// LDS row stride (in complex values) of a compile-time x tile of n values
+// per row: rows 256 B apart would put all the rows a wave touches on the
+// same half of the LDS banks, so a multiple of 8 is padded by 4 (64 B).
+__host__ __device__ constexpr int x_row(int n) { return (n % 8 == 0) ? n + 4 : n; }
+
 // forward chain: rho (z,y transformed) -> x FFT -> weighted copies ->
 // inverse x FFT of all of them -> F[0..NF).  The 1/N of the inverse 3-D
 // transform is folded into the weights.  NF = 6 (fields n2, n3, nv2 x/y/z,
@@ -819,8 +824,10 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
   constexpr int NC = XF::nc;
   const XTile<TBC> xt(TBr, fTB, g.nx);
   const int TB = xt.TB;
-  extern __shared__ double2 sm[];    // [NC T] fields, then the nx twiddles
-  double2* s_tw = sm + NC * xt.nx * TB;
+  extern __shared__ double2 sm[];    // [nx][SR] fields, then the nx twiddles
+  constexpr int SRC = TBC > 0 ? x_row(NC * TBC) : 0;
+  const int SR = TBC > 0 ? SRC : NC * TB;   // LDS row stride
+  double2* s_tw = sm + xt.nx * SR;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
@@ -828,8 +835,8 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
   if (tile >= ntiles) return;
   const int T = xt.nx * TB;
   const int S6 = NC * TB;
-  double2* B = sm;                    // [nx][NC TB]
-  double2* A = sm + (NC - 1) * TB;    // rho^ in the slot of the last field: [x * S6 + b]
+  double2* B = sm;                    // [nx][SR]: NC fields of TB columns per row
+  double2* A = sm + (NC - 1) * TB;    // rho^ in the slot of the last field: [x * SR + b]
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
@@ -847,12 +854,12 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       const int i = i0 + u * (int)blockDim.x;
       if (i < T) {
         const int x = xt.div(i), b = i - x * TB;
-        A[x * S6 + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+        A[x * SR + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
       }
     }
   }
   __syncthreads();
-  x_fft<-1, TBC, TBC, NC * TBC>(A, TB, fTB, S6, g, s_tw);
+  x_fft<-1, TBC, TBC, SRC>(A, TB, fTB, SR, g, s_tw);
   constexpr bool E = MFA || XF::wlin;   // energies reduced here
   double emfa = 0.0, ewl = 0.0;
   // each element's weighted copies overwrite only that element's own slots
@@ -863,7 +870,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
     const KW w = x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
-    const double2 a = A[x * S6 + b];
+    const double2 a = A[x * SR + b];
     if (E && b < tbv) {
       const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
       const double a2 = half * (a.x * a.x + a.y * a.y);
@@ -872,7 +879,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      B[x * S6 + c * TB + b] = cscale(apply_w(g, w, x, wy, wz, XF::field(c), a, -1), g.inv_npts);
+      B[x * SR + c * TB + b] = cscale(apply_w(g, w, x, wy, wz, XF::field(c), a, -1), g.inv_npts);
   }
   if (E) {
     __shared__ double sh[CMM_RED_SH];
@@ -882,7 +889,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     grid_reduce_store<2>(acc, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
   }
   __syncthreads();
-  x_fft<1, TBC, NC * TBC, NC * TBC>(B, S6, fS6, S6, g, s_tw);
+  x_fft<1, TBC, NC * TBC, SRC>(B, S6, fS6, SR, g, s_tw);
   if constexpr (XF::red) {
     // one element's six fields per thread; consecutive threads write
     // consecutive columns of a row of each field
@@ -894,17 +901,17 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       const int wy = (int)fdiv(bb, fnzh);
       const int wz = (int)bb - wy * g.nzh;
       const long long o = (long long)x * nb + b;
-      const double2 n3 = B[x * S6 + TB + b];
-      dst[o] = B[x * S6 + b];
+      const double2 n3 = B[x * SR + TB + b];
+      dst[o] = B[x * SR + b];
       dst[Ffs + o] = n3;
-      dst[2 * Ffs + o] = B[x * S6 + 2 * TB + b];
+      dst[2 * Ffs + o] = B[x * SR + 2 * TB + b];
       if (vz) {
         double kyp, kzp;
         column_k(g, wy, wz, kyp, kzp);
         dst[3 * Ffs + o] = cmul_i(cscale(n3, kyp), -1.0);
         dst[4 * Ffs + o] = cmul_i(cscale(n3, kzp), -1.0);
       }
-      if constexpr (!XF::wlin) dst[5 * Ffs + o] = B[x * S6 + 3 * TB + b];
+      if constexpr (!XF::wlin) dst[5 * Ffs + o] = B[x * SR + 3 * TB + b];
     }
   } else {
 #pragma unroll
@@ -912,7 +919,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       double2* dst = F + inst * Fs + f * Ffs + b0;
       for (int i = threadIdx.x; i < T; i += blockDim.x) {
         const int x = xt.div(i), b = i - x * TB;
-        if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + f * TB + b];
+        if (b < tbv) dst[(long long)x * nb + b] = B[x * SR + f * TB + b];
       }
     }
   }
@@ -936,8 +943,10 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
   constexpr int NL = NC + (LR ? 1 : 0);     // fields held in LDS
   const XTile<TBC> xt(TBr, fTB, g.nx);
   const int TB = xt.TB;
-  extern __shared__ double2 sm[];    // [NL T] fields, then the nx twiddles
-  double2* s_tw = sm + NL * xt.nx * TB;
+  extern __shared__ double2 sm[];    // [nx][SR] fields, then the nx twiddles
+  constexpr int SRC = TBC > 0 ? x_row(NL * TBC) : 0;
+  const int SR = TBC > 0 ? SRC : NL * TB;   // LDS row stride
+  double2* s_tw = sm + xt.nx * SR;
   load_twiddles(s_tw, g.fx);
   const int inst = blockIdx.z;
   if (active && !active[inst]) return;
@@ -945,7 +954,7 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
   if (tile >= ntiles) return;
   const int T = xt.nx * TB;
   const int S6 = NL * TB;
-  double2* B = sm;    // [nx][NL TB]; the weighted sum goes to field 0's slot
+  double2* B = sm;    // [nx][SR]: NL fields of TB columns per row; the weighted sum goes to field 0's slot
   const long long nb = (long long)g.ny * g.nzh;
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
@@ -964,11 +973,11 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
       // columns past the tile's end hold zeros (selected per component: a
       // select of whole double2 values is lowered through scratch memory)
       const bool in = b < tbv;
-      B[x * S6 + b] = make_double2(in ? v0.x : 0.0, in ? v0.y : 0.0);
-      B[x * S6 + TB + b] = make_double2(in ? h.x : 0.0, in ? h.y : 0.0);
-      B[x * S6 + 2 * TB + b] = make_double2(in ? v2.x : 0.0, in ? v2.y : 0.0);
-      if constexpr (!XF::wlin) B[x * S6 + 3 * TB + b] = make_double2(in ? v5.x : 0.0, in ? v5.y : 0.0);
-      if (LR) B[x * S6 + NC * TB + b] = make_double2(in ? r.x : 0.0, in ? r.y : 0.0);
+      B[x * SR + b] = make_double2(in ? v0.x : 0.0, in ? v0.y : 0.0);
+      B[x * SR + TB + b] = make_double2(in ? h.x : 0.0, in ? h.y : 0.0);
+      B[x * SR + 2 * TB + b] = make_double2(in ? v2.x : 0.0, in ? v2.y : 0.0);
+      if constexpr (!XF::wlin) B[x * SR + 3 * TB + b] = make_double2(in ? v5.x : 0.0, in ? v5.y : 0.0);
+      if (LR) B[x * SR + NC * TB + b] = make_double2(in ? r.x : 0.0, in ? r.y : 0.0);
     }
   } else {
     for (int i0 = threadIdx.x; i0 < NL * T; i0 += LU * blockDim.x) {
@@ -987,13 +996,13 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
         if (i < NL * T) {
           const int f = (int)fdiv((unsigned)i, fT), r = i - f * T;
           const int x = (int)fdiv((unsigned)r, fTB), b = r - x * TB;
-          B[x * S6 + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
+          B[x * SR + f * TB + b] = (b < tbv) ? v[u] : make_double2(0.0, 0.0);
         }
       }
     }
   }
   __syncthreads();
-  x_fft<-1, TBC, NL * TBC, NL * TBC>(B, S6, fS6, S6, g, s_tw);
+  x_fft<-1, TBC, NL * TBC, SRC>(B, S6, fS6, SR, g, s_tw);
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = xt.div(i), b = i - x * TB;
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
@@ -1003,20 +1012,20 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
-      acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, XF::field(c), B[x * S6 + c * TB + b], 1), g.inv_npts));
+      acc = cadd(acc, cscale(apply_w(g, w, x, wy, wz, XF::field(c), B[x * SR + c * TB + b], 1), g.inv_npts));
     if (LR) {
-      const double2 r = B[x * S6 + NC * TB + b];
+      const double2 r = B[x * SR + NC * TB + b];
       if (MFA) acc = cadd(acc, cmul(r, fp.kmfa[(long long)x * nb + bb]));
       if (XF::wlin) acc = cadd(acc, cscale(r, fp.wl2 * g.inv_npts * (w.wD * w.wD)));
     }
-    B[x * S6 + b] = acc;    // this element's slots are read above by this thread only
+    B[x * SR + b] = acc;    // this element's slots are read above by this thread only
   }
   __syncthreads();
-  x_fft<1, TBC, TBC, NL * TBC>(B, TB, fTB, S6, g, s_tw);
+  x_fft<1, TBC, TBC, SRC>(B, TB, fTB, SR, g, s_tw);
   double2* dst = F + inst * Fs + b0;
   for (int i = threadIdx.x; i < T; i += blockDim.x) {
     const int x = xt.div(i), b = i - x * TB;
-    if (b < tbv) dst[(long long)x * nb + b] = B[x * S6 + b];
+    if (b < tbv) dst[(long long)x * nb + b] = B[x * SR + b];
   }
 }
 
