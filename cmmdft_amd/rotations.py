@@ -6,22 +6,30 @@ orientations: a Lebedev grid on the unit sphere for the direction of the
 guest's z axis (``generate_rotation_matrix(degree, 3)``), combined with
 ``degree`` equally spaced angles about z (``generate_rotation_matrix(degree,
 2)``).  Its Lebedev points and weights come from tabulated files
-(``rotations/lebedev/lebedev_<degree>_<size>.npz``); here the rules of degree
-3, 5, 7, 9 and 11 (6, 14, 26, 38 and 50 points) are generated from their
-octahedral orbits (V. I. Lebedev, Zh. Vychisl. Mat. Mat. Fiz. 16 (1976)
-293-306), in the tables' point order, with the tables' stored values: the
-weights of the rules of degree >= 7 are rounded to 15 decimals, and the two
-irrational orbit coordinates of the 38- and 50-point rules take the stored
-doubles (within one ulp of the closed forms
-q = sqrt((1 - 1/sqrt(3))/2) and l = 1/sqrt(11)).
+(``rotations/lebedev/lebedev_<degree>_<size>.npz``, degrees 3..131).  Here
+the rules of degree 3, 5, 7, 9 and 11 (6, 14, 26, 38 and 50 points) are
+generated from their closed-form octahedral orbits (V. I. Lebedev, Zh.
+Vychisl. Mat. Mat. Fiz. 16 (1976) 293-306), in the tables' point order, with
+the tables' stored values: the weights of the rules of degree >= 7 are
+rounded to 15 decimals, and the two irrational orbit coordinates of the 38-
+and 50-point rules take the stored doubles (within one ulp of the closed
+forms q = sqrt((1 - 1/sqrt(3))/2) and l = 1/sqrt(11)).  The rules of degree
+13..131 are rebuilt from their orbits -- one generator and one weight per
+orbit, and the order in which the tables list each orbit type's signed
+permutations -- kept in ``lebedev_orbits.npz`` (written from the tables by
+``make_lebedev_orbits.py``, which checks the rebuilt rules against them bit
+for bit).
 """
 from __future__ import annotations
+
+import os
 
 import numpy as np
 
 __all__ = ['LEBEDEV_DEGREES', 'lebedev', 'generate_rotation_matrix', 'combined_rotations']
 
-# degree -> number of points of the rules generated here
+# degree -> number of points of the rules built from closed-form orbits (the
+# other degrees the reference lists come from the orbit table)
 LEBEDEV_DEGREES = {3: 6, 5: 14, 7: 26, 9: 38, 11: 50}
 # every Lebedev degree the reference's tables hold (AngGrid.py:38-71); a
 # requested degree maps to the next one up, as AngularGrid does
@@ -83,18 +91,60 @@ def _b1(l, m):
 def _supported_degree(degree):
     if degree is None or degree < 0 or degree > _ALL_DEGREES[-1]:
         raise ValueError('Lebedev degree should be a positive integer <= %d, got %r' % (_ALL_DEGREES[-1], degree))
-    d = next(x for x in _ALL_DEGREES if x >= degree)
-    if d not in LEBEDEV_DEGREES:
-        raise NotImplementedError('Lebedev rules of degree %d (requested %d) are not available; supported degrees: %s'
-                                  % (d, degree, sorted(LEBEDEV_DEGREES)))
-    return d
+    return next(x for x in _ALL_DEGREES if x >= degree)
+
+
+_ORBITS = None
+
+
+def _orbits():
+    """The orbit table of the rules of degree 13..131 (lebedev_orbits.npz,
+    written by make_lebedev_orbits.py from the reference's tables)."""
+    global _ORBITS
+    if _ORBITS is None:
+        fn = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'lebedev_orbits.npz')
+        with np.load(fn, allow_pickle=False) as z:
+            _ORBITS = {k: z[k] for k in z.files}
+    return _ORBITS
+
+
+def _from_orbits(d):
+    """A rule rebuilt from its orbits, in the tables' order: the orbit types
+    in the order the table lists them (a1, a2, a3 as _a1/_a2/_a3; b (l, l, m),
+    c (p, q, 0) and d (r, s, t) as the signed permutations ops_<type> of each
+    orbit's generator, the orbits of a type interleaved, orbit index
+    fastest)."""
+    O = _orbits()
+    pts, wts = [], []
+    aw = O['a_%d' % d]
+    for t in O['order_%d' % d]:
+        t = int(t)
+        if t <= 2:
+            o = (_a1, _a2, _a3)[t]()
+            pts += o
+            wts += [aw[t]] * len(o)
+            continue
+        name = 'bcd'[t - 3]
+        gens = O['%s_%d' % (name, d)]
+        ops = O['ops_' + name]
+        perm = ops[:, :3].astype(int)
+        sgn = ops[:, 3:]
+        g = gens[:, :3]
+        block = np.stack([sgn[j] * g[:, perm[j]] for j in range(len(ops))])   # (ops, orbits, 3)
+        pts += [tuple(v) for v in block.reshape(-1, 3)]
+        wts += list(np.tile(gens[:, 3], len(ops)))
+    return np.array(pts, dtype=np.float64), np.array(wts, dtype=np.float64)
 
 
 def lebedev(degree):
     """(points (N, 3), weights (N,)) of the Lebedev rule of the given degree
     (or the next one up), weights summing to 1 as stored in the reference's
-    tables (AngularGrid multiplies them by 4 pi, AngGrid.py:169)."""
+    tables (AngularGrid multiplies them by 4 pi, AngGrid.py:169).  Degrees
+    3..11 are built from their closed-form orbits, 13..131 from the orbit
+    table."""
     d = _supported_degree(degree)
+    if d not in LEBEDEV_DEGREES:
+        return _from_orbits(d)
     if d == 3:
         orbits = [(_a1(), 1.0 / 6.0)]
     elif d == 5:

@@ -773,7 +773,7 @@ _LEBEDEV_FIXTURE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(
 
 def lebedev_table(degree):                          # AngGrid.py:150-169, 256-287
     """The reference's Lebedev points and weights x 4 pi (its tables, kept as a
-    test fixture); degrees 3, 5, 7, 9, 11."""
+    test fixture); degrees 3, 5, 7, 9, 11, 13 and 17."""
     with np.load(_LEBEDEV_FIXTURE, allow_pickle=False) as z:
         return z['points_%d' % degree], z['weights_%d' % degree] * 4 * np.pi
 

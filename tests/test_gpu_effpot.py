@@ -54,7 +54,8 @@ def _rel_err(V, Vr):
 
 
 @pytest.mark.parametrize('name,degree,shape', [('co2', 3, (16, 12, 10)), ('co2', 9, (16, 12, 10)),
-                                               ('c2h6', 5, (12, 12, 12))])
+                                               ('c2h6', 5, (12, 12, 12)), ('c2h6', 13, (8, 8, 8)),
+                                               ('co2', 17, (10, 8, 8))])
 def test_effective_potential_grid_matches_oracle(name, degree, shape):
     from cmmdft_amd.system import Grid
     from cmmdft_amd.extpot import get_external_potential_dict, generate_effective_potential
@@ -141,8 +142,8 @@ def test_argument_errors():
     host = W.host_case1()
     chk_fn, par_fn, chk, gs, ge = _guest('co2')
     gens = get_external_potential_dict(host.ff, par_fn, host, cutoff=W.CUTOFF)
-    with pytest.raises(NotImplementedError):
-        generate_effective_potential(chk_fn, np.zeros((4, 3)), BETA, gens, degree=13)
+    with pytest.raises(ValueError):
+        generate_effective_potential(chk_fn, np.zeros((4, 3)), BETA, gens, degree=133)
     with pytest.raises(KeyError):
         generate_effective_potential(chk_fn, np.zeros((4, 3)), BETA, {'C_CD': gens['C_CD']}, degree=5)
     with pytest.raises(TypeError):

@@ -53,6 +53,66 @@ def test_lebedev_rules_integrate_monomials_exactly(degree):
         assert abs(got - exact) <= 2e-14, (a, b, c, got, exact)
 
 
+def test_lebedev_orbit_table_equals_reference_tables():
+    """Every degree AngularGrid lists (3..131, AngGrid.py:38-71): the rule
+    rebuilt from the orbit table has the SHA-256 of the reference's table
+    (points and weights, float64), and equals the fixture tables of degree 13
+    and 17 bit for bit."""
+    import hashlib
+    import json
+    from cmmdft_amd.rotations import _ALL_DEGREES
+    with open(os.path.join(HERE, 'golden', 'lebedev_digests.json')) as fh:
+        dig = json.load(fh)
+    assert sorted(int(d) for d in dig) == _ALL_DEGREES
+    for d in _ALL_DEGREES:
+        p, w = lebedev(d)
+        assert len(p) == dig[str(d)]['size'], d
+        h = hashlib.sha256(np.ascontiguousarray(p).tobytes() + np.ascontiguousarray(w).tobytes()).hexdigest()
+        assert h == dig[str(d)]['sha256'], d
+    with np.load(os.path.join(HERE, 'golden', 'lebedev_ref.npz')) as z:
+        for d in (13, 17):
+            p, w = lebedev(d)
+            assert np.array_equal(p, z['points_%d' % d]) and np.array_equal(w, z['weights_%d' % d])
+    # a requested degree maps to the next one up, as AngularGrid does
+    assert len(lebedev(12)[0]) == 74 and len(lebedev(33)[0]) == 434
+    with pytest.raises(ValueError):
+        lebedev(132)
+
+
+@pytest.mark.parametrize('degree', [13, 17, 23, 35, 53, 83, 131])
+def test_lebedev_high_degree_rules_integrate_monomials_exactly(degree):
+    """Exactness of the rebuilt rules for every monomial of total degree
+    degree - 1 and degree, and for a sample of lower ones; points on the
+    unit sphere and weights summing to 1."""
+    p, w = lebedev(degree)
+    assert np.allclose(np.sum(p * p, axis=1), 1.0, atol=1e-15)
+    assert abs(w.sum() - 1.0) <= 1e-12
+    n = degree
+    px = np.stack([p[:, 0] ** k for k in range(n + 2)])
+    py = np.stack([p[:, 1] ** k for k in range(n + 2)])
+    pz = np.stack([p[:, 2] ** k for k in range(n + 2)])
+    rng = np.random.default_rng(degree)
+    tot = [n - 1, n] + [int(v) for v in rng.integers(0, n - 1, 4)]
+    for t in tot:
+        for a in range(t + 1):
+            for b in range(t - a + 1):
+                c = t - a - b
+                got = np.sum(w * px[a] * py[b] * pz[c])
+                exact = 0.0
+                if a % 2 == 0 and b % 2 == 0 and c % 2 == 0:
+                    exact = _dfact(a - 1) * _dfact(b - 1) * _dfact(c - 1) / _dfact(a + b + c + 1)
+                assert abs(got - exact) <= 1e-13, (a, b, c, got, exact)
+    # the low-degree rules are no more exact than stated: some even monomial
+    # of degree + 1 has a visible error (high degrees make those integrals
+    # too small to tell)
+    if n <= 23:
+        t = n + 1
+        err = max(abs(np.sum(w * px[a] * py[b] * pz[t - a - b]) -
+                      _dfact(a - 1) * _dfact(b - 1) * _dfact(t - a - b - 1) / _dfact(t + 1))
+                  for a in range(0, t + 1, 2) for b in range(0, t - a + 1, 2))
+        assert err > 1e-10, err
+
+
 @pytest.mark.parametrize('degree', DEGREES)
 def test_rotation_matrices(degree):
     R, w = O.rotation_matrix(degree, 3)
