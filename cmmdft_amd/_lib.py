@@ -27,6 +27,7 @@ S_CHK_NONFIN, S_CHK_RES2, S_CHK_DER2, S_CHK_DERN2 = 6, 7, 8, 9
 S_PN, S_PIUE, S_PDMAX, S_PNONFIN, S_PFID, S_PEXT = 16, 17, 18, 19, 20, 21
 S_ADJ_PQ, S_ADJ_RR, S_ADJ_LAM = 24, 25, 27
 S_FMFA = 28
+S_ADAMP = 30
 
 
 class CmmFunctional(C.Structure):
@@ -83,6 +84,7 @@ EXPORTS = {
     'cmm_anderson_mix': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                           C.c_void_p, C.c_void_p, C.c_int, C.c_double], C.c_int),
     'cmm_anderson_state': ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    'cmm_anderson_set_damping': ([C.c_void_p, C.c_void_p], C.c_int),
     'cmm_trial': ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double,
                    C.c_int], C.c_int),
     'cmm_get_scalars': ([C.c_void_p, C.c_void_p], C.c_int),
@@ -334,6 +336,10 @@ class Plan:
         check(self.lib.cmm_anderson_mix(self.h, _ptr(rhohist), _ints(order_rho), _ptr(rhist), _ints(order),
                                         len(order), _ptr(rho_cur), _ptr(rho_out), _ptr(vext), int(vext_batched),
                                         float(lam3)))
+
+    def anderson_set_damping(self, damping):
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(damping, dtype=np.float64), (self.nbatch,)))
+        check(self.lib.cmm_anderson_set_damping(self.h, d.ctypes.data_as(C.c_void_p)))
 
     def anderson_state(self):
         d = np.zeros(self.nbatch)

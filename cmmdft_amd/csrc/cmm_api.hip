@@ -1111,6 +1111,13 @@ int cmm_anderson_mix(cmm_plan* p, const double* rhohist, const int* order_rho, c
   return launch_produce<MODE_MIX>(p, el, "k_zproduce(mix)");
 }
 
+int cmm_anderson_set_damping(cmm_plan* p, const double* damping) {
+  if (!p || !damping) return fail(CMM_E_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(p->damping, damping, p->nb * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return CMM_OK;
+}
+
 int cmm_anderson_state(cmm_plan* p, double* damping, double* alpha) {
   if (!p) return fail(CMM_E_ARG, "null plan");
   if (damping) HIPCHK(hipMemcpyAsync(damping, p->damping, p->nb * sizeof(double), hipMemcpyDeviceToHost, p->stream));

@@ -21,6 +21,7 @@ constexpr int S_NCUR = 3;   // N, ideal sum, ext sum of rho, then per mode:  (fi
                             //          without the tolerance weights
 constexpr int S_GRAM = S_NCUR + 4;
 constexpr int S_FMFA = 28;  // (1/2) sum rho c_MFA (mean-field energy / dr)   (x weights pass)
+constexpr int S_ADAMP = 30; // the damping the last Anderson coefficients used  (k_anderson_coef)
 constexpr int S_PN = 16;    // N', sum|d|, max|d|, nonfinite, ideal', ext'  (density producers)
 
 // A z-pass tile: P pencils (B = P/2 complex sequences per field), S = nf*B

@@ -1929,6 +1929,7 @@ __global__ void __launch_bounds__(64) k_anderson_coef(AndDev ad, const int* __re
       else d = fmax(d * ad.fac_down, ad.dmin);
       ad.damping[inst] = d;
     }
+    red.scal[inst * NSCAL + S_ADAMP] = d;
     const double dc = d * ad.cf[inst];
     double* coef = ad.coef + inst * 2 * CMM_MMAX;
     for (int i = 0; i < CMM_MMAX; ++i) {

@@ -17,6 +17,7 @@ measures of the new density's own Boltzmann update; one more final pass
 """
 from __future__ import annotations
 
+import os
 import time
 
 import numpy as np
@@ -230,6 +231,18 @@ class Solver:
     def _omega_of(self, eng, meas):
         return eng.omega(self.chempot, meas['N'], meas['fid'], meas['ext'], meas['phi'], meas['fex'])
 
+    def _track_size(self):
+        """Length of the energy-tracking file (None without one)."""
+        fn = self.fener.fn_tracking
+        return os.path.getsize(fn) if fn is not None and os.path.exists(fn) else None
+
+    def _untrack(self, size):
+        """Drops the tracking lines written after the file had ``size`` bytes."""
+        fn = self.fener.fn_tracking
+        if fn is not None and size is not None and os.path.exists(fn):
+            with open(fn, 'r+') as f:
+                f.truncate(size)
+
     def _track(self, eng, meas, rho):
         """One line of the energy-tracking file for the density just measured
         (Solver._get_Omega / _track_energy, solver.py:103-126)."""
@@ -306,10 +319,10 @@ class Picard(Solver):
     # ------------------------------------------------------------------
     _nring = 1
 
-    def _alloc(self, eng, m):
+    def _alloc(self, eng, m, nrho=None):
         torch = torch_mod()
         n = eng.npts
-        self._rho_ring = torch.empty((m + 1, n), dtype=torch.float64, device='cuda')
+        self._rho_ring = torch.empty((nrho or m + 1, n), dtype=torch.float64, device='cuda')
         self._res_ring = torch.zeros((max(m, 1), n), dtype=torch.float64, device='cuda')
         self._tmp = torch.empty(n, dtype=torch.float64, device='cuda')
 
@@ -445,20 +458,39 @@ class Picard(Solver):
         self._advance()
         return ok
 
+    def _confirm(self):
+        """None when a density that meets the criteria is accepted; else the
+        step taken again (see Anderson)."""
+        return None
+
     def _solve(self, chempot, rho, log_level):
         eng, r0 = self._start(chempot, rho)
         tstart = time.perf_counter()
         s = self._begin(eng, r0)
         converged = False
         N_new = s[L.S_PN] * eng.dr
-        for istep in range(self.nsteps):
+        istep = 0
+        while istep < self.nsteps:
             t0 = time.perf_counter()
             self.curr_step = istep + 1
             info = self._step()
+            if info.get('redo'):
+                # the previous step was taken again (Anderson: its density
+                # had a non-finite Boltzmann factor)
+                istep -= 1
+                self.curr_step = istep + 1
             N_new = info['N']
-            if self._finish_step(info, t0, tstart):
-                converged = True
-                break
+            ok = self._finish_step(info, t0, tstart)
+            if ok:
+                info = self._confirm()
+                if info is None:
+                    converged = True
+                    break
+                N_new = info['N']
+                if self._finish_step(info, t0, tstart):
+                    converged = True
+                    break
+            istep += 1
         out = self._rho_ring[self._cur].reshape(self.grid.npoints).clone()
         return N_new, (out.cpu().numpy() if self.was_numpy else out), converged
 
@@ -521,6 +553,12 @@ class Anderson(Picard):
         self.damping_factors = damping_factors
         self.delta = delta
         self.anderson_log = []
+        # steps whose Anderson density is treated as having a non-finite
+        # Boltzmann factor (a host-side switch for tests of the fallback;
+        # empty in use), and the steps taken again as hybrid steps
+        self._force_nonfinite = ()
+        self.redone_steps = []
+        self._rb = None
 
     def _initiate_solving(self, chempot):           # solver.py:515-525
         super()._initiate_solving(chempot)
@@ -532,14 +570,20 @@ class Anderson(Picard):
     def _begin(self, eng, r0):
         m = self.m
         self._eng = eng
-        self._alloc(eng, m)
+        # m + 1 density slots hold the history and the new density; one more
+        # keeps a step's starting density while the next step runs, in case
+        # that step has to be taken again (_redo)
+        self._alloc(eng, m, m + 2)
         eng.plan.anderson_init([self.original_damping], [self.correction_factor])
         self.damping = self.original_damping
         self.And_true = False
         self.it_eps0 = np.nan
+        self.last_step_anderson = False
+        self.redone_steps = []
+        self._rb = None
         # history bookkeeping: entries (rho slot, residual slot), oldest first
         self._hist = []
-        self._free = list(range(m + 1))
+        self._free = list(range(m + 2))
         self._cur = self._free.pop(0)
         self._nsaved = 0
         ring = self._rho_ring
@@ -587,6 +631,10 @@ class Anderson(Picard):
             if self.curr_step < 3:
                 self.it_eps0 = it_eps
         cond = deferred or ((it_eps <= self.it_eps0 * self.delta) and self.curr_step > 4) or self.And_true
+        # this step's bookkeeping, to take it again as a hybrid step should
+        # its Anderson density turn out to have a non-finite Boltzmann factor
+        rb = dict(cur=cur, rslot=rslot, omega=prev_omega, n3max=self._n3max_cur, hist=list(hist),
+                  free=list(free), nsaved=self._nsaved, track=self._track_size())
         nxt = free.pop(0)
         used_anderson = False
         if cond:
@@ -601,12 +649,20 @@ class Anderson(Picard):
             eng.plan.derive()
             s = eng.plan.scalars()[0]
             if deferred:
-                if s[L.S_AND_NONFIN] > 0:
+                forced = self.last_step_anderson and (self.curr_step - 1) in self._force_nonfinite
+                if s[L.S_AND_NONFIN] > 0 or forced:
+                    # G of the current density is not finite.  If an
+                    # Anderson step produced it, the reference would have
+                    # taken a hybrid step there instead (solver.py:567-568)
+                    if self.last_step_anderson and self._rb is not None:
+                        return self._redo(self.curr_step - 1)
                     raise FloatingPointError('non-finite Boltzmann factor')
                 res_norm = np.sqrt(max(s[L.S_GRAM + rslot], 0.0))
                 N_cur = s[L.S_NCUR] * eng.dr
                 if self.curr_step < 3:
                     self.it_eps0 = res_norm / N_cur if N_cur != 0 else np.inf
+            rb['damping'] = s[L.S_ADAMP]
+            self._rb = rb
             if self.track_history:
                 d, a = eng.plan.anderson_state()
                 self.anderson_log.append((self.curr_step, a[0, :mk].copy(), float(d[0])))
@@ -634,6 +690,55 @@ class Anderson(Picard):
         self.last_step_anderson = used_anderson
         self._nxt = nxt
         return dict(s=s, N=s[L.S_PN] * eng.dr, iue=s[L.S_PIUE], dmax=s[L.S_PDMAX], produced=True)
+
+    def _redo(self, step):
+        """Takes the last step (``step``) again as a hybrid step from its starting
+        density: its Anderson density has a non-finite Boltzmann factor, so
+        the reference would have taken update_rho_hybrid there
+        (solver.py:561-568).  The history, the free ring slots, the grand
+        potential and max n3 of the starting density and the tracking file
+        return to that step; the damping keeps that step's update (the
+        reference adapts it before testing the new density,
+        solver.py:596-597) and drops the later one.  Returns the step's
+        info, flagged as a redo."""
+        rb = self._rb
+        self._rb = None
+        eng, m = self._eng, self.m
+        ring, rring = self._rho_ring, self._res_ring
+        self._hist, self._free, self._nsaved = list(rb['hist']), list(rb['free']), rb['nsaved']
+        self._cur = cur = rb['cur']
+        self.omega0 = rb['omega']
+        self._n3max_cur = rb['n3max']
+        eng.plan.anderson_set_damping([rb['damping']])
+        self._untrack(rb['track'])
+        if m == 1:
+            # the rejected density's residual took the only ring slot: the
+            # starting density's residual (and Gram row) again
+            eng.plan.seed(ring[cur], eng.V, False, eng.lam3)
+            eng.plan.derive()
+            eng.plan.finish_anderson(ring[cur], eng.V, False, rring, m, rb['rslot'], None, eng.beta, eng.lam3)
+        nxt = self._free.pop(0)
+        s = self._hybrid(eng, ring[cur], rb['rslot'], self._n3max_cur, ring[nxt])
+        if s[L.S_PNONFIN] > 0:
+            raise FloatingPointError('non-finite density')
+        self.last_step_anderson = False
+        self._nxt = nxt
+        self.redone_steps.append(step)
+        return dict(s=s, N=s[L.S_PN] * eng.dr, iue=s[L.S_PIUE], dmax=s[L.S_PDMAX], produced=True, redo=True)
+
+    def _confirm(self):
+        """A density that meets the criteria and came from an Anderson step is
+        accepted only if its Boltzmann factor is finite (one check pass);
+        else the step is taken again as a hybrid step (_redo), as the
+        reference's update_rho would have (solver.py:567-568)."""
+        if not self.last_step_anderson or self._rb is None:
+            return None
+        eng = self._eng
+        eng.plan.finish_check(self._rho_ring[self._cur], eng.V, False, eng.beta, eng.lam3, self.a_tol, self.r_tol)
+        c = eng.plan.scalars()[0]
+        if c[L.S_CHK_NONFIN] == 0 and self.curr_step not in self._force_nonfinite:
+            return None
+        return self._redo(self.curr_step)
 
     def _advance(self):
         prev_cur = self._cur

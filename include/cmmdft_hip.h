@@ -262,8 +262,14 @@ int cmm_anderson_mix(cmm_plan* plan, const double* rhohist, const int* order_rho
                      const int* order, int mk, const double* rho_cur, double* rho_out, const double* vext,
                      int vext_batched, double lam3);
 /* host copies of the per-instance damping and the last coefficients
- * (alpha[nbatch][CMM_MMAX_HIST]) */
+ * (alpha[nbatch][CMM_MMAX_HIST]); the damping the last coefficients used is
+ * also scalar [30] */
 int cmm_anderson_state(cmm_plan* plan, double* damping, double* alpha);
+/* set the per-instance damping (host array of nbatch): when the Anderson
+ * density of a step turns out to have a non-finite Boltzmann factor, that
+ * step is taken again as a hybrid Picard step, keeping its damping update
+ * and undoing the later one                     cmmdft/solver.py:537-544, 561-574 */
+int cmm_anderson_set_damping(cmm_plan* plan, const double* damping);
 /* rho_out = rho + alpha_inst * r_slot (clip to 1e-30 below 1e-12 if clip) */
 int cmm_trial(cmm_plan* plan, const double* rho, const double* rhist, int slot, double* rho_out,
               const double* vext, int vext_batched, double lam3, int clip);

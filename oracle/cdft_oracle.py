@@ -1253,6 +1253,9 @@ class Anderson(Picard):
         self.damping_max, self.damping_min = damping_max, damping_min
         self.adaptive, self.factors, self.delta = adaptive_damping, damping_factors, delta
         self.update_rho = self.update_rho_anderson_driver
+        # test switch: the Anderson density of these steps is treated as
+        # having a non-finite Boltzmann factor (a planted overflow)
+        self.force_nonfinite = ()
 
     def _initiate(self, mu):                        # solver.py:515-525
         super()._initiate(mu)
@@ -1291,6 +1294,8 @@ class Anderson(Picard):
             try:
                 rn, kn, C1n = self.anderson_step()
                 Gn = self.grho(C1n)
+                if self.curr_step in self.force_nonfinite:
+                    Gn = np.full_like(Gn, np.inf)
                 self.and_true = True
                 if np.isinf(Gn).any() or np.isnan(Gn).any():
                     rn, kn, C1n = self.update_rho_hybrid(rho, krho, C1)
