@@ -59,10 +59,12 @@ PASSES = (0, 1, 2, 3)
 
 
 def chain_fields(plan):
-    """Fields that travel between the passes of the six-field functional:
-    five (n2, n3, n_v2 x and y, rho_bar; the z component of n_v2 is i kz
-    times n3 and is formed in the z pass), four when the WDA is linear and
-    applied in k space (rho_bar is never formed)."""
+    """Fields the z pass of the six-field functional transforms: five (n2,
+    n3, n_v2 x and y, rho_bar; the z component of n_v2 is i kz times n3 and
+    is formed in the z pass), four when the WDA is linear and applied in k
+    space (rho_bar is never formed).  The paired y passes move them (the y
+    component formed from n3, or folded into n3's derivative, there); the x
+    passes one fewer."""
     return 4 if plan.info['wda_kspace'] else 5
 
 
@@ -490,7 +492,7 @@ def main():
     points_per_s = value
     iters_per_point = float(np.mean(iters))
 
-    # dominant pass: per-launch time x launches per iteration (2 y passes of 5 fields),
+    # dominant pass: per-launch time x launches per iteration (2 paired y passes),
     # timed on the single-instance plan (solve one point first if only the batch ran)
     if plain_pps is None:
         solver.solve(mus[0], prog._set_initial_density(chempot=mus[0], temperature=W.TEMPERATURE))

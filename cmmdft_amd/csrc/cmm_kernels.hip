@@ -727,18 +727,19 @@ constexpr int NTX = 512;
 // Six-field functionals (NF = 6) weight and transform four fields only: on a
 // column of the x pass ky and kz are constant, so the y and z components of
 // n_v2, i ky w3 rho^ and i kz w3 rho^ (zero on their axis' Nyquist plane),
-// are i ky' and i kz' times the n3 column before and after the x transform,
-// and in the back-convolution i ky' G_vy joins G_n3 before it (the z pass has
-// folded i kz' G_vz into G_n3 already: kz is constant along y and x too, so
-// the z component is not stored between the passes at all, see
-// z_load_half_multi; vz = 1 stores it for callers that read the fields).
+// are i ky' and i kz' times the n3 column before and after the x transform.
+// ky is the y pass's own frequency and kz the z pass's: the paired y pass
+// forms the y component from n3 and folds i ky' G_vy into G_n3 (k_ypass,
+// PAIR), the z pass forms the z component and folds i kz' G_vz
+// (z_load_half_multi), so neither is stored by the x passes (vz = 1 stores
+// both, for callers that read the fields).
 // Slots 0..3 hold the fields n2, n3, nv2_x, rho_bar (weights 0, 1, 2, 5).
 // NF = 5: the six-field functional whose WDA is linear (every EOS term mean
 // field, f_ex = A rho_bar^2): rho_bar is not formed; its energy
 // A sum rho_bar^2 = (A/N) sum_k |w_D rho^|^2 and its derivative term
 // w_D * (2A rho_bar) = IFFT(2A w_D^2 rho^) are taken in k space by the x
 // passes, so slots 0..2 hold n2, n3, nv2_x and four fields travel through
-// the y and z passes instead of five.
+// the z pass instead of five.
 //
 // TBC > 0: the tile fixed at compile time -- nx = 128 with the radix sequence
 // 8, 4, 4 and TB = TBC columns, in blocks of 128 TBC threads -- so LDS
