@@ -121,14 +121,24 @@ def time_passes(plan, reps=20):
 
 
 def host_cpu_model():
-    """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
+    """The host CPU's model name with its family / model numbers and the
+    logical CPUs (/proc/cpuinfo), for the CPU baseline (a virtualised host
+    may report only the brand in the model name)."""
+    info, ncpu = {}, 0
     try:
         for line in open('/proc/cpuinfo'):
-            if line.startswith('model name'):
-                return line.split(':', 1)[1].strip()
+            k, _, v = line.partition(':')
+            k = k.strip()
+            if k == 'processor':
+                ncpu += 1
+            if k in ('model name', 'cpu family', 'model', 'stepping') and k not in info:
+                info[k] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if 'model name' not in info:
+        return None
+    return '%s (family %s, model %s, stepping %s; %d logical CPUs)' % (
+        info['model name'], info.get('cpu family', '?'), info.get('model', '?'), info.get('stepping', '?'), ncpu)
 
 
 def cpu_baseline(V, shape, mu, gpu_iters_per_point, nsample, cores=1):
