@@ -842,6 +842,15 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = rz + inst * rzs + b0;
+  // compile-time tile: one element per thread, whose weights (table reads)
+  // are loaded with the tile so that the transform hides their latency
+  KW wpre{};
+  if constexpr (TBC > 0) {
+    const int x = xt.div(threadIdx.x), b = (int)threadIdx.x - x * TB;
+    const unsigned bb = (unsigned)min(b0 + b, nb - 1);
+    const int wy = (int)fdiv(bb, fnzh);
+    wpre = x_kw(g, fp, x, wy, (int)bb - wy * g.nzh, (long long)x * nb + bb);
+  }
   for (int i0 = threadIdx.x; i0 < T; i0 += LU * blockDim.x) {
     double2 v[LU];
 #pragma unroll
@@ -870,7 +879,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
-    const KW w = x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
+    const KW w = TBC > 0 ? wpre : x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
     const double2 a = A[x * SR + b];
     if (E && b < tbv) {
       const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
