@@ -230,50 +230,6 @@ __device__ __forceinline__ double& z_val_c(double2* lds, int f, int pl, int e) {
   return reinterpret_cast<double*>(lds)[2 * (e * TC::ES + f * TC::B + (pl >> 1)) + (pl & 1)];
 }
 
-template <bool VZ, class TC>
-__device__ __forceinline__ void z_load_half_multi_c(double2* lds, const double2* src, long long fs, long long p0,
-                                                    long long npen, const double* kzk, int kzn) {
-  constexpr int n = TC::N, nzh = TC::NZH;
-  constexpr int per = TC::B * nzh;
-  constexpr int tot = TC::NF * per;
-  const int ip0 = (int)p0, ilast = (int)npen - 1;
-  for (int i0 = threadIdx.x; i0 < tot; i0 += LU * NT) {
-    double2 A[LU], B[LU];
-#pragma unroll
-    for (int u = 0; u < LU; ++u) {
-      const int i = min(i0 + u * NT, tot - 1);
-      const int f = (int)((unsigned)i / (unsigned)per);
-      const int r = i - f * per;
-      const int q = (int)((unsigned)r / (unsigned)nzh), k = r - q * nzh;
-      const int pa = ip0 + 2 * q;
-      const double2* sf = src + ((VZ && f == 4) ? 1 : f) * fs;
-      A[u] = sf[min(pa, ilast) * nzh + k];
-      B[u] = sf[min(pa + 1, ilast) * nzh + k];
-    }
-#pragma unroll
-    for (int u = 0; u < LU; ++u) {
-      const int i = i0 + u * NT;
-      if (i < tot) {
-        const int f = (int)((unsigned)i / (unsigned)per);
-        const int r = i - f * per;
-        const int q = (int)((unsigned)r / (unsigned)nzh), k = r - q * nzh;
-        const int pa = ip0 + 2 * q;
-        double2 a = (pa <= ilast) ? A[u] : make_double2(0.0, 0.0);
-        double2 b = (pa + 1 <= ilast) ? B[u] : make_double2(0.0, 0.0);
-        if (VZ && f == 4) {
-          const double kk = kz_of(kzk, kzn, k);
-          a = cmul_i(cscale(a, kk), -1.0);
-          b = cmul_i(cscale(b, kk), -1.0);
-        }
-        const bool selfconj = (k == 0) || (2 * k == n);
-        if (selfconj) { a.y = 0.0; b.y = 0.0; }
-        lds[k * TC::ES + f * TC::B + q] = make_double2(a.x - b.y, a.y + b.x);
-        if (!selfconj) lds[(n - k) * TC::ES + f * TC::B + q] = make_double2(a.x + b.y, b.x - a.y);
-      }
-    }
-  }
-}
-
 template <class TC>
 __device__ __forceinline__ void z_unpack_c(const double2* lds, int f, int q, int k, double2& A, double2& Bv) {
   const double2 zk = lds[k * TC::ES + f * TC::B + q];
@@ -282,31 +238,74 @@ __device__ __forceinline__ void z_unpack_c(const double2* lds, int f, int q, int
   Bv = make_double2(0.5 * (zk.y + zm.y), -0.5 * (zk.x - zm.x));
 }
 
-template <bool VZ, class TC>
-__device__ __forceinline__ void z_store_half_multi_c(const double2* lds, double2* dst, long long fs, long long p0,
-                                                     long long npen, const double* kzk, int kzn) {
-  constexpr int nzh = TC::NZH;
-  constexpr int per = TC::B * nzh;
-  constexpr int tot = (VZ ? TC::NF - 1 : TC::NF) * per;
-#pragma unroll 4
-  for (int i = threadIdx.x; i < tot; i += NT) {
-    const int fs0 = (int)((unsigned)i / (unsigned)per);
-    const int r = i - fs0 * per;
-    const int f = (VZ && fs0 >= 4) ? fs0 + 1 : fs0;
-    const int q = (int)((unsigned)r / (unsigned)nzh), k = r - q * nzh;
-    double2 A, Bv;
-    z_unpack_c<TC>(lds, f, q, k, A, Bv);
-    if (VZ && f == 1) {
-      double2 A4, B4;
-      z_unpack_c<TC>(lds, 4, q, k, A4, B4);
-      const double kk = kz_of(kzk, kzn, k);
-      A = cadd(A, cmul_i(cscale(A4, kk), 1.0));
-      Bv = cadd(Bv, cmul_i(cscale(B4, kk), 1.0));
+// The tile's half-spectrum fields by columns: one thread per (pencil pair q,
+// kz k) column, holding that column of every field, so the index arithmetic
+// is done once per column rather than once per field, and field 4 (VZ: the z
+// component of n_v2, -i kz times field 1's spectrum, zero on the Nyquist
+// plane) is formed from field 1's loaded values instead of loading them
+// again.  FULL: every pencil of the tile exists (all tiles but the grid's
+// last), so no address is clamped and no value zero-filled.  Per element the
+// same operations as the runtime tile's z_load_half_multi / z_store_half_multi.
+template <bool VZ, class TC, bool FULL>
+__device__ __forceinline__ void z_load_cols_c(double2* lds, const double2* src, long long fs, long long p0,
+                                              long long npen, const double* kzk, int kzn) {
+  constexpr int n = TC::N, nzh = TC::NZH, NF = TC::NF;
+  constexpr int ncol = TC::B * nzh;
+  const int ip0 = (int)p0, ilast = (int)npen - 1;
+  for (int c = threadIdx.x; c < ncol; c += NT) {
+    const int q = (int)((unsigned)c / (unsigned)nzh), k = c - q * nzh;
+    const int pa = ip0 + 2 * q;
+    const int oa = (FULL ? pa : min(pa, ilast)) * nzh + k;
+    const int ob = (FULL ? pa + 1 : min(pa + 1, ilast)) * nzh + k;
+    double2 A[NF], B[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (VZ && f == 4) continue;
+      A[f] = src[f * fs + oa];
+      B[f] = src[f * fs + ob];
     }
+    const double kk = VZ ? kz_of(kzk, kzn, k) : 0.0;
+    const bool selfconj = (k == 0) || (2 * k == n);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int fl = (VZ && f == 4) ? 1 : f;
+      double2 a = (FULL || pa <= ilast) ? A[fl] : make_double2(0.0, 0.0);
+      double2 b = (FULL || pa + 1 <= ilast) ? B[fl] : make_double2(0.0, 0.0);
+      if (VZ && f == 4) {
+        a = cmul_i(cscale(a, kk), -1.0);
+        b = cmul_i(cscale(b, kk), -1.0);
+      }
+      if (selfconj) { a.y = 0.0; b.y = 0.0; }
+      lds[k * TC::ES + f * TC::B + q] = make_double2(a.x - b.y, a.y + b.x);
+      if (!selfconj) lds[(n - k) * TC::ES + f * TC::B + q] = make_double2(a.x + b.y, b.x - a.y);
+    }
+  }
+}
+
+template <bool VZ, class TC, bool FULL>
+__device__ __forceinline__ void z_store_cols_c(const double2* lds, double2* dst, long long fs, long long p0,
+                                               long long npen, const double* kzk, int kzn) {
+  constexpr int nzh = TC::NZH, NF = TC::NF;
+  constexpr int ncol = TC::B * nzh;
+  for (int c = threadIdx.x; c < ncol; c += NT) {
+    const int q = (int)((unsigned)c / (unsigned)nzh), k = c - q * nzh;
     const int pa = (int)p0 + 2 * q, o = pa * nzh + k;
-    double2* df = dst + f * fs;
-    if (pa < (int)npen) df[o] = A;
-    if (pa + 1 < (int)npen) df[o + nzh] = Bv;
+    const double kk = VZ ? kz_of(kzk, kzn, k) : 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (VZ && f == 4) continue;
+      double2 A, Bv;
+      z_unpack_c<TC>(lds, f, q, k, A, Bv);
+      if (VZ && f == 1) {
+        double2 A4, B4;
+        z_unpack_c<TC>(lds, 4, q, k, A4, B4);
+        A = cadd(A, cmul_i(cscale(A4, kk), 1.0));
+        Bv = cadd(Bv, cmul_i(cscale(B4, kk), 1.0));
+      }
+      double2* df = dst + f * fs;
+      if (FULL || pa < (int)npen) df[o] = A;
+      if (FULL || pa + 1 < (int)npen) df[o + nzh] = Bv;
+    }
   }
 }
 
@@ -1408,7 +1407,9 @@ __global__ void __launch_bounds__(NT) k_zphi128(double2* __restrict__ F, long lo
   const long long npen = (long long)g.nx * g.ny;
   const long long p0 = (long long)blockIdx.x * P;
   constexpr bool VZ = (NF == 5 || NF == 6);
-  z_load_half_multi_c<VZ, TC>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
+  const bool full = p0 + P <= npen;   // every pencil of the tile exists
+  if (full) z_load_cols_c<VZ, TC, true>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
+  else z_load_cols_c<VZ, TC, false>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft_c<1, TC>(sm, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
@@ -1430,7 +1431,8 @@ __global__ void __launch_bounds__(NT) k_zphi128(double2* __restrict__ F, long lo
   if (store) {
     __syncthreads();
     z_fft_c<-1, TC>(sm, s_tw);
-    z_store_half_multi_c<VZ, TC>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
+    if (full) z_store_cols_c<VZ, TC, true>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
+    else z_store_cols_c<VZ, TC, false>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
