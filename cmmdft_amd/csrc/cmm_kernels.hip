@@ -265,7 +265,8 @@ __device__ __forceinline__ void z_load_cols_c(double2* lds, const double2* src, 
       B[f] = src[f * fs + ob];
     }
     const double kk = VZ ? kz_of(kzk, kzn, k) : 0.0;
-    const bool selfconj = (k == 0) || (2 * k == n);
+    const bool selfconj = (k & (n / 2 - 1)) == 0;   // k = 0 or n/2 (k <= n/2)
+    double2 hi[NF];   // the values of kz slot n - k, stored after the loop
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       const int fl = (VZ && f == 4) ? 1 : f;
@@ -277,7 +278,11 @@ __device__ __forceinline__ void z_load_cols_c(double2* lds, const double2* src, 
       }
       if (selfconj) { a.y = 0.0; b.y = 0.0; }
       lds[k * TC::ES + f * TC::B + q] = make_double2(a.x - b.y, a.y + b.x);
-      if (!selfconj) lds[(n - k) * TC::ES + f * TC::B + q] = make_double2(a.x + b.y, b.x - a.y);
+      hi[f] = make_double2(a.x + b.y, b.x - a.y);
+    }
+    if (!selfconj) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) lds[(n - k) * TC::ES + f * TC::B + q] = hi[f];
     }
   }
 }
