@@ -56,9 +56,12 @@ def _same(out, ref):
         assert out[key][1][L.S_N3MAX] == out[ref][1][L.S_N3MAX]
 
 
-@pytest.mark.parametrize('shape', [(128, 128, 128), (16, 20, 128)])
+@pytest.mark.parametrize('shape', [(128, 128, 128), (16, 20, 128), (10, 25, 128)])
 @pytest.mark.parametrize('kspace', [True, False])
 def test_tiles_give_the_same_derivative(shape, kspace):
+    """Every pencil count gives the same bits.  The grids' last tiles are
+    partial for P = 6 at 16 x 20 and for P = 4, 6 and 8 at 10 x 25 (250
+    pencils), where the loader clamps and zero-fills; P = 2 tiles are full."""
     out = _derive_at_tiles(shape, kspace, [(P, dict(p_phi=P)) for P in (2, 4, 6, 8)])
     _same(out, 6)
 
