@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
   constexpr bool VZ = (NF == 5 || NF == 6);
-  z_load_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  z_load_tile<VZ, NF>(sm, F + inst * Fs, Ffs, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft<1>(sm, t, g.fz, s_tw);
   const double* nb = nfld + (long long)inst * (NF == 5 ? 6 : NF) * npts;   // six fields when NF = 5
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(NT) k_zhess(double2* __restrict__ F, long long
   }
   __syncthreads();
   z_fft<-1>(sm, t, g.fz, s_tw);
-  z_store_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  z_store_tile<VZ, NF>(sm, F + inst * Fs, Ffs, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
 }
 
 // CG direction and its transform:

@@ -165,6 +165,54 @@ __device__ __forceinline__ void z_load_half_multi(double2* lds, const double2* s
   }
 }
 
+// The same fields of a runtime-shaped tile by columns: one thread per (pencil
+// pair q, kz k) column holding that column of all NF fields, the index
+// arithmetic once per column and field 4 (VZ) formed from field 1's loaded
+// values; FULL: every pencil of the tile exists.  Per element the operations
+// of z_load_half_multi.  NF <= 6 (a column of twelve fields would hold 48
+// complex values in registers).
+template <bool VZ, int NF, bool FULL>
+__device__ __forceinline__ void z_load_cols(double2* lds, const double2* src, long long fs, long long p0,
+                                            const ZTile& t, long long npen, int n, int nzh, const double* kzk,
+                                            int kzn) {
+  static_assert(NF <= 6, "column loads hold every field of a column in registers");
+  const int ncol = t.B * nzh;
+  const int ip0 = (int)p0, ilast = (int)npen - 1;
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+    const int q = (int)fdiv((unsigned)c, t.fnzh), k = c - q * nzh;
+    const int pa = ip0 + 2 * q;
+    const int oa = (FULL ? pa : min(pa, ilast)) * nzh + k;
+    const int ob = (FULL ? pa + 1 : min(pa + 1, ilast)) * nzh + k;
+    double2 A[NF], B[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (VZ && f == 4) continue;
+      A[f] = src[f * fs + oa];
+      B[f] = src[f * fs + ob];
+    }
+    const double kk = VZ ? kz_of(kzk, kzn, k) : 0.0;
+    const bool selfconj = (k == 0) || (2 * k == n);
+    double2 hi[NF];   // the values of kz slot n - k, stored after the loop
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int fl = (VZ && f == 4) ? 1 : f;
+      double2 a = (FULL || pa <= ilast) ? A[fl] : make_double2(0.0, 0.0);
+      double2 b = (FULL || pa + 1 <= ilast) ? B[fl] : make_double2(0.0, 0.0);
+      if (VZ && f == 4) {
+        a = cmul_i(cscale(a, kk), -1.0);
+        b = cmul_i(cscale(b, kk), -1.0);
+      }
+      if (selfconj) { a.y = 0.0; b.y = 0.0; }
+      lds[k * t.ES + f * t.B + q] = make_double2(a.x - b.y, a.y + b.x);
+      hi[f] = make_double2(a.x + b.y, b.x - a.y);
+    }
+    if (!selfconj) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) lds[(n - k) * t.ES + f * t.B + q] = hi[f];
+    }
+  }
+}
+
 // the two half spectra of pair q of field f at index k of a transformed tile
 __device__ __forceinline__ void z_unpack(const double2* lds, const ZTile& t, int n, int f, int q, int k, double2& A,
                                          double2& Bv) {
@@ -202,6 +250,62 @@ __device__ __forceinline__ void z_store_half_multi(const double2* lds, double2* 
     double2* df = dst + f * fs;
     if (pa < (int)npen) df[o] = A;
     if (pa + 1 < (int)npen) df[o + nzh] = Bv;
+  }
+}
+
+// The transformed tile's fields by columns (see z_load_cols): per element
+// the operations of z_store_half_multi.
+template <bool VZ, int NF, bool FULL>
+__device__ __forceinline__ void z_store_cols(const double2* lds, double2* dst, long long fs, long long p0,
+                                             const ZTile& t, long long npen, int n, int nzh, const double* kzk,
+                                             int kzn) {
+  static_assert(NF <= 6, "column stores are built for six fields at most");
+  const int ncol = t.B * nzh;
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+    const int q = (int)fdiv((unsigned)c, t.fnzh), k = c - q * nzh;
+    const int pa = (int)p0 + 2 * q, o = pa * nzh + k;
+    const double kk = VZ ? kz_of(kzk, kzn, k) : 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      if (VZ && f == 4) continue;
+      double2 A, Bv;
+      z_unpack(lds, t, n, f, q, k, A, Bv);
+      if (VZ && f == 1) {
+        double2 A4, B4;
+        z_unpack(lds, t, n, 4, q, k, A4, B4);
+        A = cadd(A, cmul_i(cscale(A4, kk), 1.0));
+        Bv = cadd(Bv, cmul_i(cscale(B4, kk), 1.0));
+      }
+      double2* df = dst + f * fs;
+      if (FULL || pa < (int)npen) df[o] = A;
+      if (FULL || pa + 1 < (int)npen) df[o + nzh] = Bv;
+    }
+  }
+}
+
+// A z tile's NF fields in and out: by columns for up to six fields (the
+// full-tile form on every tile but the grid's last), else per element.
+template <bool VZ, int NF>
+__device__ __forceinline__ void z_load_tile(double2* lds, const double2* src, long long fs, long long p0,
+                                            const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn,
+                                            const double* kzk, int kzn) {
+  if constexpr (NF <= 6) {
+    if (p0 + t.P <= npen) z_load_cols<VZ, NF, true>(lds, src, fs, p0, t, npen, n, nzh, kzk, kzn);
+    else z_load_cols<VZ, NF, false>(lds, src, fs, p0, t, npen, n, nzh, kzk, kzn);
+  } else {
+    z_load_half_multi<VZ>(lds, src, fs, NF, p0, t, npen, n, nzh, fBn, kzk, kzn);
+  }
+}
+
+template <bool VZ, int NF>
+__device__ __forceinline__ void z_store_tile(const double2* lds, double2* dst, long long fs, long long p0,
+                                             const ZTile& t, long long npen, int n, int nzh, const FastDiv& fBn,
+                                             const double* kzk, int kzn) {
+  if constexpr (NF <= 6) {
+    if (p0 + t.P <= npen) z_store_cols<VZ, NF, true>(lds, dst, fs, p0, t, npen, n, nzh, kzk, kzn);
+    else z_store_cols<VZ, NF, false>(lds, dst, fs, p0, t, npen, n, nzh, kzk, kzn);
+  } else {
+    z_store_half_multi<VZ>(lds, dst, fs, NF, p0, t, npen, n, nzh, fBn, kzk, kzn);
   }
 }
 
@@ -1364,7 +1468,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   const long long p0 = (long long)blockIdx.x * t.P;
   const FastDiv fBn = make_fastdiv((unsigned)(t.B * g.nzh));
   constexpr bool VZ = (NF == 5 || NF == 6);   // n_v2's z component formed from n3 here
-  z_load_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+  z_load_tile<VZ, NF>(sm, F + inst * Fs, Ffs, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   __syncthreads();
   z_fft<1, true>(sm, t, g.fz, s_tw);
   double acc[3] = {0.0, 0.0, -INFINITY};
@@ -1389,7 +1493,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   if (store) {
     __syncthreads();
     z_fft<-1, true>(sm, t, g.fz, s_tw);
-    z_store_half_multi<VZ>(sm, F + inst * Fs, Ffs, NF, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
+    z_store_tile<VZ, NF>(sm, F + inst * Fs, Ffs, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
   block_reduce<3>(acc, ops, sh);
@@ -1397,9 +1501,10 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
 }
 
 // k_zphi for nz = 128 with the tile fixed at compile time (P pencils): the
-// same operations in the same order, so the same results bit for bit at the
-// same P.  Its LDS addressing and index arithmetic are constants, which cuts
-// the integer work that otherwise issues beside the FP64 work.
+// same operations per element (the results agree with k_zphi's to rounding:
+// the compiler contracts the kz products differently).  Its LDS addressing
+// and index arithmetic are constants, which cuts the integer work that
+// otherwise issues beside the FP64 work.
 template <int NF, int P>
 __global__ void __launch_bounds__(NT) k_zphi128(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g,
                                                FuncDev fp, int store, const int* __restrict__ active, RedDev red) {
