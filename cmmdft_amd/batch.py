@@ -192,8 +192,13 @@ class BatchedAnderson:
                 s += 1
                 self._iteration(s)
             self._s = s
-            self._st = self._state()
-            self._graphed = bool(self.use_graph and self._st['active'].any() and s < self.nsteps)
+            # no read of the status block here (it would wait for this
+            # stream): solvers started one after another (UnevenBatch) then
+            # run their warm-ups concurrently.  The period is captured even
+            # when every instance has already finished; replays over
+            # finished instances are fully masked and change nothing.
+            self._st = None
+            self._graphed = bool(self.use_graph and s < self.nsteps)
             if self._graphed and self._graphs is None:
                 p0 = self._groups[0]['plan']
                 seg = m + 1
@@ -213,6 +218,8 @@ class BatchedAnderson:
     def running(self):
         """True while some instance is active and the step limit is not reached
         (as of the last poll)."""
+        if self._st is None:   # not polled since start()
+            return self._s < self.nsteps
         return bool(self._st['active'].any()) and self._s < self.nsteps
 
     def advance(self):
@@ -245,6 +252,8 @@ class BatchedAnderson:
         iters, status, omega, resolved)."""
         torch = torch_mod()
         B = len(self.chempots)
+        if self._st is None:
+            self._st = self._state()
         st = self._st
         with torch.cuda.stream(self.stream):
             scal = np.concatenate([g['plan'].scalars() for g in self._groups])
