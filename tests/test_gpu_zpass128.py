@@ -66,6 +66,16 @@ def test_tiles_give_the_same_derivative(shape, kspace):
     _same(out, 6)
 
 
+@pytest.mark.parametrize('kspace', [True, False])
+def test_runtime_z_tiles_give_the_same_derivative(kspace):
+    """The runtime-shaped z pass (nz != 128), which loads and stores its tile
+    by (pencil pair, kz) columns like the compile-time one: every pencil
+    count gives the same bits, with partial last tiles at 10 x 25 (250
+    pencils) for P = 4, 6 and 8 and full ones for P = 2 and 10."""
+    out = _derive_at_tiles((10, 25, 24), kspace, [(P, dict(p_phi=P)) for P in (2, 4, 6, 8, 10)])
+    _same(out, 2)
+
+
 @pytest.mark.parametrize('shape', [(128, 128, 128), (128, 24, 20)])
 @pytest.mark.parametrize('kspace,mfa', [(True, False), (False, False), (True, True)])
 def test_x_tiles_give_the_same_derivative(shape, kspace, mfa):
