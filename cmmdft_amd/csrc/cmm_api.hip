@@ -50,7 +50,7 @@ struct cmm_plan {
   int has_func;
   double2* rz;
   double2* F;
-  double* kwt;       // the functional's weights w2, w3, w_D on this grid [3][nhalf] (x passes)
+  double* kwt;       // the functional's weights w2, w3, w_D on this grid, rows x <= nx/2 [3][..] (x passes)
   long long nhalf;   // complex elements per half spectrum
   long long npts;
   double* partials;
@@ -405,14 +405,16 @@ int cmm_set_functional(cmm_plan* p, const cmm_functional* f) {
   d.inv4piR2 = 1.0 / (4.0 * M_PI * d.R * d.R);
   d.inv4piR = 1.0 / (4.0 * M_PI * d.R);
   d.c_tor = 3.0 / d.R;
-  // the x passes' weights of every half-spectrum element, computed once per
-  // functional (they depend on R, D and the grid) on the plan's stream
-  if (!p->kwt) HIPCHK(hipMalloc(&p->kwt, (size_t)3 * p->nhalf * sizeof(double)));
+  // the x passes' weights of the half-spectrum elements with x <= nx/2 (those
+  // of x and nx - x are equal), computed once per functional (they depend on
+  // R, D and the grid) on the plan's stream
+  const long long kwt_n = (long long)(p->nx / 2 + 1) * p->ny * p->nzh;   // rows x <= nx/2
+  if (!p->kwt) HIPCHK(hipMalloc(&p->kwt, (size_t)3 * kwt_n * sizeof(double)));
   hipLaunchKernelGGL(k_kw_table, dim3((unsigned)std::min<long long>((p->nhalf + NT - 1) / NT, 65536)), dim3(NT), 0,
                      p->stream, p->g, d, p->kwt);
   LAUNCHCHK("k_kw_table");
   d.kwt = p->kwt;
-  d.kwt_n = p->nhalf;
+  d.kwt_n = kwt_n;
   p->has_func = 1;
   return CMM_OK;
 }

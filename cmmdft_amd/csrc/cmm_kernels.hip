@@ -768,11 +768,16 @@ __device__ __forceinline__ KW kweights(const GridDev& g, const FuncDev& fp, int 
   return r;
 }
 
-// The weights w2, w3, w_D of every half-spectrum element (flat index
-// x (ny nzh) + y nzh + z) into out[3][n]: the table the x passes read.
+// The weights w2, w3, w_D of the half-spectrum elements with x <= nx/2
+// (flat index x (ny nzh) + y nzh + z) into out[3][n]: the table the x passes
+// read.  Elements x and nx - x have the same weights bit for bit (their kx
+// differ in sign only, and the Lanczos factor is even), so the passes read
+// row min(x, nx - x), and a block, which holds every x of its columns, reads
+// each row twice at nearly the same time: the second read is served by the
+// cache instead of HBM.
 __global__ void __launch_bounds__(NT) k_kw_table(GridDev g, FuncDev fp, double* __restrict__ out) {
-  const long long n = (long long)g.nx * g.ny * g.nzh;
   const long long nb = (long long)g.ny * g.nzh;
+  const long long n = (long long)(g.nx / 2 + 1) * nb;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const int x = (int)(i / nb);
     const long long bb = i - x * nb;
@@ -888,8 +893,9 @@ __device__ __forceinline__ void x_fft(double2* buf, int batch, const FastDiv& fb
 // Weights of x-pass element (x, wy, wz), flat half-spectrum index idx: from
 // the plan's table (filled by k_kw_table with kweights' values) when there is
 // one, else computed.  The k = 0 flag and wt are formed as in kweights.
-__device__ __forceinline__ KW x_kw(const GridDev& g, const FuncDev& fp, int x, int wy, int wz, long long idx) {
+__device__ __forceinline__ KW x_kw(const GridDev& g, const FuncDev& fp, int x, int wy, int wz, long long bb) {
   if (fp.kwt) {
+    const long long idx = (long long)min(x, g.nx - x) * ((long long)g.ny * g.nzh) + bb;
     KW w;
     w.w2 = fp.kwt[idx];
     w.w3 = fp.kwt[fp.kwt_n + idx];
@@ -988,7 +994,7 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
-    const KW w = TBC > 0 ? wpre : x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
+    const KW w = TBC > 0 ? wpre : x_kw(g, fp, x, wy, wz, bb);
     const double2 a = A[x * SR + b];
     if (E && b < tbv) {
       const double half = (wz == 0 || wz == g.kz.nyq) ? 0.5 : 1.0;
@@ -1127,7 +1133,7 @@ k_xback(double2* __restrict__ F, long long Fs, long long Ffs, GridDev g, FuncDev
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
     const int wz = (int)bb - wy * g.nzh;
-    const KW w = x_kw(g, fp, x, wy, wz, (long long)x * nb + bb);
+    const KW w = x_kw(g, fp, x, wy, wz, bb);
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
     for (int c = 0; c < NC; ++c)
