@@ -956,15 +956,15 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
   const long long b0 = tile * TB;
   const int tbv = (int)min((long long)TB, nb - b0);
   const double2* src = rz + inst * rzs + b0;
-  // compile-time tile: one element per thread, whose weights are computed
-  // (kweights, the values the table holds) while the tile's loads are in
-  // flight, instead of read from the table after the forward transform
+  // compile-time tile: one element per thread, whose weights are read from
+  // the table with the tile's loads (the transform then hides their
+  // latency), instead of after the forward transform
   KW wpre{};
   if constexpr (TBC > 0) {
     const int x = xt.div(threadIdx.x), b = (int)threadIdx.x - x * TB;
     const unsigned bb = (unsigned)min(b0 + b, nb - 1);
     const int wy = (int)fdiv(bb, fnzh);
-    wpre = kweights(g, fp, x, wy, (int)bb - wy * g.nzh);
+    wpre = x_kw(g, fp, x, wy, (int)bb - wy * g.nzh, bb);
   }
   for (int i0 = threadIdx.x; i0 < T; i0 += LU * blockDim.x) {
     double2 v[LU];
