@@ -139,6 +139,7 @@ RedDev red_of(cmm_plan* p) { return RedDev{p->partials, p->pstride, p->scal}; }
 
 // totals of the partials the previous kernel stored (see k_reduce_partials)
 int launch_reduce(cmm_plan* p, long long nblocks, int nq, unsigned maxmask, int slot, unsigned skipmask = 0u) {
+  if ((long long)nq * nblocks > p->pstride) return fail(CMM_E_ARG, "partials beyond the per-instance stride");
   hipLaunchKernelGGL(k_reduce_partials, dim3(p->nb, nq), dim3(NT), 0, p->stream, red_of(p), (int)nblocks, maxmask,
                      slot, p->active, skipmask);
   hipError_t e_ = hipGetLastError();
@@ -662,8 +663,9 @@ static int xpass_tbc(const cmm_plan* p, int tb) {
   return (tb == 2 || tb == 4) ? tb : 0;
 }
 
+// returns the block size (its waves each store one partial per energy)
 template <int NF, bool MFA>
-static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, bool full) {
+static unsigned launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, bool full) {
   constexpr int NC = XFields<NF>::nc;
   const int T = p->nx * tb;
   const unsigned nt = xpass_threads(T, NC);
@@ -682,6 +684,7 @@ static void launch_xweights_nf(cmm_plan* p, int tb, long long ntiles, int per, b
                      p->rz, p->nhalf, p->F, (long long)p->fcap * p->nhalf, p->nhalf, p->g, p->fp, tb,
                      make_fastdiv((unsigned)tb), make_fastdiv((unsigned)(NC * tb)), make_fastdiv((unsigned)p->nzh), per,
                      ntiles, p->active, red_of(p), full ? 1 : 0);
+  return nt;
 }
 
 // Energies reduced here: the MFA energy into S_FMFA; with a linear WDA its
@@ -693,22 +696,21 @@ static int launch_xweights(cmm_plan* p, bool full = false) {
   const long long ntiles = (nb + tb - 1) / tb;
   const int per = (int)((ntiles + 7) / 8);
   const bool mfa = p->fp.kmfa != nullptr;
+  unsigned nt;
   if (nfx == 12) {
-    if (mfa) launch_xweights_nf<12, true>(p, tb, ntiles, per, full);
-    else launch_xweights_nf<12, false>(p, tb, ntiles, per, full);
+    nt = mfa ? launch_xweights_nf<12, true>(p, tb, ntiles, per, full) : launch_xweights_nf<12, false>(p, tb, ntiles, per, full);
   } else if (nfx == 5) {
-    if (mfa) launch_xweights_nf<5, true>(p, tb, ntiles, per, full);
-    else launch_xweights_nf<5, false>(p, tb, ntiles, per, full);
+    nt = mfa ? launch_xweights_nf<5, true>(p, tb, ntiles, per, full) : launch_xweights_nf<5, false>(p, tb, ntiles, per, full);
   } else {
-    if (mfa) launch_xweights_nf<6, true>(p, tb, ntiles, per, full);
-    else launch_xweights_nf<6, false>(p, tb, ntiles, per, full);
+    nt = mfa ? launch_xweights_nf<6, true>(p, tb, ntiles, per, full) : launch_xweights_nf<6, false>(p, tb, ntiles, per, full);
   }
   LAUNCHCHK("k_xweights");
+  const long long nparts = ntiles * (long long)((nt + 63) / 64);   // one partial per wave
   if (mfa) {
-    const int rc = launch_reduce(p, ntiles, 1, 0u, S_FMFA);
+    const int rc = launch_reduce(p, nparts, 1, 0u, S_FMFA);
     if (rc) return rc;
   }
-  return nfx == 5 ? launch_reduce(p, ntiles, 2, 0u, S_PHI, 1u) : CMM_OK;
+  return nfx == 5 ? launch_reduce(p, nparts, 2, 0u, S_PHI, 1u) : CMM_OK;
 }
 
 template <int NF, bool MFA>
@@ -858,7 +860,8 @@ static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
     const int P = p->pphi;
     if (wl ? launch_zphi128<5>(p, store) : launch_zphi128<6>(p, store)) {
       LAUNCHCHK("k_zphi128");
-      return reduce ? launch_reduce(p, zblocks(p, P), 3, 1u << 2, S_PHI, wl ? 2u : 0u) : CMM_OK;
+      // one partial per wave (wave_reduce_store)
+      return reduce ? launch_reduce(p, zblocks(p, P) * (NT / 64), 3, 1u << 2, S_PHI, wl ? 2u : 0u) : CMM_OK;
     }
   }
   const int P = zphi_p(p);
@@ -873,7 +876,7 @@ static int launch_zphi(cmm_plan* p, int store, bool reduce = true) {
     hipLaunchKernelGGL(k_zphi<6>, grid, dim3(NT), zlds(p, P, 6), p->stream, p->F, (long long)p->fcap * p->nhalf,
                        p->nhalf, p->g, p->fp, ztile(p, P, 6), store, p->active, red_of(p));
   LAUNCHCHK("k_zphi");
-  return reduce ? launch_reduce(p, grid.x, 3, 1u << 2, S_PHI, wl ? 2u : 0u) : CMM_OK;
+  return reduce ? launch_reduce(p, (long long)grid.x * (NT / 64), 3, 1u << 2, S_PHI, wl ? 2u : 0u) : CMM_OK;
 }
 
 // back-convolution of the nf derivative fields in F: inverse y, the x pass

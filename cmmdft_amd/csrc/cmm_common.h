@@ -422,20 +422,51 @@ __device__ __forceinline__ void load_twiddles(double2* tws, const Fft1D& p) {
 }
 
 // ---------------------------------------------------------------------------
-// Deterministic block + grid reductions.  Each block reduces NQ doubles
-// (op 0 = sum, 1 = max), writes them to partials[q*nblocks + bid]; the last
-// block to arrive (global counter) sums the partials in index order and writes
-// out[q].  Order of summation is fixed, so results are bitwise reproducible.
+// Deterministic reductions.  A producer kernel stores partials -- one per
+// block (block_reduce + grid_reduce_store: partials[q nblocks + bid]) or one
+// per wave (wave_reduce_store) -- and k_reduce_partials, launched after it,
+// sums (or maxes) them in a fixed order.  Every tree is fixed, so results
+// are bitwise reproducible.
 // ---------------------------------------------------------------------------
+// Cross-lane move of a double by a DPP control (both 32-bit halves): lanes
+// whose row is outside ROWMASK get idv.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double v, double idv) {
+  const long long b = __double_as_longlong(v), ib = __double_as_longlong(idv);
+  const int lo = __builtin_amdgcn_update_dpp((int)ib, (int)b, CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ib >> 32), (int)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum (max) over the 64 lanes of a wave, every lane active, in registers
+// (DPP moves, no LDS round trips): pairs and quads by quad permutes, half
+// rows and rows by mirrors, then rows 1 and 3 take the ends of rows 0 and 2
+// (row_bcast:15) and rows 2 and 3 the end of row 1 (row_bcast:31), which
+// leaves the total in lane 63; every lane returns it.  A fixed tree, so the
+// results are reproducible.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_f64<0xB1, 0xF>(v, 0.0);    // quad_perm [1, 0, 3, 2]
+  v += dpp_f64<0x4E, 0xF>(v, 0.0);    // quad_perm [2, 3, 0, 1]
+  v += dpp_f64<0x141, 0xF>(v, 0.0);   // row_half_mirror
+  v += dpp_f64<0x140, 0xF>(v, 0.0);   // row_mirror
+  v += dpp_f64<0x142, 0xA>(v, 0.0);   // row_bcast:15 into rows 1, 3
+  v += dpp_f64<0x143, 0xC>(v, 0.0);   // row_bcast:31 into rows 2, 3
+  return lane63(v);
 }
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
+  v = fmax(v, dpp_f64<0xB1, 0xF>(v, -INFINITY));
+  v = fmax(v, dpp_f64<0x4E, 0xF>(v, -INFINITY));
+  v = fmax(v, dpp_f64<0x141, 0xF>(v, -INFINITY));
+  v = fmax(v, dpp_f64<0x140, 0xF>(v, -INFINITY));
+  v = fmax(v, dpp_f64<0x142, 0xA>(v, -INFINITY));
+  v = fmax(v, dpp_f64<0x143, 0xC>(v, -INFINITY));
+  return lane63(v);
 }
 
 // LDS scratch of a block reduction: 16 wave slots (up to 1024 threads) for
@@ -463,6 +494,22 @@ __device__ void block_reduce(double (&v)[NQ], const int (&ops)[NQ], double* sh /
     }
   }
   __syncthreads();
+}
+
+// Each wave's reduced values (op 0 = sum, 1 = max) stored as partials
+// [q nparts + bid nw + wave], nparts = nblocks nw (nw waves per block): no
+// block-level combine, so no barriers in the producer; k_reduce_partials
+// sums or maxes the nparts partials of each quantity in a fixed order.
+template <int NQ>
+__device__ __forceinline__ void wave_reduce_store(const double (&v)[NQ], const int (&ops)[NQ], double* partials,
+                                                  int nblocks, int bid) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  const long long nparts = (long long)nblocks * nw;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const double x = ops[q] ? wave_max(v[q]) : wave_sum(v[q]);
+    if (lane == 0) partials[q * nparts + (long long)bid * nw + wid] = x;
+  }
 }
 
 // Called by every block with its block-reduced values valid in thread 0:

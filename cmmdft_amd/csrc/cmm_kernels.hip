@@ -1007,11 +1007,9 @@ k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ 
       B[x * SR + c * TB + b] = cscale(apply_w(g, w, x, wy, wz, XF::field(c), a, -1), g.inv_npts);
   }
   if (E) {
-    __shared__ double sh[CMM_RED_SH];
-    double acc[2] = {emfa, ewl * fp.wl2 * g.inv_npts};
+    const double acc[2] = {emfa, ewl * fp.wl2 * g.inv_npts};
     const int ops[2] = {0, 0};
-    block_reduce<2>(acc, ops, sh);
-    grid_reduce_store<2>(acc, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
+    wave_reduce_store<2>(acc, ops, red.partials + (long long)inst * red.pstride, (int)ntiles, (int)tile);
   }
   __syncthreads();
   x_fft<1, TBC, NC * TBC, SRC>(B, S6, fS6, SR, g, s_tw);
@@ -1466,7 +1464,6 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[CMM_MAX_N];
   load_twiddles(s_tw, g.fz);
-  __shared__ double sh[CMM_RED_SH];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const int n = g.nz;
@@ -1502,8 +1499,7 @@ __global__ void __launch_bounds__(NT) k_zphi(double2* __restrict__ F, long long 
     z_store_tile<VZ, NF>(sm, F + inst * Fs, Ffs, p0, t, npen, n, g.nzh, fBn, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
-  block_reduce<3>(acc, ops, sh);
-  grid_reduce_store<3>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
+  wave_reduce_store<3>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 // k_zphi for nz = 128 with the tile fixed at compile time (P pencils): the
@@ -1518,7 +1514,6 @@ __global__ void __launch_bounds__(NT) k_zphi128(double2* __restrict__ F, long lo
   extern __shared__ double2 sm[];
   __shared__ double2 s_tw[128];
   load_twiddles(s_tw, g.fz);
-  __shared__ double sh[CMM_RED_SH];
   const int inst = blockIdx.y;
   if (active && !active[inst]) return;
   const long long npen = (long long)g.nx * g.ny;
@@ -1552,8 +1547,7 @@ __global__ void __launch_bounds__(NT) k_zphi128(double2* __restrict__ F, long lo
     else z_store_cols_c<VZ, TC, false>(sm, F + inst * Fs, Ffs, p0, npen, g.kz.k, g.kz.nyq);
   }
   const int ops[3] = {0, 0, 1};
-  block_reduce<3>(acc, ops, sh);
-  grid_reduce_store<3>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
+  wave_reduce_store<3>(acc, ops, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
 // Pointwise free-energy densities of given weighted densities: kT Phi from
