@@ -927,9 +927,9 @@ __host__ __device__ constexpr int x_row(int n) { return (n % 8 == 0) ? n + 4 : n
 // |rho^(k)|^2 K(k) over the full spectrum (K Hermitian, so the sum is real):
 // on the half spectrum the planes kz = 0 and kz = Nyquist count once, the
 // others twice.  NF = 5 (linear WDA): also sum_x f_ex(rho_bar) =
-// (2A/N) sum_half |rho^|^2 w_D^2 with the same plane weights.  Block partials
-// (quantities: the MFA energy, then the WDA energy) are indexed by tile
-// (ntiles of them).
+// (2A/N) sum_half |rho^|^2 w_D^2 with the same plane weights.  Partials, one
+// per wave (quantities: the MFA energy, then the WDA energy), are indexed by
+// tile (ntiles of them, wave_reduce_store).
 template <int NF, bool MFA, int OCC, int TBC = 0>
 __global__ void __launch_bounds__(NTX) __attribute__((amdgpu_waves_per_eu(OCC)))
 k_xweights(const double2* __restrict__ rz, long long rzs, double2* __restrict__ F, long long Fs, long long Ffs,
@@ -1863,8 +1863,9 @@ __global__ void __launch_bounds__(NT) k_zproduce(GridDev g, ElDev el, ZTile t, c
   grid_reduce_store<7>(acc, red.partials + (long long)inst * red.pstride, gridDim.x, blockIdx.x);
 }
 
-// Per-instance totals of the block partials a producer kernel stored
-// (grid_reduce_store): out[q] = sum (or max, bit q of maxmask) over the
+// Per-instance totals of the partials a producer kernel stored, one per
+// block (grid_reduce_store) or per wave (wave_reduce_store; nblocks counts
+// them): out[q] = sum (or max, bit q of maxmask) over the
 // nblocks partials of quantity q, in a fixed order (lane-strided, then the
 // wavefront tree, then the waves in order).  One block per (instance,
 // quantity); each thread issues its loads in groups of 8 before summing them
